@@ -1,0 +1,106 @@
+/* planes_hip.h -- C ABI of the MI355X fitness-evaluation library (libplanes_hip.so).
+ *
+ * The reference keeps its GPU path behind seven C++ entry points (src/sim.h:6-13).
+ * This header is the plain-C layer underneath their drop-in replacements
+ * (include/planes_sim.h): plain pointers, sizes and int status codes, no C++ types,
+ * so it binds from ctypes/cffi/cgo/JNI alike.  Every entry point returns 0 on success
+ * and a non-zero code (a hipError_t value, or -1 for an argument error) otherwise;
+ * pr_last_error() gives the message of the calling thread's last failure.
+ *
+ * Pointers named d_* are device (HBM) pointers; everything else is host memory.
+ * One context = one HIP stream; calls on one context must come from one thread at a
+ * time (the reference's single-thread, one-stream model, sim.cu:214-248).
+ */
+#ifndef PLANES_HIP_H
+#define PLANES_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pr_ctx pr_ctx;
+
+/* Scenario preset and flight settings.  The reference hard-codes them as #defines
+ * (common.h:389-421, 470-478; sim.cu:241): preset 0 with max_steps 7200, chunk 600. */
+typedef struct pr_sim_params {
+    int preset;          /* 0: reference hash preset (common.h:389-421); 1: parametric config preset */
+    float gust_rms_max;  /* preset 1: gust field 3-D RMS ~ U(0, gust_rms_max) [m/s]   */
+    float warp_max;      /* preset 1: print warp pitch/roll ~ U(-warp_max, warp_max) [rad] */
+    float wind_sd;       /* preset 1: steady wind ~ N(0, wind_sd) per axis [m/s]       */
+    int max_steps;       /* flight cap in 600 Hz steps (SIM_MAX_STEPS, common.h:471)    */
+    int chunk;           /* steps per launch; the clock restarts at step0*dt (sim.cu:113,241) */
+    float spread;        /* showcase launch grid spacing [m], 0 for evaluation (sim.cu:71-78) */
+} pr_sim_params;
+
+typedef struct pr_stats {
+    unsigned long long flight_steps; /* integrated 600 Hz steps of the last pr_fly (= plane-flight-s * 600) */
+    unsigned long long panel_steps;  /* sum over those steps of the plane's triangle count */
+    double fly_ms;                   /* GPU time of the flight-kernel launches (HIP events on the context stream) */
+    int fly_launches;
+    double prep_ms;                  /* GPU time of the last pr_prepare */
+} pr_stats;
+
+/* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).
+ * Capacities: genomes per prepare, packed panels (sum of triangle counts), scenarios,
+ * flights per pr_fly, and the largest mesh resolution (ch, sp) any genome will use
+ * (0 = the reference's CH=5, SP=8). */
+int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scenarios, int max_flights,
+              int max_ch, int max_sp);
+void pr_destroy(pr_ctx* ctx);
+const char* pr_last_error(void);
+
+/* Replaces gpu_prepare (sim.h:11, sim.cu:214-218; k_prepare sim.cu:25-49): meshes n
+ * genomes (d_genomes [n*35] floats, the Genome byte layout), computes mass properties,
+ * the print-volume check and packed panel records.  d_res: optional [n*2] (ch, sp)
+ * per genome (null = reference resolution).  Asynchronous on the context stream. */
+int pr_prepare(pr_ctx* ctx, const float* d_genomes, const int* d_res, int n);
+
+/* Scenario descriptors for n seeds (d_seeds device int32) -- scenario_desc, common.h:389. */
+int pr_scenarios(pr_ctx* ctx, const int* d_seeds, int n, const pr_sim_params* p);
+
+/* Flies every flight and writes d_fitness / d_steps (integrated steps) per flight.
+ * layout 0: flight f = genome f / n_scen under scenario f % n_scen (n_genomes*n_scen
+ *           flights; the GA's seed batch and the holdout batch, sim.cu:434-456);
+ * layout 1: flight f = genome f under scenario f (n_genomes == n_scen flights; gpu_sim
+ *           with vary_scenarios=1, sim.cu:64).
+ * Uses the genomes of the last pr_prepare and the scenarios of the last pr_scenarios.
+ * Returns when d_fitness is complete (like gpu_sim, sim.cu:238-248). */
+int pr_fly(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+           int* d_steps);
+
+/* Only the flight set-up of pr_fly: per-flight constants and launch states (sim.cu:63-109),
+ * fitness -100 for unprintable genomes.  pr_read_states then returns the launch states. */
+int pr_init_flights(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                    int* d_steps);
+
+/* Tail-aware robust score per genome (robust_scores, sim.cu:340-357):
+ * 0.65 mean + 0.25 CVaR(worst quarter) + 0.10 worst, from d_samples [pop*nseeds]. */
+int pr_scores(pr_ctx* ctx, const float* d_samples, int pop, int nseeds, float* d_scores);
+
+int pr_stats_get(pr_ctx* ctx, pr_stats* out);
+
+/* Flight-kernel mapping per triangle-count bucket: flights whose count is <= nt_max[i]
+ * (and above the previous bucket's) are flown by groups of L[i] lanes holding P[i]
+ * panels per lane (L*P >= nt_max).  Available (L, P) pairs: pr_fly_variants. */
+int pr_set_buckets(pr_ctx* ctx, int n, const int* nt_max, const int* L, const int* P);
+int pr_fly_variants(int* n, int* L, int* P, int cap);
+
+/* Readback of intermediate results (tests and tools). */
+int pr_read_prepared(pr_ctx* ctx, int n, float* mass20, int* nt, int* nv, int* off);
+int pr_read_panels(pr_ctx* ctx, long long first, int count, float* out7); /* cx cy cz A nx ny nz rows */
+int pr_read_scenarios(pr_ctx* ctx, int n, float* out26);
+int pr_read_states(pr_ctx* ctx, int first, int n, float* out16);       /* FlightState rows, 16 words */
+
+/* Device memory helpers so that hosts without a GPU array library can drive the API. */
+int pr_dev_alloc(void** p, size_t bytes);
+int pr_dev_free(void* p);
+int pr_h2d(void* d_dst, const void* src, size_t bytes);
+int pr_d2h(void* dst, const void* d_src, size_t bytes);
+int pr_device_sync(void);
+int pr_device_name(char* buf, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLANES_HIP_H */

@@ -1,0 +1,389 @@
+// capi.hip -- host runtime behind include/planes_hip.h.
+//
+// One pr_ctx owns the HBM buffers of one evaluation slot (the reference's PanelBuf,
+// sim.cu:15-21, without its MAX_T padding) and one HIP stream.  pr_fly schedules the
+// flights as 1-s chunks: each chunk launches the persistent flight kernel once per
+// triangle-count bucket over that bucket's work list; survivors are appended to the
+// next chunk's list by the kernel itself, so the host never synchronises between chunks.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/planes_hip.h"
+#include "pr_kernels.h"
+
+using namespace pr;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* what) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s (code %d: %s)", what, code,
+             code > 0 ? hipGetErrorString((hipError_t)code) : "invalid argument");
+    g_err = buf;
+    return code ? code : -1;
+}
+#define HCK(x)                                                        \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) return fail((int)e_, #x);               \
+    } while (0)
+#define ARG(cond, msg)                          \
+    do {                                        \
+        if (!(cond)) { g_err = msg; return -1; } \
+    } while (0)
+
+struct Bucket { int nt_max, L, P; };
+
+struct pr_ctx {
+    hipStream_t stream = nullptr;
+    int cap_genomes = 0, cap_scen = 0, cap_flights = 0;
+    long long cap_panels = 0;
+    int max_ch = 5, max_sp = 8, nv_max = 0;
+    // genome side
+    int n_prepared = 0;
+    int *d_ntraw = nullptr, *d_off = nullptr, *d_nt = nullptr, *d_nv = nullptr;
+    Mass* d_mass = nullptr;
+    float* d_panel[7] = {};
+    void* d_scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    // scenarios
+    int n_scen = 0;
+    int* d_seeds = nullptr;
+    float* d_scen = nullptr;
+    // flights
+    FlightState* d_state = nullptr;
+    FlightConst* d_fc = nullptr;
+    int* d_lists[2] = {};
+    int* d_counts = nullptr;  // [2][NB] list counts, [NB] queue heads
+    int* d_bucket_nt = nullptr;
+    unsigned long long* d_stats = nullptr;
+    std::vector<Bucket> buckets;
+    std::vector<int> grid;    // persistent grid per bucket
+    hipEvent_t ev[2 * 256 + 2] = {};
+    int n_events = 0;
+    pr_stats stats = {};
+    hipEvent_t prep0 = nullptr, prep1 = nullptr;
+};
+
+static const int NB_MAX = 8;
+
+static int mesh_bounds(int ch, int sp, int* nv, int* nt) {
+    int nw = (2 * sp + 1) * (ch + 1), tw = 4 * sp * ch;
+    *nv = 2 * nw + 9 + 12 + 8 + 8;
+    *nt = 2 * tw + 8 + 6 + 6 + 4;
+    return 0;
+}
+
+static int default_buckets(pr_ctx* c) {
+    static const Bucket table[] = {{192, 8, 24}, {352, 16, 22}, {384, 16, 24}, {768, 32, 24},
+                                   {928, 32, 29}, {1536, 64, 24}, {2048, 64, 32}};
+    int nv, nt;
+    mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
+    c->buckets.clear();
+    for (const Bucket& b : table) {
+        c->buckets.push_back(b);
+        if (b.nt_max >= nt) break;
+    }
+    if (c->buckets.back().nt_max < nt) return fail(-1, "mesh resolution exceeds the largest flight-kernel bucket (2048 triangles)");
+    return 0;
+}
+
+static int upload_buckets(pr_ctx* c) {
+    int nb = (int)c->buckets.size();
+    std::vector<int> h(nb);
+    for (int i = 0; i < nb; i++) h[i] = c->buckets[i].nt_max;
+    HCK(hipMemcpyAsync(c->d_bucket_nt, h.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    HCK(hipStreamSynchronize(c->stream));
+    int dev = 0, ncu = 0;
+    HCK(hipGetDevice(&dev));
+    HCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    c->grid.assign(nb, 0);
+    for (int i = 0; i < nb; i++) {
+        int occ = 0;
+        HCK(fly_occupancy(c->buckets[i].L, c->buckets[i].P, &occ));
+        if (occ < 1) occ = 1;
+        c->grid[i] = occ * ncu;
+    }
+    return 0;
+}
+
+extern "C" {
+
+const char* pr_last_error(void) { return g_err.c_str(); }
+
+int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scenarios, int max_flights, int max_ch,
+              int max_sp) {
+    ARG(out, "pr_create: null output");
+    ARG(max_genomes > 0 && max_scenarios > 0 && max_flights > 0, "pr_create: capacities must be positive");
+    pr_ctx* c = new pr_ctx;
+    *out = nullptr;
+    c->max_ch = max_ch > 0 ? max_ch : 5;
+    c->max_sp = max_sp > 0 ? max_sp : 8;
+    int nv, nt;
+    mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
+    c->nv_max = nv;
+    if (max_panels <= 0) max_panels = (long long)max_genomes * nt;
+    c->cap_genomes = max_genomes;
+    c->cap_panels = max_panels;
+    c->cap_scen = max_scenarios;
+    c->cap_flights = max_flights;
+    int rc;
+    if ((rc = default_buckets(c))) { delete c; return rc; }
+#define ALLOC(p, bytes)                                              \
+    do {                                                             \
+        hipError_t e_ = hipMalloc((void**)&(p), (bytes));           \
+        if (e_ != hipSuccess) { pr_destroy(c); return fail((int)e_, "hipMalloc " #p); } \
+    } while (0)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return fail(-1, "stream"); }
+    ALLOC(c->d_ntraw, sizeof(int) * (max_genomes + 1));
+    ALLOC(c->d_off, sizeof(int) * (max_genomes + 1));
+    ALLOC(c->d_nt, sizeof(int) * max_genomes);
+    ALLOC(c->d_nv, sizeof(int) * max_genomes);
+    ALLOC(c->d_mass, sizeof(Mass) * max_genomes);
+    for (int k = 0; k < 7; k++) ALLOC(c->d_panel[k], sizeof(float) * max_panels);
+    ALLOC(c->d_seeds, sizeof(int) * max_scenarios);
+    ALLOC(c->d_scen, sizeof(float) * NSF * max_scenarios);
+    ALLOC(c->d_state, sizeof(FlightState) * max_flights);
+    ALLOC(c->d_fc, sizeof(FlightConst) * max_flights);
+    for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
+    ALLOC(c->d_counts, sizeof(int) * 3 * NB_MAX);
+    ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
+    ALLOC(c->d_stats, sizeof(unsigned long long) * 2);
+    hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream);
+    ALLOC(c->d_scan_tmp, c->scan_tmp_bytes + 16);
+#undef ALLOC
+    for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
+    if (hipEventCreate(&c->prep0) != hipSuccess || hipEventCreate(&c->prep1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
+    if ((rc = upload_buckets(c))) { pr_destroy(c); return rc; }
+    *out = c;
+    return 0;
+}
+
+void pr_destroy(pr_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->d_ntraw); hipFree(c->d_off); hipFree(c->d_nt); hipFree(c->d_nv); hipFree(c->d_mass);
+    for (int k = 0; k < 7; k++) hipFree(c->d_panel[k]);
+    hipFree(c->d_seeds); hipFree(c->d_scen); hipFree(c->d_state); hipFree(c->d_fc);
+    hipFree(c->d_lists[0]); hipFree(c->d_lists[1]); hipFree(c->d_counts); hipFree(c->d_bucket_nt);
+    hipFree(c->d_stats); hipFree(c->d_scan_tmp);
+    for (auto& e : c->ev) if (e) hipEventDestroy(e);
+    if (c->prep0) hipEventDestroy(c->prep0);
+    if (c->prep1) hipEventDestroy(c->prep1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pr_set_buckets(pr_ctx* c, int n, const int* nt_max, const int* L, const int* P) {
+    ARG(c && n > 0 && n <= NB_MAX, "pr_set_buckets: 1..8 buckets");
+    std::vector<Bucket> b(n);
+    for (int i = 0; i < n; i++) {
+        b[i] = {nt_max[i], L[i], P[i]};
+        ARG(L[i] * P[i] >= nt_max[i], "pr_set_buckets: L*P below the bucket's triangle count");
+        ARG(i == 0 || nt_max[i] > nt_max[i - 1], "pr_set_buckets: nt_max must increase");
+        int occ = 0;
+        ARG(fly_occupancy(L[i], P[i], &occ) == hipSuccess, "pr_set_buckets: (L, P) is not a compiled variant");
+    }
+    int nv, nt;
+    mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
+    ARG(b.back().nt_max >= nt, "pr_set_buckets: last bucket below the context's largest triangle count");
+    c->buckets = b;
+    return upload_buckets(c);
+}
+
+int pr_fly_variants(int* n, int* L, int* P, int cap) {
+    int k = fly_variant_count();
+    *n = k;
+    for (int i = 0; i < k && i < cap; i++) fly_variant_info(i, L + i, P + i);
+    return 0;
+}
+
+int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
+    ARG(c && d_genomes, "pr_prepare: null argument");
+    ARG(n > 0 && n <= c->cap_genomes, "pr_prepare: n outside 1..max_genomes");
+    HCK(hipEventRecord(c->prep0, c->stream));
+    HCK(launch_layout(d_genomes, d_res, 5, 8, n, c->d_ntraw, c->stream));
+    HCK(launch_zero(c->d_ntraw + n, 1, c->stream));
+    size_t tb = c->scan_tmp_bytes;
+    HCK(hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, tb, c->d_ntraw, c->d_off, n + 1, c->stream));
+    int nv, nt;
+    mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
+    if ((long long)n * nt > c->cap_panels) {  // the packed total may still fit: check it exactly
+        int total = 0;
+        HCK(hipMemcpyAsync(&total, c->d_off + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HCK(hipStreamSynchronize(c->stream));
+        ARG(total <= c->cap_panels, "pr_prepare: packed panel total exceeds max_panels");
+    }
+    PrepareArgs a;
+    a.genomes = d_genomes; a.res = d_res; a.def_ch = 5; a.def_sp = 8; a.nv_max = c->nv_max;
+    a.off = c->d_off; a.mass = c->d_mass; a.nt = c->d_nt; a.nv = c->d_nv;
+    a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
+    a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
+    HCK(launch_prepare(a, n, c->stream));
+    HCK(hipEventRecord(c->prep1, c->stream));
+    c->n_prepared = n;
+    return 0;
+}
+
+int pr_scenarios(pr_ctx* c, const int* d_seeds, int n, const pr_sim_params* p) {
+    ARG(c && d_seeds && p, "pr_scenarios: null argument");
+    ARG(n > 0 && n <= c->cap_scen, "pr_scenarios: n outside 1..max_scenarios");
+    ARG(p->preset == 0 || p->preset == 1, "pr_scenarios: preset must be 0 or 1");
+    HCK(hipMemcpyAsync(c->d_seeds, d_seeds, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    ScenarioParams sp;
+    sp.preset = p->preset; sp.gust_rms_max = p->gust_rms_max; sp.warp_max = p->warp_max; sp.wind_sd = p->wind_sd;
+    HCK(launch_scenarios(c->d_seeds, n, sp, c->d_scen, c->stream));
+    c->n_scen = n;
+    return 0;
+}
+
+static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                        int* d_steps, long long* nf_out) {
+    ARG(c && p && d_fitness && d_steps, "pr_fly: null argument");
+    ARG(layout == 0 || layout == 1, "pr_fly: layout must be 0 or 1");
+    ARG(n_genomes > 0 && n_genomes <= c->n_prepared, "pr_fly: n_genomes exceeds the prepared genomes");
+    ARG(n_scen > 0 && n_scen <= c->n_scen, "pr_fly: n_scen exceeds the prepared scenarios");
+    ARG(layout == 0 || n_genomes == n_scen, "pr_fly: layout 1 needs n_genomes == n_scen");
+    ARG(p->max_steps > 0 && p->chunk > 0, "pr_fly: max_steps and chunk must be positive");
+    long long nf = layout == 0 ? (long long)n_genomes * n_scen : n_genomes;
+    ARG(nf <= c->cap_flights, "pr_fly: flights exceed max_flights");
+    const int NB = (int)c->buckets.size();
+    HCK(launch_zero(c->d_counts, 3 * NB_MAX, c->stream));
+    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 4, c->stream));
+    QueueArgs q;
+    q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
+    q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
+    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.list = c->d_lists[0];
+    q.list_stride = c->cap_flights; q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
+    HCK(launch_queue(q, c->stream));
+    *nf_out = nf;
+    return 0;
+}
+
+int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                    int* d_steps) {
+    long long nf;
+    int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
+    if (rc) return rc;
+    HCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness, int* d_steps) {
+    long long nf;
+    int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
+    if (rc) return rc;
+    const int NB = (int)c->buckets.size();
+    int* cnt[2] = {c->d_counts, c->d_counts + NB_MAX};
+    int* head = c->d_counts + 2 * NB_MAX;
+    FlyArgs a;
+    a.nt = c->d_nt; a.off = c->d_off; a.fc = c->d_fc;
+    a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
+    a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
+    a.state = c->d_state; a.fitness = d_fitness; a.steps = d_steps;
+    a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
+    const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
+    int ne = 0;
+    for (int ck = 0; ck < n_chunks; ck++) {
+        const int in = ck & 1, outb = in ^ 1;
+        HCK(launch_zero(cnt[outb], NB_MAX, c->stream));
+        HCK(launch_zero(head, NB_MAX, c->stream));
+        for (int b = 0; b < NB; b++) {
+            a.list_in = c->d_lists[in] + (size_t)b * c->cap_flights;
+            a.count_in = cnt[in] + b;
+            a.list_out = c->d_lists[outb] + (size_t)b * c->cap_flights;
+            a.count_out = cnt[outb] + b;
+            a.head = head + b;
+            const bool timed = ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
+            if (timed) HCK(hipEventRecord(c->ev[ne], c->stream));
+            HCK(launch_fly(c->buckets[b].L, c->buckets[b].P, a, c->grid[b], c->stream));
+            if (timed) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); ne += 2; }
+        }
+    }
+    HCK(hipStreamSynchronize(c->stream));
+    double ms = 0;
+    for (int k = 0; k < ne; k += 2) {
+        float t = 0;
+        HCK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+        ms += t;
+    }
+    unsigned long long st[2];
+    HCK(hipMemcpy(st, c->d_stats, sizeof st, hipMemcpyDeviceToHost));
+    c->stats.flight_steps = st[0];
+    c->stats.panel_steps = st[1];
+    c->stats.fly_ms = ms;
+    c->stats.fly_launches = ne / 2;
+    float pm = 0;
+    if (hipEventElapsedTime(&pm, c->prep0, c->prep1) == hipSuccess) c->stats.prep_ms = pm;
+    return 0;
+}
+
+int pr_scores(pr_ctx* c, const float* d_samples, int pop, int nseeds, float* d_scores) {
+    ARG(c && d_samples && d_scores, "pr_scores: null argument");
+    ARG(pop > 0 && nseeds > 0 && nseeds <= 256, "pr_scores: pop > 0 and 1 <= nseeds <= 256");
+    HCK(launch_scores(d_samples, pop, nseeds, d_scores, c->stream));
+    HCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pr_stats_get(pr_ctx* c, pr_stats* out) {
+    ARG(c && out, "pr_stats_get: null argument");
+    *out = c->stats;
+    return 0;
+}
+
+int pr_read_prepared(pr_ctx* c, int n, float* mass20, int* nt, int* nv, int* off) {
+    ARG(c && n >= 0 && n <= c->n_prepared, "pr_read_prepared: n exceeds the prepared genomes");
+    HCK(hipStreamSynchronize(c->stream));
+    if (mass20) HCK(hipMemcpy(mass20, c->d_mass, sizeof(Mass) * n, hipMemcpyDeviceToHost));
+    if (nt) HCK(hipMemcpy(nt, c->d_nt, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (nv) HCK(hipMemcpy(nv, c->d_nv, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (off) HCK(hipMemcpy(off, c->d_off, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pr_read_panels(pr_ctx* c, long long first, int count, float* out7) {
+    ARG(c && out7 && first >= 0 && count >= 0 && first + count <= c->cap_panels, "pr_read_panels: range");
+    HCK(hipStreamSynchronize(c->stream));
+    std::vector<float> tmp(count);
+    for (int k = 0; k < 7; k++) {
+        HCK(hipMemcpy(tmp.data(), c->d_panel[k] + first, sizeof(float) * count, hipMemcpyDeviceToHost));
+        for (int i = 0; i < count; i++) out7[7 * i + k] = tmp[i];
+    }
+    return 0;
+}
+
+int pr_read_scenarios(pr_ctx* c, int n, float* out26) {
+    ARG(c && out26 && n >= 0 && n <= c->n_scen, "pr_read_scenarios: n exceeds the prepared scenarios");
+    HCK(hipStreamSynchronize(c->stream));
+    HCK(hipMemcpy(out26, c->d_scen, sizeof(float) * NSF * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pr_read_states(pr_ctx* c, int first, int n, float* out16) {
+    ARG(c && out16 && first >= 0 && n >= 0 && first + n <= c->cap_flights, "pr_read_states: range");
+    HCK(hipStreamSynchronize(c->stream));
+    HCK(hipMemcpy(out16, c->d_state + first, sizeof(FlightState) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pr_dev_alloc(void** p, size_t bytes) { HCK(hipMalloc(p, bytes)); return 0; }
+int pr_dev_free(void* p) { HCK(hipFree(p)); return 0; }
+int pr_h2d(void* d, const void* h, size_t bytes) { HCK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice)); return 0; }
+int pr_d2h(void* h, const void* d, size_t bytes) { HCK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost)); return 0; }
+int pr_device_sync(void) { HCK(hipDeviceSynchronize()); return 0; }
+int pr_device_name(char* buf, int cap) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    HCK(hipGetDevice(&dev));
+    HCK(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,595 @@
+// kernels.hip -- gfx950 kernels of the fitness-evaluation path.
+//
+//   k_layout     genome -> closed-form triangle count (for the exact panel prefix ranges)
+//   k_prepare    one wave64 per genome: lane-parallel meshing into LDS, print-volume
+//                check, mass properties in the reference's float summation order,
+//                packed SoA panel records at exact prefix offsets (no MAX_T padding)
+//   k_scenarios  scenario descriptors per seed (reference or config preset)
+//   k_init       (genome, scenario) flights -> constants, launch state, per-bucket work
+//                lists; unprintable -> -100
+//   k_fly<L,P>   persistent flight kernel: each group of L lanes flies one flight with
+//                P panels per lane held in VGPRs; groups refill from the work list
+//   k_scores     tail-aware robust score per genome
+#include <hip/hip_runtime.h>
+
+#include "pr_kernels.h"
+#include "pr_physics.h"
+
+namespace pr {
+
+// ------------------------------------------------------------------ cross-lane helpers
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over each aligned group of L lanes; every lane of the group receives the same bits
+// (each butterfly stage adds a symmetric partner, and IEEE addition commutes).
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+    if (L >= 2) v += dpp_mov<0xB1>(v);    // quad_perm [1,0,3,2]: lane ^ 1
+    if (L >= 4) v += dpp_mov<0x4E>(v);    // quad_perm [2,3,0,1]: lane ^ 2
+    if (L >= 8) v += dpp_mov<0x141>(v);   // row_half_mirror: quad 0 <-> quad 1 within 8
+    if (L >= 16) v += dpp_mov<0x140>(v);  // row_mirror: half-row 0 <-> 1 within 16
+    if (L >= 32) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));  // lane ^ 16
+    if (L >= 64) v += __shfl_xor(v, 32);
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ------------------------------------------------------------------ k_layout
+__global__ void k_layout(const float* __restrict__ genomes, const int* __restrict__ res, int def_ch, int def_sp,
+                         int n, int* __restrict__ nt_raw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g[NG];
+    for (int k = 0; k < NG; k++) g[k] = genomes[(size_t)i * NG + k];
+    int ch = res ? res[2 * i] : def_ch, sp = res ? res[2 * i + 1] : def_sp;
+    nt_raw[i] = layout_of(g, ch, sp).nt;
+}
+
+// ------------------------------------------------------------------ k_prepare
+// block = one wave64 = one genome.  LDS: vertices [nv_max] + 12 x 64 triangle terms.
+__global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
+    extern __shared__ float lds[];
+    __shared__ float g[NG];
+    const int gi = blockIdx.x, lane = threadIdx.x;
+    if (lane < NG) g[lane] = a.genomes[(size_t)gi * NG + lane];
+    __syncthreads();
+    const int ch = a.res ? a.res[2 * gi] : a.def_ch, sp = a.res ? a.res[2 * gi + 1] : a.def_sp;
+    const MeshLayout L = layout_of(g, ch, sp);
+    vec3* Vx = reinterpret_cast<vec3*>(lds);
+    float* term = lds + 3 * a.nv_max;  // [12][64]
+
+    // vertices, lane-parallel; extent by exact min/max reductions
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int v = lane; v < L.nv; v += 64) {
+        vec3 p = mesh_vertex(g, L, v);
+        Vx[v] = p;
+        lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+        hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+    }
+    bool fits = true;
+    for (int k = 0; k < 3; k++) {
+        float e;
+        {
+#pragma clang fp contract(off)
+            e = wave_max(hi[k]) - wave_min(lo[k]);
+        }
+        fits = fits && (e <= PR_PRINT_LIMIT);
+    }
+    __syncthreads();
+
+    // pass 1: mass and first moment; lanes 0..3 each own one serial accumulator chain
+    float acc = 0.0f;
+    for (int base = 0; base < L.nt; base += 64) {
+        int t = base + lane;
+        if (t < L.nt) {
+            int id[3];
+            tri_of(L, t, id);
+            float mi;
+            vec3 mct;
+            tri_mass_terms(Vx[id[0]], Vx[id[1]], Vx[id[2]], &mi, &mct);
+            term[0 * 64 + lane] = mi; term[1 * 64 + lane] = mct.x; term[2 * 64 + lane] = mct.y; term[3 * 64 + lane] = mct.z;
+        }
+        __syncthreads();
+        if (lane < 4) {
+            int cnt = min(64, L.nt - base);
+            for (int k = 0; k < cnt; k++) {
+#pragma clang fp contract(off)
+                acc = acc + term[lane * 64 + k];
+            }
+        }
+        __syncthreads();
+    }
+    Mass mp;
+    vec3 c;
+    float m, mn;
+    {
+#pragma clang fp contract(off)
+        m = __shfl(acc, 0);
+        c = V(__shfl(acc, 1), __shfl(acc, 2), __shfl(acc, 3));
+        float shell = m;
+        vec3 np = V(0.5f * g[G_CHORD], 0, 0);
+        mn = g[G_NOSE_MASS] * shell;
+        m += mn;
+        c = vadd(c, vmul(np, mn));
+        c = vmul(c, 1.0f / m);
+    }
+    // pass 2: inertia about c; lanes 0..5 own I[0..5]
+    float Ik = 0.0f;
+    for (int base = 0; base < L.nt; base += 64) {
+        int t = base + lane;
+        if (t < L.nt) {
+            int id[3];
+            tri_of(L, t, id);
+            float o[12];
+            tri_inertia_terms(Vx[id[0]], Vx[id[1]], Vx[id[2]], c, o);
+            for (int k = 0; k < 12; k++) term[k * 64 + lane] = o[k];
+        }
+        __syncthreads();
+        if (lane < 6) {
+            int cnt = min(64, L.nt - base);
+            const float* ta = term + lane * 64;
+            const float* tb = term + (lane + 6) * 64;
+            for (int k = 0; k < cnt; k++) {
+#pragma clang fp contract(off)
+                if (lane < 3) { Ik = Ik + ta[k]; Ik = Ik + tb[k]; }
+                else { Ik = Ik - ta[k]; Ik = Ik - tb[k]; }
+            }
+        }
+        __syncthreads();
+    }
+    {
+#pragma clang fp contract(off)
+        float I[6];
+        for (int k = 0; k < 6; k++) I[k] = __shfl(Ik, k);
+        vec3 r = vsub(V(0.5f * g[G_CHORD], 0, 0), c);
+        I[0] += mn * (r.y * r.y + r.z * r.z); I[1] += mn * (r.x * r.x + r.z * r.z);
+        I[2] += mn * (r.x * r.x + r.y * r.y); I[3] -= mn * r.x * r.y;
+        I[4] -= mn * r.x * r.z; I[5] -= mn * r.y * r.z;
+        const float reg = 1e-9f;
+        float xx = I[0] + reg, yy = I[1] + reg, zz = I[2] + reg;
+        invert_sym3(xx, yy, zz, I[3], I[4], I[5], false, mp.invI);
+        mp.I[0] = xx; mp.I[1] = yy; mp.I[2] = zz; mp.I[3] = I[3]; mp.I[4] = I[4]; mp.I[5] = I[5];
+        mp.mass = m;
+        mp.invmass = 1.0f / m;
+        mp.com[0] = c.x; mp.com[1] = c.y; mp.com[2] = c.z;
+    }
+    const int nt_eff = fits ? L.nt : 0;
+    if (lane < 20) reinterpret_cast<float*>(a.mass + gi)[lane] = reinterpret_cast<const float*>(&mp)[lane];
+    if (lane == 0) {
+        a.nt[gi] = nt_eff;
+        a.nv[gi] = L.nv;
+    }
+    // panel records relative to the shell COM (sim.cu:40-48), exact prefix range
+    const int off = a.off[gi];
+    for (int t = lane; t < nt_eff; t += 64) {
+        int id[3];
+        tri_of(L, t, id);
+        vec3 cen, nrm;
+        float area;
+        panel_of(Vx[id[0]], Vx[id[1]], Vx[id[2]], c, &cen, &area, &nrm);
+        size_t o = (size_t)off + t;
+        a.pcx[o] = cen.x; a.pcy[o] = cen.y; a.pcz[o] = cen.z; a.pA[o] = area;
+        a.pnx[o] = nrm.x; a.pny[o] = nrm.y; a.pnz[o] = nrm.z;
+    }
+}
+
+// ------------------------------------------------------------------ k_scenarios
+__global__ void k_scenarios(const int* __restrict__ seeds, int n, ScenarioParams sp, float* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d[NSF];
+    if (sp.preset == 0) scenario_reference(seeds[i], d);
+    else scenario_config(seeds[i], sp, d);
+    for (int k = 0; k < NSF; k++) out[(size_t)i * NSF + k] = d[k];
+}
+
+// ------------------------------------------------------------------ k_init
+// Flight f -> (genome, scenario) by layout.  Writes the flight's constants (printed mass,
+// scenario terms) and launch state (sim.cu:63-109), or fitness -100 for an unprintable
+// genome (sim.cu:35-36,109), and appends the flight to the work list of its genome's
+// triangle-count bucket with wave-aggregated atomics (lane order kept within a wave).
+__global__ void k_init(QueueArgs a) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int b = -1;
+    if (f < a.n_flights) {
+        const int gi = a.layout == 0 ? f / a.nseeds : f;
+        const int si = a.layout == 0 ? f - gi * a.nseeds : f;
+        const int nt = a.nt[gi];
+        FlightState st;
+        FlightConst fc;
+        float d[NSF];
+        for (int k = 0; k < NSF; k++) d[k] = a.scen[(size_t)si * NSF + k];
+        Mass m0, mp;
+        for (int k = 0; k < 20; k++) reinterpret_cast<float*>(&m0)[k] = reinterpret_cast<const float*>(a.mass + gi)[k];
+        vec3 shift;
+        printed_mass(m0, d, mp, shift);
+        launch_state(d, mp, shift, a.layout == 0 ? gi : f, a.spread, st);
+        fc.mass = mp.mass; fc.invmass = mp.invmass;
+        for (int k = 0; k < 6; k++) fc.I[k] = mp.I[k];
+        fc.iv[0] = mp.invI[0]; fc.iv[1] = mp.invI[1]; fc.iv[2] = mp.invI[2];
+        fc.iv[3] = mp.invI[4]; fc.iv[4] = mp.invI[5]; fc.iv[5] = mp.invI[8];
+        fc.cf = d[S_CF]; fc.half_rho = 0.5f * d[S_RHO];
+        fc.base[0] = d[S_BASE_X]; fc.base[1] = d[S_BASE_Y]; fc.base[2] = d[S_BASE_Z];
+        fc.gust = d[S_GUST]; fc.phase = (float)phase_of(a.seeds[si]);
+        fc.shift[0] = shift.x; fc.shift[1] = shift.y; fc.shift[2] = shift.z;
+        fc.warp_p = d[S_WARP_P]; fc.warp_r = d[S_WARP_R];
+        fc.genome = gi; fc.scen = si;
+        fc.pad[0] = fc.pad[1] = fc.pad[2] = 0.0f;
+        if (nt <= 0) {
+            st.done = 1; st.fitness = -100.0f;
+            a.fitness[f] = -100.0f;
+            a.steps[f] = 0;
+        } else {
+            b = 0;
+            while (b < a.n_buckets - 1 && nt > a.bucket_nt[b]) b++;
+        }
+        a.fc[f] = fc;
+        a.state[f] = st;
+    }
+    for (int k = 0; k < a.n_buckets; k++) {
+        unsigned long long m = __ballot(b == k);
+        if (m == 0) continue;
+        int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.count + k, __popcll(m));
+        base = __shfl(base, leader);
+        if (b == k) {
+            int rank = __popcll(m & ((1ull << lane) - 1ull));
+            a.list[(size_t)k * a.list_stride + base + rank] = f;
+        }
+    }
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------ k_fly
+// Per-lane copy of one group's flight.  All lanes of a group hold bitwise-identical
+// values (they compute the same expressions on the same inputs).
+struct Body {
+    float px, py, pz, qw, qx, qy, qz, vx, vy, vz, wx, wy, wz, t;
+    float mass, ks, I0, I1, I2, I3, I4, I5, v0, v1, v2, v4, v5, v8;  // ks = invmass * dt
+    float cf, bx, by, bz;
+    int step, end, fid, nt;
+};
+
+// wind terms (common.h:462-465): term k = weight * sin(a*p_axis + b*t + c*phase)
+__device__ __forceinline__ void wind_term(int k, int& axis, float& a, float& b, float& c, float& w, int& comp) {
+    switch (k) {
+        case 0: axis = 0; a = 0.9f; b = 1.3f; c = 1.0f; w = 0.55f; comp = 0; break;
+        case 1: axis = 2; a = 2.3f; b = 2.1f; c = 1.7f; w = 0.35f; comp = 0; break;
+        case 2: axis = 1; a = 4.1f; b = 3.3f; c = 0.4f; w = 0.25f; comp = 0; break;
+        case 3: axis = 0; a = 1.1f; b = 1.7f; c = 2.2f; w = 0.50f; comp = 1; break;
+        case 4: axis = 2; a = 3.1f; b = 2.9f; c = 0.9f; w = 0.30f; comp = 1; break;
+        case 5: axis = 0; a = 1.4f; b = 2.3f; c = 3.1f; w = 0.30f; comp = 2; break;
+        case 6: axis = 1; a = 2.7f; b = 1.9f; c = 1.2f; w = 0.20f; comp = 2; break;
+        default: axis = 0; a = 0; b = 0; c = 0; w = 0; comp = 3; break;
+    }
+}
+
+template <int L>
+struct WindLane {  // this lane's share of the 7 gust terms
+    static constexpr int NT = (7 + L - 1) / L;
+    float ax[NT], ay[NT], az[NT], bt[NT], cs[NT], wg[NT];
+    int comp[NT];
+};
+
+template <int L, int P>
+__global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & (L - 1);
+    const int gbase = lane & ~(L - 1);
+    const int n_in = *a.count_in;
+    Body B;
+    WindLane<L> W;
+    float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];
+    bool active = false, exhausted = false;
+    unsigned long long fsteps = 0, psteps = 0;
+    B.fid = -1; B.nt = 0; B.step = 0; B.end = 0;
+
+    for (;;) {
+        const bool need = !active && !exhausted;
+        if (__ballot(need)) {
+            if (need) {
+                int k = 0;
+                if (j == 0) k = atomicAdd(a.head, 1);
+                k = __shfl(k, gbase);
+                if (k >= n_in) {
+                    exhausted = true;
+                } else {
+                    // ---- set up flight fid (all L lanes of the group, identical values)
+                    const int fid = a.list_in[k];
+                    const FlightConst& fc = a.fc[fid];
+                    const FlightState& st = a.state[fid];
+                    const int gi = fc.genome;
+                    B.fid = fid;
+                    B.nt = a.nt[gi];
+                    const int off = a.off[gi];
+                    B.px = st.pos[0]; B.py = st.pos[1]; B.pz = st.pos[2];
+                    B.qw = st.q[0]; B.qx = st.q[1]; B.qy = st.q[2]; B.qz = st.q[3];
+                    B.vx = st.vel[0]; B.vy = st.vel[1]; B.vz = st.vel[2];
+                    B.wx = st.wb[0]; B.wy = st.wb[1]; B.wz = st.wb[2];
+                    B.step = st.step;
+                    B.t = B.step == 0 ? 0.0f : (float)B.step * PR_DT;  // chunk clock rule (sim.cu:113)
+                    B.end = min(B.step + a.chunk, a.max_steps);
+                    B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
+                    B.I0 = fc.I[0]; B.I1 = fc.I[1]; B.I2 = fc.I[2]; B.I3 = fc.I[3]; B.I4 = fc.I[4]; B.I5 = fc.I[5];
+                    B.v0 = fc.iv[0]; B.v1 = fc.iv[1]; B.v2 = fc.iv[2]; B.v4 = fc.iv[3]; B.v5 = fc.iv[4]; B.v8 = fc.iv[5];
+                    B.cf = fc.cf;
+                    B.bx = fc.base[0]; B.by = fc.base[1]; B.bz = fc.base[2];
+                    const float phase = fc.phase, gust = fc.gust, hr = fc.half_rho;
+                    const vec3 shift = V(fc.shift[0], fc.shift[1], fc.shift[2]);
+                    const float wp = fc.warp_p, wr = fc.warp_r;
+#pragma unroll
+                    for (int m = 0; m < WindLane<L>::NT; m++) {
+                        int axis, comp;
+                        float ca, cb, cc, cw;
+                        wind_term(j + m * L, axis, ca, cb, cc, cw, comp);
+                        W.ax[m] = axis == 0 ? ca : 0.0f; W.ay[m] = axis == 1 ? ca : 0.0f; W.az[m] = axis == 2 ? ca : 0.0f;
+                        W.bt[m] = cb; W.cs[m] = cc * phase; W.wg[m] = cw * gust; W.comp[m] = comp;
+                    }
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        const int t = j + p * L;
+                        if (t < B.nt) {
+                            const size_t o = (size_t)off + t;
+                            vec3 rb = vsub(V(a.pcx[o], a.pcy[o], a.pcz[o]), shift);
+                            vec3 nb = warped_normal(rb, V(a.pnx[o], a.pny[o], a.pnz[o]), wp, wr);
+                            rx[p] = rb.x; ry[p] = rb.y; rz[p] = rb.z;
+                            nx[p] = nb.x; ny[p] = nb.y; nz[p] = nb.z;
+                            kk[p] = hr * a.pA[o];
+                        } else {
+                            rx[p] = 0; ry[p] = 0; rz[p] = 0; nx[p] = 0; ny[p] = 0; nz[p] = 1; kk[p] = 0;
+                        }
+                    }
+                    active = true;
+                }
+            }
+        }
+        if (!__ballot(active)) break;
+        if (!active) continue;
+
+        // ---------------------------------------------------------------- one 600 Hz step
+        const float qw = B.qw, qx = B.qx, qy = B.qy, qz = B.qz;
+        const float r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
+        const float r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
+        const float r20 = 2 * (qx * qz - qw * qy), r21 = 2 * (qy * qz + qw * qx), r22 = 1 - 2 * (qx * qx + qy * qy);
+        // gust field at the CG: this lane's terms, then a group sum per component
+        float gx = 0, gy = 0, gz = 0;
+#pragma unroll
+        for (int m = 0; m < WindLane<L>::NT; m++) {
+            float arg = W.ax[m] * B.px + W.ay[m] * B.py + W.az[m] * B.pz + W.bt[m] * B.t + W.cs[m];
+            float sv = W.wg[m] * sin_fast(arg);
+            gx += W.comp[m] == 0 ? sv : 0.0f;
+            gy += W.comp[m] == 1 ? sv : 0.0f;
+            gz += W.comp[m] == 2 ? sv : 0.0f;
+        }
+        gx = group_sum<L>(gx); gy = group_sum<L>(gy); gz = group_sum<L>(gz);
+        // air velocity relative to the CG, in body axes
+        const float ux = B.vx - (B.bx + gx), uy = B.vy - (B.by + gy), uz = B.vz - (B.bz + gz);
+        const float vbx = r00 * ux + r10 * uy + r20 * uz;
+        const float vby = r01 * ux + r11 * uy + r21 * uz;
+        const float vbz = r02 * ux + r12 * uy + r22 * uz;
+        const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
+        float Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            // relative flow at the panel: v_b + w x r (body axes)
+            const float vrx = vbx + (wy * rz[p] - wz * ry[p]);
+            const float vry = vby + (wz * rx[p] - wx * rz[p]);
+            const float vrz = vbz + (wx * ry[p] - wy * rx[p]);
+            const float V2 = vrx * vrx + vry * vry + vrz * vrz;
+            const float inv = __builtin_amdgcn_rsqf(fmaxf(V2, 1e-6f));
+            const float sn = (vrx * nx[p] + vry * ny[p] + vrz * nz[p]) * inv;
+            const float qd = V2 >= 1e-6f ? kk[p] * V2 : 0.0f;
+            const float sq = __builtin_amdgcn_sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
+            // cn = 1.9 s sqrt(1-s^2) + 2 s|s| = 2 s (0.95 sqrt(1-s^2) + |s|)
+            const float h = 0.95f * sq + fabsf(sn);
+            const float qs = qd * sn;
+            const float c1 = qs * (cf - 2.0f * h);     // coefficient on n: -qd cn + qd cf s
+            const float c2 = -(qd * inv) * cf;           // coefficient on v_r: -qd cf / |v_r|
+            const float fx = c1 * nx[p] + c2 * vrx;
+            const float fy = c1 * ny[p] + c2 * vry;
+            const float fz = c1 * nz[p] + c2 * vrz;
+            Fx += fx; Fy += fy; Fz += fz;
+            Tx += ry[p] * fz - rz[p] * fy;
+            Ty += rz[p] * fx - rx[p] * fz;
+            Tz += rx[p] * fy - ry[p] * fx;
+        }
+        Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
+        Tx = group_sum<L>(Tx); Ty = group_sum<L>(Ty); Tz = group_sum<L>(Tz);
+        // rigid-body update (sim.cu:177-197); torque is already in body axes
+        float Fwx = r00 * Fx + r01 * Fy + r02 * Fz;
+        float Fwy = r10 * Fx + r11 * Fy + r12 * Fz;
+        float Fwz = r20 * Fx + r21 * Fy + r22 * Fz;
+        Fwz -= B.mass * PR_GRAV;
+        const float Iwx = B.I0 * wx + B.I3 * wy + B.I4 * wz;
+        const float Iwy = B.I3 * wx + B.I1 * wy + B.I5 * wz;
+        const float Iwz = B.I4 * wx + B.I5 * wy + B.I2 * wz;
+        const float tqx = Tx - (wy * Iwz - wz * Iwy);
+        const float tqy = Ty - (wz * Iwx - wx * Iwz);
+        const float tqz = Tz - (wx * Iwy - wy * Iwx);
+        const float dwx = B.v0 * tqx + B.v1 * tqy + B.v2 * tqz;
+        const float dwy = B.v1 * tqx + B.v4 * tqy + B.v5 * tqz;
+        const float dwz = B.v2 * tqx + B.v5 * tqy + B.v8 * tqz;
+        float nwx = wx + dwx * PR_DT, nwy = wy + dwy * PR_DT, nwz = wz + dwz * PR_DT;
+        const float wl = sqrtf(nwx * nwx + nwy * nwy + nwz * nwz);
+        if (wl > 150.f) { const float s = 150.f / wl; nwx *= s; nwy *= s; nwz *= s; }
+        const float ks = B.ks;
+        float nvx = B.vx + Fwx * ks, nvy = B.vy + Fwy * ks, nvz = B.vz + Fwz * ks;
+        const float vl = sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
+        if (vl > 60.f) { const float s = 60.f / vl; nvx *= s; nvy *= s; nvz *= s; }
+        const float npx = B.px + nvx * PR_DT, npy = B.py + nvy * PR_DT, npz = B.pz + nvz * PR_DT;
+        // q += 0.5 dt q (x) (0, w)
+        const float dqw = -qx * nwx - qy * nwy - qz * nwz;
+        const float dqx = qw * nwx + qy * nwz - qz * nwy;
+        const float dqy = qw * nwy - qx * nwz + qz * nwx;
+        const float dqz = qw * nwz + qx * nwy - qy * nwx;
+        const float hd = 0.5f * PR_DT;
+        float nqw = qw + dqw * hd, nqx = qx + dqx * hd, nqy = qy + dqy * hd, nqz = qz + dqz * hd;
+        const float il = 1.0f / sqrtf(nqw * nqw + nqx * nqx + nqy * nqy + nqz * nqz);
+        B.qw = nqw * il; B.qx = nqx * il; B.qy = nqy * il; B.qz = nqz * il;
+        B.vx = nvx; B.vy = nvy; B.vz = nvz;
+        B.wx = nwx; B.wy = nwy; B.wz = nwz;
+        B.px = npx; B.py = npy; B.pz = npz;
+        B.t += PR_DT;
+        B.step += 1;
+        fsteps += 1;
+        psteps += (unsigned long long)B.nt;
+        // termination (sim.cu:198-204)
+        const bool bad = !(npx == npx && nvx == nvx);
+        const bool ended = npz <= 0.0f || bad || fabsf(npy) > 300.f || npx < -50.f;
+        const bool capped = B.step >= a.max_steps;
+        if (ended || capped || B.step >= B.end) {
+            if (j == 0) {
+                const bool fin = ended || capped;
+                FlightState st;
+                st.pos[0] = B.px; st.pos[1] = B.py; st.pos[2] = B.pz;
+                st.q[0] = B.qw; st.q[1] = B.qx; st.q[2] = B.qy; st.q[3] = B.qz;
+                st.vel[0] = B.vx; st.vel[1] = B.vy; st.vel[2] = B.vz;
+                st.wb[0] = B.wx; st.wb[1] = B.wy; st.wb[2] = B.wz;
+                st.step = B.step; st.done = fin ? 1 : 0;
+                st.fitness = fin ? (bad ? -10.0f : npx) : 0.0f;
+                a.state[B.fid] = st;
+                if (fin) {
+                    a.fitness[B.fid] = st.fitness;
+                    a.steps[B.fid] = B.step;
+                } else {
+                    int pos = atomicAdd(a.count_out, 1);
+                    a.list_out[pos] = B.fid;
+                }
+            }
+            active = false;
+        }
+    }
+    // algorithmic work counters: one lane per group counted each step
+    if (j != 0) { fsteps = 0; psteps = 0; }
+    for (int o = 32; o > 0; o >>= 1) {
+        fsteps += __shfl_xor(fsteps, o);
+        psteps += __shfl_xor(psteps, o);
+    }
+    if (lane == 0 && fsteps) {
+        atomicAdd(a.stats + 0, fsteps);
+        atomicAdd(a.stats + 1, psteps);
+    }
+}
+
+// ------------------------------------------------------------------ k_scores (sim.cu:340-357)
+template <int MAXS>
+__global__ void k_scores(const float* __restrict__ samples, int pop, int nseeds, float* __restrict__ scores) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pop) return;
+    float row[MAXS];
+    float mean = 0.0f;
+    for (int s = 0; s < nseeds; s++) {
+        float x = samples[(size_t)i * nseeds + s];
+        mean += x / (float)nseeds;
+        int k = s;  // insertion sort, ascending
+        while (k > 0 && row[k - 1] > x) { row[k] = row[k - 1]; k--; }
+        row[k] = x;
+    }
+    int tail = (nseeds + 3) / 4;
+    float cv = 0.0f;
+    for (int s = 0; s < tail; s++) cv += row[s] / (float)tail;
+    scores[i] = 0.65f * mean + 0.25f * cv + 0.10f * row[0];
+}
+
+// ------------------------------------------------------------------ launchers
+hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_layout, dim3((n + 255) / 256), dim3(256), 0, s, genomes, res, ch, sp, n, nt_raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare(const PrepareArgs& a, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    size_t lds = sizeof(float) * (3 * (size_t)a.nv_max + 12 * 64);
+    hipLaunchKernelGGL(k_prepare, dim3(n), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scenarios, dim3((n + 127) / 128), dim3(128), 0, s, seeds, n, sp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_queue(const QueueArgs& a, hipStream_t s) {
+    if (a.n_flights <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init, dim3((a.n_flights + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(int* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_ints, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+template <int L, int P>
+static hipError_t fly_one(const FlyArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_fly<L, P>), dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+// compiled (L, P) variants; capacity L*P triangles per flight
+#define PR_VARIANTS(X) \
+    X(4, 16) X(4, 24) X(8, 12) X(8, 16) X(8, 20) X(8, 24) X(16, 12) X(16, 16) X(16, 22) X(16, 24) \
+    X(32, 16) X(32, 24) X(32, 29) X(64, 16) X(64, 24) X(64, 32)
+
+int fly_variant_count() {
+    int n = 0;
+#define CNT(L, P) n++;
+    PR_VARIANTS(CNT)
+#undef CNT
+    return n;
+}
+
+int fly_variant_info(int i, int* L, int* P) {
+    int k = 0;
+#define INF(LL, PP) if (k++ == i) { *L = LL; *P = PP; return 0; }
+    PR_VARIANTS(INF)
+#undef INF
+    return -1;
+}
+
+hipError_t launch_fly(int L, int P, const FlyArgs& a, int grid, hipStream_t s) {
+#define DISP(LL, PP) if (L == LL && P == PP) return fly_one<LL, PP>(a, grid, s);
+    PR_VARIANTS(DISP)
+#undef DISP
+    return hipErrorInvalidValue;
+}
+
+hipError_t fly_occupancy(int L, int P, int* blocks_per_cu) {
+#define OCC(LL, PP) if (L == LL && P == PP) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fly<LL, PP>, 64, 0);
+    PR_VARIANTS(OCC)
+#undef OCC
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_scores(const float* samples, int pop, int nseeds, float* scores, hipStream_t s) {
+    if (pop <= 0) return hipSuccess;
+    dim3 grid((pop + 127) / 128), blk(128);
+    if (nseeds <= 16) hipLaunchKernelGGL(k_scores<16>, grid, blk, 0, s, samples, pop, nseeds, scores);
+    else if (nseeds <= 64) hipLaunchKernelGGL(k_scores<64>, grid, blk, 0, s, samples, pop, nseeds, scores);
+    else if (nseeds <= 256) hipLaunchKernelGGL(k_scores<256>, grid, blk, 0, s, samples, pop, nseeds, scores);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace pr

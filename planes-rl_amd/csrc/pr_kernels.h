@@ -1,0 +1,70 @@
+// pr_kernels.h -- argument blocks and host launchers of kernels.hip (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pr_physics.h"
+
+namespace pr {
+
+struct PrepareArgs {
+    const float* genomes;  // [n*35]
+    const int* res;        // [n*2] (ch, sp) per genome, or null -> def_ch/def_sp
+    int def_ch, def_sp;
+    int nv_max;            // LDS vertex capacity
+    const int* off;        // [n] exclusive prefix of the closed-form triangle counts
+    Mass* mass;            // [n]
+    int* nt;               // [n] triangles flown (0 if outside the print volume)
+    int* nv;               // [n]
+    float *pcx, *pcy, *pcz, *pA, *pnx, *pny, *pnz;  // panel SoA at off[g] + t
+};
+
+struct QueueArgs {
+    int n_flights;
+    int layout;            // 0: flight f = (genome f / nseeds, scenario f % nseeds); 1: (f, f)
+    int nseeds;
+    const int* nt;
+    const Mass* mass;
+    const float* scen;     // [n_scen*26]
+    const int* seeds;
+    float spread;
+    FlightConst* fc;
+    int n_buckets;
+    const int* bucket_nt;  // device [n_buckets] upper triangle count of each bucket
+    int* count;            // [n_buckets]
+    int* list;             // [n_buckets * list_stride]
+    int list_stride;
+    FlightState* state;
+    float* fitness;
+    int* steps;
+};
+
+struct FlyArgs {
+    const int* nt;
+    const int* off;
+    const float *pcx, *pcy, *pcz, *pA, *pnx, *pny, *pnz;
+    const FlightConst* fc;
+    FlightState* state;
+    float* fitness;
+    int* steps;
+    const int* list_in;
+    const int* count_in;
+    int* list_out;
+    int* count_out;
+    int* head;
+    int chunk, max_steps;
+    float spread;
+    unsigned long long* stats;  // [0] flight-steps, [1] panel-steps
+};
+
+hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
+hipError_t launch_prepare(const PrepareArgs& a, int n, hipStream_t s);
+hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, float* out, hipStream_t s);
+hipError_t launch_queue(const QueueArgs& a, hipStream_t s);
+hipError_t launch_zero(int* p, int n, hipStream_t s);
+hipError_t launch_fly(int L, int P, const FlyArgs& a, int grid, hipStream_t s);
+hipError_t fly_occupancy(int L, int P, int* blocks_per_cu);
+int fly_variant_count();
+int fly_variant_info(int i, int* L, int* P);
+hipError_t launch_scores(const float* samples, int pop, int nseeds, float* scores, hipStream_t s);
+
+}  // namespace pr
