@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark of the fitness-evaluation path on one MI355X.
+
+One step = one GA generation's evaluation (pr_gen_evaluate): meshing + mass properties of
+every genome, scenario generation, every (genome, seed) flight at 600 Hz in 1-s chunks,
+tail-aware robust scores, ranking, and the top-8 re-test on the 128-scenario holdout
+suite.  Breeding and file I/O are outside the timed region (the metric's definition).
+
+value = plane-flight-seconds per wall-second = sum of integrated steps / 600 over the
+timed steps (training and holdout flights) / wall time, genomes resident in HBM.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "planes-rl_amd"))
+
+METRIC = "plane-flight-sec/s at 600 Hz; GA gens/hour at 100k×12 env, plain vs surrogate"
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+FLOP_PANEL, FLOP_PLANE = 68, 222  # algorithmic FLOPs per panel-step and per plane-step
+
+WORKLOADS = {
+    # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
+    "c1": dict(pop=64, nseeds=8, max_steps=4800, gust=2.0, warp_deg=1.0, wind_sd=2.0,
+               desc="64 genomes x 8 seeded launch scenarios, config preset, 8 s cap"),
+    "c2": dict(pop=10000, nseeds=8, max_steps=9000, gust=2.0, warp_deg=1.0, wind_sd=2.0,
+               desc="one GA generation: pop 10000 x 8 seeds, config preset, 15 s cap, top-8 x 128 holdout"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def seeds_for(gen, nseeds):
+    return np.array([gen * nseeds + s + 1 for s in range(nseeds)], np.int32)
+
+
+def cpu_baseline(genomes, w, budget_s):
+    """The oracle's FP32 transcription on this host's cores over a bounded sample of the
+    same workload (same genome distribution, scenario preset and cap)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    nth = os.cpu_count() or 1
+    rng = np.random.default_rng(1)
+    kw = dict(prec=32, preset=1, gust=w["gust"], warp=float(np.deg2rad(w["warp_deg"])), wsd=w["wind_sd"],
+              max_steps=w["max_steps"], nthreads=nth)
+
+    def sample(n):
+        gi = rng.integers(0, len(genomes), n).astype(np.int32)
+        sd = rng.integers(1, 1 + 8 * 1000, n).astype(np.int32)
+        return gi, sd
+
+    gi, sd = sample(max(64, 8 * nth))
+    _, steps, wall = O.fly(genomes, gi, sd, **kw)
+    per = wall / len(gi)
+    n = int(min(200000, max(len(gi), budget_s / max(per, 1e-6))))
+    gi, sd = sample(n)
+    _, steps, wall = O.fly(genomes, gi, sd, **kw)
+    pfs = steps.sum() / 600.0
+    return {"value": pfs / wall, "unit": "plane-flight-s/s", "cores": nth, "kind": "port",
+            "sample": f"{n} random (genome, seed) flights of the same workload, FP32 oracle "
+                      f"(g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {pfs:.0f} plane-flight-s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
+              f"described in DESIGN.md and not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import planes_amd as PA
+
+    w = WORKLOADS[args.workload]
+    log("device:", PA.device_name())
+    lo, hi = PA.gene_bounds()
+    rng = np.random.default_rng(0)
+    genomes = (lo + rng.random((w["pop"], 35), dtype=np.float32) * (hi - lo)).astype(np.float32)
+    params = PA.SimParams.config(gust_rms_max=w["gust"], warp_max_deg=w["warp_deg"], wind_sd=w["wind_sd"],
+                                 max_steps=w["max_steps"])
+    E = PA.Generation(w["pop"], w["nseeds"])
+    E.upload(genomes)
+
+    for k in range(args.warmup):
+        E.evaluate(w["pop"], seeds_for(10000 + k, w["nseeds"]), params)
+    PA.LIB.pr_device_sync()
+    fsteps = 0
+    bucket = {}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = E.evaluate(w["pop"], seeds_for(k, w["nseeds"]), params)
+        for st in E.stats():
+            fsteps += st["flight_steps"]
+            for b in st["buckets"]:
+                key = (b["L"], b["P"])
+                acc = bucket.setdefault(key, {"flight_steps": 0, "panel_steps": 0, "fly_ms": 0.0, "launches": 0})
+                for f in ("flight_steps", "panel_steps", "fly_ms", "launches"):
+                    acc[f] += b[f]
+    PA.LIB.pr_device_sync()
+    wall = time.perf_counter() - t0
+    pfs = fsteps / 600.0
+    value = pfs / wall
+    ms_per_step = 1000.0 * wall / args.steps
+    log(f"{args.steps} generations in {wall:.3f} s, {pfs:.0f} plane-flight-s, last holdout {r['holdout']:.3f}")
+
+    # dominant flight-kernel variant: algorithmic FLOPs / its event-timed duration
+    key, b = max(bucket.items(), key=lambda kv: kv[1]["fly_ms"])
+    flops = FLOP_PANEL * b["panel_steps"] + FLOP_PLANE * b["flight_steps"]
+    achieved = flops / (b["fly_ms"] * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": f"k_fly<{key[0]},{key[1]}>", "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": None, "avg_launch_ms": round(b["fly_ms"] / max(1, b["launches"]), 4),
+            "flops_per_launch": flops / max(1, b["launches"])}
+    for kk, bb in sorted(bucket.items()):
+        log(f"  k_fly<{kk[0]},{kk[1]}>: {bb['launches']} launches, {bb['fly_ms']:.2f} ms, "
+            f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(genomes, w, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.1f} pfs/s on {cpu['cores']} threads")
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{args.workload.upper()}: {w['desc']}", "pop": w["pop"], "seeds": w["nseeds"],
+                      "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU"},
+           "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

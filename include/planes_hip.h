@@ -33,12 +33,17 @@ typedef struct pr_sim_params {
     float spread;        /* showcase launch grid spacing [m], 0 for evaluation (sim.cu:71-78) */
 } pr_sim_params;
 
+#define PR_MAX_BUCKETS 8
 typedef struct pr_stats {
     unsigned long long flight_steps; /* integrated 600 Hz steps of the last pr_fly (= plane-flight-s * 600) */
     unsigned long long panel_steps;  /* sum over those steps of the plane's triangle count */
     double fly_ms;                   /* GPU time of the flight-kernel launches (HIP events on the context stream) */
     int fly_launches;
     double prep_ms;                  /* GPU time of the last pr_prepare */
+    int n_buckets;                   /* per triangle-count bucket (one flight-kernel variant each): */
+    int b_nt_max[PR_MAX_BUCKETS], b_L[PR_MAX_BUCKETS], b_P[PR_MAX_BUCKETS], b_launches[PR_MAX_BUCKETS];
+    unsigned long long b_flight_steps[PR_MAX_BUCKETS], b_panel_steps[PR_MAX_BUCKETS];
+    double b_fly_ms[PR_MAX_BUCKETS];
 } pr_stats;
 
 /* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).
@@ -55,6 +60,11 @@ const char* pr_last_error(void);
  * the print-volume check and packed panel records.  d_res: optional [n*2] (ch, sp)
  * per genome (null = reference resolution).  Asynchronous on the context stream. */
 int pr_prepare(pr_ctx* ctx, const float* d_genomes, const int* d_res, int n);
+
+/* Arithmetic of pr_prepare: 32 (default) reproduces the reference's float meshing and
+ * mass_props bit for bit; 64 meshes and integrates in double and rounds the results
+ * once to float (mass/CG/inertia then agree with an FP64 evaluation to float rounding). */
+int pr_set_prepare_precision(pr_ctx* ctx, int bits);
 
 /* Scenario descriptors for n seeds (d_seeds device int32) -- scenario_desc, common.h:389. */
 int pr_scenarios(pr_ctx* ctx, const int* d_seeds, int n, const pr_sim_params* p);
@@ -74,11 +84,24 @@ int pr_fly(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_para
 int pr_init_flights(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
                     int* d_steps);
 
+/* pr_fly with pose recording (gpu_sim's d_rec path, sim.cu:117-128): d_rec device
+ * [n_flights * rec_cap * 8] floats, frame k of flight f at (f*rec_cap + k)*8 =
+ * {pos.xyz, q.wxyz, alive}; frames every rec_every steps; a flight that has ended keeps
+ * its final pose with alive = 0. */
+int pr_fly_record(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                  int* d_steps, float* d_rec, int rec_every, int rec_cap);
+
 /* Tail-aware robust score per genome (robust_scores, sim.cu:340-357):
  * 0.65 mean + 0.25 CVaR(worst quarter) + 0.10 worst, from d_samples [pop*nseeds]. */
 int pr_scores(pr_ctx* ctx, const float* d_samples, int pop, int nseeds, float* d_scores);
 
 int pr_stats_get(pr_ctx* ctx, pr_stats* out);
+
+/* 1: pr_fly captures its whole launch sequence (flight set-up + every bucket's chain of
+ * 1-s chunk launches, buckets on parallel branches) as a hipGraph on first use and
+ * replays it while the launch parameters stay the same; per-launch timings are then
+ * replaced by one whole-graph time.  0 (default): direct launches. */
+int pr_set_graph(pr_ctx* ctx, int on);
 
 /* Flight-kernel mapping per triangle-count bucket: flights whose count is <= nt_max[i]
  * (and above the previous bucket's) are flown by groups of L[i] lanes holding P[i]
@@ -90,7 +113,33 @@ int pr_fly_variants(int* n, int* L, int* P, int cap);
 int pr_read_prepared(pr_ctx* ctx, int n, float* mass20, int* nt, int* nv, int* off);
 int pr_read_panels(pr_ctx* ctx, long long first, int count, float* out7); /* cx cy cz A nx ny nz rows */
 int pr_read_scenarios(pr_ctx* ctx, int n, float* out26);
+int pr_read_extent(pr_ctx* ctx, int n, float* ext3);                      /* mesh bounding-box extents */
 int pr_read_states(pr_ctx* ctx, int first, int n, float* out16);       /* FlightState rows, 16 words */
+
+/* ---------------------------------------------------------------- GA host side
+ * Genome bounds (common.h:98-103), the GA's random stream and operators with the
+ * reference's libstdc++ call order (sim.cu:315-325, 398, 429-430, 440-441, 472-490). */
+void pr_gene_bounds(float* lo35, float* hi35);
+void* pr_ga_new(unsigned seed);                       /* std::mt19937(seed) + the GA's distributions */
+void pr_ga_free(void* ga);
+void pr_ga_random(void* ga, float* genomes, int n);   /* grand(): uniform in bounds */
+void pr_ga_sanitize(float* genomes, int n);           /* clamp to bounds */
+void pr_ga_rank(const float* acc, int pop, int* idx); /* std::sort by score, descending */
+void pr_ga_breed(void* ga, const float* gs, const float* acc, const int* idx, int pop, int gen, int gens,
+                 float* next);
+float pr_robust_score_host(const float* row, int n);
+
+/* One generation's evaluation (sim.cu:432-456), batched: every genome under every seed in
+ * one pr_fly, robust scores, std::sort ranking, then the top min(holdout_k, pop) genomes on
+ * the 128-scenario holdout suite (seeds 100003 + p*7919) in one batch; champion = first
+ * strict maximum of the holdout score.  d_genomes on the device; everything else host. */
+typedef struct pr_gen pr_gen;
+int pr_gen_create(pr_gen** out, int pop, int nseeds_max, int max_ch, int max_sp);
+void pr_gen_destroy(pr_gen* e);
+int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds, int nseeds,
+                    int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
+                    float* holdout, int* champion);
+int pr_gen_stats(pr_gen* e, pr_stats* main_stats, pr_stats* holdout_stats);
 
 /* Device memory helpers so that hosts without a GPU array library can drive the API. */
 int pr_dev_alloc(void** p, size_t bytes);

@@ -1060,5 +1060,8 @@ float or_percentile(const float* sorted, int n, float q) {
 }
 
 int or_hw_threads() { return (int)std::thread::hardware_concurrency(); }
+float or_sinf(float x) { return osin(x); }
+float or_cosf(float x) { return ocos(x); }
+float or_tanf(float x) { return otan(x); }
 
 }  // extern "C"

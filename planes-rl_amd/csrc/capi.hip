@@ -47,6 +47,7 @@ struct pr_ctx {
     // genome side
     int n_prepared = 0;
     int *d_ntraw = nullptr, *d_off = nullptr, *d_nt = nullptr, *d_nv = nullptr;
+    float* d_ext = nullptr;
     Mass* d_mass = nullptr;
     float* d_panel[7] = {};
     void* d_scan_tmp = nullptr;
@@ -65,7 +66,14 @@ struct pr_ctx {
     std::vector<Bucket> buckets;
     std::vector<int> grid;    // persistent grid per bucket
     hipEvent_t ev[2 * 256 + 2] = {};
-    int n_events = 0;
+    int ev_bucket[256] = {};
+    hipStream_t bstream[8] = {};     // one stream per triangle-count bucket (its chunk chain)
+    hipEvent_t fork = nullptr, join[8] = {}, g0 = nullptr, g1 = nullptr;
+    // hipGraph of the whole flight sequence (optional), keyed by its launch parameters
+    int use_graph = 0;
+    int prep_fp64 = 0;
+    hipGraphExec_t gexec = nullptr;
+    long long gkey[10] = {};
     pr_stats stats = {};
     hipEvent_t prep0 = nullptr, prep1 = nullptr;
 };
@@ -144,6 +152,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_off, sizeof(int) * (max_genomes + 1));
     ALLOC(c->d_nt, sizeof(int) * max_genomes);
     ALLOC(c->d_nv, sizeof(int) * max_genomes);
+    ALLOC(c->d_ext, sizeof(float) * 3 * max_genomes);
     ALLOC(c->d_mass, sizeof(Mass) * max_genomes);
     for (int k = 0; k < 7; k++) ALLOC(c->d_panel[k], sizeof(float) * max_panels);
     ALLOC(c->d_seeds, sizeof(int) * max_scenarios);
@@ -153,11 +162,21 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
     ALLOC(c->d_counts, sizeof(int) * 3 * NB_MAX);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
-    ALLOC(c->d_stats, sizeof(unsigned long long) * 2);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream);
+    ALLOC(c->d_stats, sizeof(unsigned long long) * 2 * NB_MAX);
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream) !=
+        hipSuccess) { pr_destroy(c); return fail(-1, "scan workspace query"); }
     ALLOC(c->d_scan_tmp, c->scan_tmp_bytes + 16);
 #undef ALLOC
     for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
+    for (int b = 0; b < NB_MAX; b++) {
+        if (hipStreamCreateWithFlags(&c->bstream[b], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
+            pr_destroy(c);
+            return fail(-1, "bucket stream");
+        }
+    }
+    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&c->g0) != hipSuccess ||
+        hipEventCreate(&c->g1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
     if (hipEventCreate(&c->prep0) != hipSuccess || hipEventCreate(&c->prep1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
     if ((rc = upload_buckets(c))) { pr_destroy(c); return rc; }
     *out = c;
@@ -166,16 +185,24 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
 
 void pr_destroy(pr_ctx* c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->d_ntraw); hipFree(c->d_off); hipFree(c->d_nt); hipFree(c->d_nv); hipFree(c->d_mass);
-    for (int k = 0; k < 7; k++) hipFree(c->d_panel[k]);
-    hipFree(c->d_seeds); hipFree(c->d_scen); hipFree(c->d_state); hipFree(c->d_fc);
-    hipFree(c->d_lists[0]); hipFree(c->d_lists[1]); hipFree(c->d_counts); hipFree(c->d_bucket_nt);
-    hipFree(c->d_stats); hipFree(c->d_scan_tmp);
-    for (auto& e : c->ev) if (e) hipEventDestroy(e);
-    if (c->prep0) hipEventDestroy(c->prep0);
-    if (c->prep1) hipEventDestroy(c->prep1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    // teardown: release everything; failures here have nowhere to be reported
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
+                    c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp};
+    for (void* b : bufs) if (b) (void)hipFree(b);
+    for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->prep0) (void)hipEventDestroy(c->prep0);
+    if (c->prep1) (void)hipEventDestroy(c->prep1);
+    for (int b = 0; b < NB_MAX; b++) {
+        if (c->join[b]) (void)hipEventDestroy(c->join[b]);
+        if (c->bstream[b]) (void)hipStreamDestroy(c->bstream[b]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->g0) (void)hipEventDestroy(c->g0);
+    if (c->g1) (void)hipEventDestroy(c->g1);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -221,10 +248,10 @@ int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
     }
     PrepareArgs a;
     a.genomes = d_genomes; a.res = d_res; a.def_ch = 5; a.def_sp = 8; a.nv_max = c->nv_max;
-    a.off = c->d_off; a.mass = c->d_mass; a.nt = c->d_nt; a.nv = c->d_nv;
+    a.off = c->d_off; a.mass = c->d_mass; a.nt = c->d_nt; a.nv = c->d_nv; a.ext = c->d_ext;
     a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
     a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
-    HCK(launch_prepare(a, n, c->stream));
+    HCK(launch_prepare(a, n, c->prep_fp64, c->stream));
     HCK(hipEventRecord(c->prep1, c->stream));
     c->n_prepared = n;
     return 0;
@@ -242,6 +269,15 @@ int pr_scenarios(pr_ctx* c, const int* d_seeds, int n, const pr_sim_params* p) {
     return 0;
 }
 
+static void fill_fly_args(pr_ctx* c, const pr_sim_params* p, float* d_fitness, int* d_steps, FlyArgs& a) {
+    a.nt = c->d_nt; a.off = c->d_off; a.fc = c->d_fc;
+    a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
+    a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
+    a.state = c->d_state; a.fitness = d_fitness; a.steps = d_steps;
+    a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
+    a.list_in = nullptr; a.count_in = nullptr; a.list_out = nullptr; a.count_out = nullptr; a.head = nullptr;
+}
+
 static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
                         int* d_steps, long long* nf_out) {
     ARG(c && p && d_fitness && d_steps, "pr_fly: null argument");
@@ -254,11 +290,11 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     ARG(nf <= c->cap_flights, "pr_fly: flights exceed max_flights");
     const int NB = (int)c->buckets.size();
     HCK(launch_zero(c->d_counts, 3 * NB_MAX, c->stream));
-    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 4, c->stream));
+    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 4 * NB_MAX, c->stream));
     QueueArgs q;
     q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
     q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
-    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.list = c->d_lists[0];
+    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.count_stride = 3; q.list = c->d_lists[0];
     q.list_stride = c->cap_flights; q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
     HCK(launch_queue(q, c->stream));
     *nf_out = nf;
@@ -274,52 +310,135 @@ int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_s
     return 0;
 }
 
-int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness, int* d_steps) {
-    long long nf;
-    int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
-    if (rc) return rc;
+// Issue the flight sequence after k_init: each bucket's chain of chunk launches on its own
+// stream, forked from and joined back to the context stream.  Counters of bucket b:
+// d_counts[3b + 0/1] = list counts (double-buffered), d_counts[3b + 2] = queue head.
+static int issue_chains(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed, int* n_events) {
     const int NB = (int)c->buckets.size();
-    int* cnt[2] = {c->d_counts, c->d_counts + NB_MAX};
-    int* head = c->d_counts + 2 * NB_MAX;
-    FlyArgs a;
-    a.nt = c->d_nt; a.off = c->d_off; a.fc = c->d_fc;
-    a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
-    a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
-    a.state = c->d_state; a.fitness = d_fitness; a.steps = d_steps;
-    a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
     const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
+    HCK(hipEventRecord(c->fork, c->stream));
     int ne = 0;
-    for (int ck = 0; ck < n_chunks; ck++) {
-        const int in = ck & 1, outb = in ^ 1;
-        HCK(launch_zero(cnt[outb], NB_MAX, c->stream));
-        HCK(launch_zero(head, NB_MAX, c->stream));
-        for (int b = 0; b < NB; b++) {
+    for (int b = 0; b < NB; b++) {
+        hipStream_t s = c->bstream[b];
+        HCK(hipStreamWaitEvent(s, c->fork, 0));
+        int* cb = c->d_counts + 3 * b;
+        a.head = cb + 2;
+        a.stats = c->d_stats + 2 * b;
+        for (int ck = 0; ck < n_chunks; ck++) {
+            const int in = ck & 1, outb = in ^ 1;
             a.list_in = c->d_lists[in] + (size_t)b * c->cap_flights;
-            a.count_in = cnt[in] + b;
             a.list_out = c->d_lists[outb] + (size_t)b * c->cap_flights;
-            a.count_out = cnt[outb] + b;
-            a.head = head + b;
-            const bool timed = ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
-            if (timed) HCK(hipEventRecord(c->ev[ne], c->stream));
-            HCK(launch_fly(c->buckets[b].L, c->buckets[b].P, a, c->grid[b], c->stream));
-            if (timed) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); ne += 2; }
+            a.count_in = cb + in;
+            a.count_out = cb + outb;
+            HCK(launch_zero_pair(cb + outb, cb + 2, s));
+            const bool tev = timed && ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
+            if (tev) HCK(hipEventRecord(c->ev[ne], s));
+            HCK(launch_fly(c->buckets[b].L, c->buckets[b].P, a, c->grid[b], s));
+            if (tev) { HCK(hipEventRecord(c->ev[ne + 1], s)); c->ev_bucket[ne / 2] = b; ne += 2; }
         }
+        HCK(hipEventRecord(c->join[b], s));
+        HCK(hipStreamWaitEvent(c->stream, c->join[b], 0));
+    }
+    *n_events = ne;
+    return 0;
+}
+
+int pr_set_prepare_precision(pr_ctx* c, int bits) {
+    ARG(c && (bits == 32 || bits == 64), "pr_set_prepare_precision: 32 or 64");
+    c->prep_fp64 = bits == 64;
+    return 0;
+}
+
+int pr_set_graph(pr_ctx* c, int on) {
+    ARG(c, "pr_set_graph: null context");
+    c->use_graph = on ? 1 : 0;
+    return 0;
+}
+
+int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness, int* d_steps) {
+    ARG(c && p, "pr_fly: null argument");
+    const int NB = (int)c->buckets.size();
+    FlyArgs a;
+    fill_fly_args(c, p, d_fitness, d_steps, a);
+    int ne = 0;
+    long long nf;
+    if (c->use_graph) {
+        long long key[10] = {layout, n_genomes, n_scen, p->max_steps, p->chunk, (long long)(size_t)d_fitness,
+                             (long long)(size_t)d_steps, p->preset, (long long)c->buckets.size(), c->n_prepared};
+        if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
+            if (c->gexec) { HCK(hipGraphExecDestroy(c->gexec)); c->gexec = nullptr; }
+            HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
+            int unused = 0;
+            if (rc == 0) rc = issue_chains(c, p, a, false, &unused);
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            HCK(ec);
+            hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCK(ei);
+            memcpy(c->gkey, key, sizeof key);
+        }
+        HCK(hipEventRecord(c->g0, c->stream));
+        HCK(hipGraphLaunch(c->gexec, c->stream));
+        HCK(hipEventRecord(c->g1, c->stream));
+    } else {
+        int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
+        if (rc) return rc;
+        rc = issue_chains(c, p, a, true, &ne);
+        if (rc) return rc;
     }
     HCK(hipStreamSynchronize(c->stream));
-    double ms = 0;
+    pr_stats& S = c->stats;
+    S = pr_stats();
+    S.n_buckets = NB;
+    for (int b = 0; b < NB; b++) { S.b_nt_max[b] = c->buckets[b].nt_max; S.b_L[b] = c->buckets[b].L; S.b_P[b] = c->buckets[b].P; }
     for (int k = 0; k < ne; k += 2) {
         float t = 0;
         HCK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
-        ms += t;
+        S.fly_ms += t;
+        S.b_fly_ms[c->ev_bucket[k / 2]] += t;
+        S.b_launches[c->ev_bucket[k / 2]] += 1;
     }
-    unsigned long long st[2];
+    if (c->use_graph) {
+        float t = 0;
+        HCK(hipEventElapsedTime(&t, c->g0, c->g1));
+        S.fly_ms = t;
+    }
+    unsigned long long st[2 * NB_MAX];
     HCK(hipMemcpy(st, c->d_stats, sizeof st, hipMemcpyDeviceToHost));
-    c->stats.flight_steps = st[0];
-    c->stats.panel_steps = st[1];
-    c->stats.fly_ms = ms;
-    c->stats.fly_launches = ne / 2;
+    for (int b = 0; b < NB; b++) {
+        S.b_flight_steps[b] = st[2 * b];
+        S.b_panel_steps[b] = st[2 * b + 1];
+        S.flight_steps += st[2 * b];
+        S.panel_steps += st[2 * b + 1];
+    }
+    S.fly_launches = ne / 2;
     float pm = 0;
-    if (hipEventElapsedTime(&pm, c->prep0, c->prep1) == hipSuccess) c->stats.prep_ms = pm;
+    if (hipEventElapsedTime(&pm, c->prep0, c->prep1) == hipSuccess) S.prep_ms = pm;
+    return 0;
+}
+
+int pr_fly_record(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                  int* d_steps, float* d_rec, int rec_every, int rec_cap) {
+    ARG(d_rec && rec_every >= 1 && rec_cap >= 0, "pr_fly_record: d_rec, rec_every >= 1, rec_cap >= 0");
+    long long nf;
+    int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
+    if (rc) return rc;
+    FlyArgs a;
+    fill_fly_args(c, p, d_fitness, d_steps, a);
+    // every flight flown by the widest-bucket mapping (recording is not throughput work)
+    const Bucket& b = c->buckets.back();
+    HCK(launch_fly_rec(b.L, b.P, a, (int)nf, d_rec, rec_every, rec_cap, c->stream));
+    HCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pr_read_extent(pr_ctx* c, int n, float* ext3) {
+    ARG(c && ext3 && n >= 0 && n <= c->n_prepared, "pr_read_extent: n exceeds the prepared genomes");
+    HCK(hipStreamSynchronize(c->stream));
+    HCK(hipMemcpy(ext3, c->d_ext, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -36,14 +36,6 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-    return v;
-}
 
 // ------------------------------------------------------------------ k_layout
 __global__ void k_layout(const float* __restrict__ genomes, const int* __restrict__ res, int def_ch, int def_sp,
@@ -57,46 +49,68 @@ __global__ void k_layout(const float* __restrict__ genomes, const int* __restric
 }
 
 // ------------------------------------------------------------------ k_prepare
-// block = one wave64 = one genome.  LDS: vertices [nv_max] + 12 x 64 triangle terms.
+// block = one wave64 = one genome.  LDS: vertices [nv_max] + 12 x 64 triangle terms, in T.
+// T = float reproduces the reference's float computation bit for bit; T = double is the
+// FP64 prepare mode (every step in double, results rounded once to float).
+template <class T>
+__device__ __forceinline__ T wave_min_t(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = tmin(v, __shfl_xor(v, o));
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max_t(T v) {
+    for (int o = 32; o > 0; o >>= 1) v = tmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <class T>
 __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
-    extern __shared__ float lds[];
-    __shared__ float g[NG];
+    extern __shared__ double lds_raw[];
+    __shared__ T g[NG];
     const int gi = blockIdx.x, lane = threadIdx.x;
-    if (lane < NG) g[lane] = a.genomes[(size_t)gi * NG + lane];
+    if (lane < NG) g[lane] = T(a.genomes[(size_t)gi * NG + lane]);
     __syncthreads();
+    float gf[2];
+    gf[0] = a.genomes[(size_t)gi * NG + G_FIN_LAYOUT];
+    gf[1] = a.genomes[(size_t)gi * NG + G_DECK_GAP];
+    float gl[NG];  // topology decisions are taken on the float genes
+    for (int k = 0; k < NG; k++) gl[k] = 0.0f;
+    gl[G_FIN_LAYOUT] = gf[0];
+    gl[G_DECK_GAP] = gf[1];
     const int ch = a.res ? a.res[2 * gi] : a.def_ch, sp = a.res ? a.res[2 * gi + 1] : a.def_sp;
-    const MeshLayout L = layout_of(g, ch, sp);
-    vec3* Vx = reinterpret_cast<vec3*>(lds);
-    float* term = lds + 3 * a.nv_max;  // [12][64]
+    const MeshLayout L = layout_of(gl, ch, sp);
+    V3T<T>* Vx = reinterpret_cast<V3T<T>*>(lds_raw);
+    T* term = reinterpret_cast<T*>(lds_raw) + 3 * (size_t)a.nv_max;  // [12][64]
 
     // vertices, lane-parallel; extent by exact min/max reductions
-    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    T lo[3] = {T(3.4e38f), T(3.4e38f), T(3.4e38f)}, hi[3] = {T(-3.4e38f), T(-3.4e38f), T(-3.4e38f)};
     for (int v = lane; v < L.nv; v += 64) {
-        vec3 p = mesh_vertex(g, L, v);
+        V3T<T> p = mesh_vertex(g, L, v);
         Vx[v] = p;
-        lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
-        hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+        lo[0] = tmin(lo[0], p.x); lo[1] = tmin(lo[1], p.y); lo[2] = tmin(lo[2], p.z);
+        hi[0] = tmax(hi[0], p.x); hi[1] = tmax(hi[1], p.y); hi[2] = tmax(hi[2], p.z);
     }
     bool fits = true;
     for (int k = 0; k < 3; k++) {
-        float e;
+        T e;
         {
 #pragma clang fp contract(off)
-            e = wave_max(hi[k]) - wave_min(lo[k]);
+            e = wave_max_t(hi[k]) - wave_min_t(lo[k]);
         }
-        fits = fits && (e <= PR_PRINT_LIMIT);
+        fits = fits && (e <= T(PR_PRINT_LIMIT));
+        if (lane == 0) a.ext[3 * gi + k] = (float)e;
     }
     __syncthreads();
 
     // pass 1: mass and first moment; lanes 0..3 each own one serial accumulator chain
-    float acc = 0.0f;
+    T acc = T(0);
     for (int base = 0; base < L.nt; base += 64) {
         int t = base + lane;
         if (t < L.nt) {
             int id[3];
             tri_of(L, t, id);
-            float mi;
-            vec3 mct;
+            T mi;
+            V3T<T> mct;
             tri_mass_terms(Vx[id[0]], Vx[id[1]], Vx[id[2]], &mi, &mct);
             term[0 * 64 + lane] = mi; term[1 * 64 + lane] = mct.x; term[2 * 64 + lane] = mct.y; term[3 * 64 + lane] = mct.z;
         }
@@ -110,36 +124,35 @@ __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
         }
         __syncthreads();
     }
-    Mass mp;
-    vec3 c;
-    float m, mn;
+    V3T<T> c;
+    T m, mn;
     {
 #pragma clang fp contract(off)
         m = __shfl(acc, 0);
-        c = V(__shfl(acc, 1), __shfl(acc, 2), __shfl(acc, 3));
-        float shell = m;
-        vec3 np = V(0.5f * g[G_CHORD], 0, 0);
+        c = VT<T>(__shfl(acc, 1), __shfl(acc, 2), __shfl(acc, 3));
+        T shell = m;
+        V3T<T> np = VT<T>(T(0.5f) * g[G_CHORD], T(0), T(0));
         mn = g[G_NOSE_MASS] * shell;
         m += mn;
         c = vadd(c, vmul(np, mn));
-        c = vmul(c, 1.0f / m);
+        c = vmul(c, T(1) / m);
     }
     // pass 2: inertia about c; lanes 0..5 own I[0..5]
-    float Ik = 0.0f;
+    T Ik = T(0);
     for (int base = 0; base < L.nt; base += 64) {
         int t = base + lane;
         if (t < L.nt) {
             int id[3];
             tri_of(L, t, id);
-            float o[12];
+            T o[12];
             tri_inertia_terms(Vx[id[0]], Vx[id[1]], Vx[id[2]], c, o);
             for (int k = 0; k < 12; k++) term[k * 64 + lane] = o[k];
         }
         __syncthreads();
         if (lane < 6) {
             int cnt = min(64, L.nt - base);
-            const float* ta = term + lane * 64;
-            const float* tb = term + (lane + 6) * 64;
+            const T* ta = term + lane * 64;
+            const T* tb = term + (lane + 6) * 64;
             for (int k = 0; k < cnt; k++) {
 #pragma clang fp contract(off)
                 if (lane < 3) { Ik = Ik + ta[k]; Ik = Ik + tb[k]; }
@@ -148,24 +161,31 @@ __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
         }
         __syncthreads();
     }
+    float out[20];
     {
 #pragma clang fp contract(off)
-        float I[6];
+        T I[6];
         for (int k = 0; k < 6; k++) I[k] = __shfl(Ik, k);
-        vec3 r = vsub(V(0.5f * g[G_CHORD], 0, 0), c);
+        V3T<T> r = vsub(VT<T>(T(0.5f) * g[G_CHORD], T(0), T(0)), c);
         I[0] += mn * (r.y * r.y + r.z * r.z); I[1] += mn * (r.x * r.x + r.z * r.z);
         I[2] += mn * (r.x * r.x + r.y * r.y); I[3] -= mn * r.x * r.y;
         I[4] -= mn * r.x * r.z; I[5] -= mn * r.y * r.z;
-        const float reg = 1e-9f;
-        float xx = I[0] + reg, yy = I[1] + reg, zz = I[2] + reg;
-        invert_sym3(xx, yy, zz, I[3], I[4], I[5], false, mp.invI);
-        mp.I[0] = xx; mp.I[1] = yy; mp.I[2] = zz; mp.I[3] = I[3]; mp.I[4] = I[4]; mp.I[5] = I[5];
-        mp.mass = m;
-        mp.invmass = 1.0f / m;
-        mp.com[0] = c.x; mp.com[1] = c.y; mp.com[2] = c.z;
+        const T reg = T(1e-9f);
+        T xx = I[0] + reg, yy = I[1] + reg, zz = I[2] + reg;
+        T inv[9];
+        invert_sym3_t(xx, yy, zz, I[3], I[4], I[5], false, inv);
+        out[0] = (float)m;
+        out[1] = (float)(T(1) / m);
+        out[2] = (float)c.x; out[3] = (float)c.y; out[4] = (float)c.z;
+        out[5] = (float)xx; out[6] = (float)yy; out[7] = (float)zz;
+        out[8] = (float)I[3]; out[9] = (float)I[4]; out[10] = (float)I[5];
+        for (int k = 0; k < 9; k++) out[11 + k] = (float)inv[k];
     }
     const int nt_eff = fits ? L.nt : 0;
-    if (lane < 20) reinterpret_cast<float*>(a.mass + gi)[lane] = reinterpret_cast<const float*>(&mp)[lane];
+    float* mo = reinterpret_cast<float*>(a.mass + gi);
+#pragma unroll
+    for (int k = 0; k < 20; k++)
+        if (lane == k) mo[k] = out[k];
     if (lane == 0) {
         a.nt[gi] = nt_eff;
         a.nv[gi] = L.nv;
@@ -175,12 +195,12 @@ __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
     for (int t = lane; t < nt_eff; t += 64) {
         int id[3];
         tri_of(L, t, id);
-        vec3 cen, nrm;
-        float area;
+        V3T<T> cen, nrm;
+        T area;
         panel_of(Vx[id[0]], Vx[id[1]], Vx[id[2]], c, &cen, &area, &nrm);
         size_t o = (size_t)off + t;
-        a.pcx[o] = cen.x; a.pcy[o] = cen.y; a.pcz[o] = cen.z; a.pA[o] = area;
-        a.pnx[o] = nrm.x; a.pny[o] = nrm.y; a.pnz[o] = nrm.z;
+        a.pcx[o] = (float)cen.x; a.pcy[o] = (float)cen.y; a.pcz[o] = (float)cen.z; a.pA[o] = (float)area;
+        a.pnx[o] = (float)nrm.x; a.pny[o] = (float)nrm.y; a.pnz[o] = (float)nrm.z;
     }
 }
 
@@ -243,7 +263,7 @@ __global__ void k_init(QueueArgs a) {
         if (m == 0) continue;
         int leader = __ffsll((long long)m) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(a.count + k, __popcll(m));
+        if (lane == leader) base = atomicAdd(a.count + k * a.count_stride, __popcll(m));
         base = __shfl(base, leader);
         if (b == k) {
             int rank = __popcll(m & ((1ull << lane) - 1ull));
@@ -255,6 +275,11 @@ __global__ void k_init(QueueArgs a) {
 __global__ void k_zero_ints(int* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void k_zero_pair(int* p, int* q) {
+    if (threadIdx.x == 0) *p = 0;
+    if (threadIdx.x == 1) *q = 0;
 }
 
 // ------------------------------------------------------------------ k_fly
@@ -288,82 +313,63 @@ struct WindLane {  // this lane's share of the 7 gust terms
     int comp[NT];
 };
 
+// One group's flight: setup from the flight's constants/state/panels, then 600 Hz steps.
 template <int L, int P>
-__global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int j = lane & (L - 1);
-    const int gbase = lane & ~(L - 1);
-    const int n_in = *a.count_in;
+struct Flyer {
     Body B;
     WindLane<L> W;
     float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];
-    bool active = false, exhausted = false;
-    unsigned long long fsteps = 0, psteps = 0;
-    B.fid = -1; B.nt = 0; B.step = 0; B.end = 0;
+    bool bad;
 
-    for (;;) {
-        const bool need = !active && !exhausted;
-        if (__ballot(need)) {
-            if (need) {
-                int k = 0;
-                if (j == 0) k = atomicAdd(a.head, 1);
-                k = __shfl(k, gbase);
-                if (k >= n_in) {
-                    exhausted = true;
-                } else {
-                    // ---- set up flight fid (all L lanes of the group, identical values)
-                    const int fid = a.list_in[k];
-                    const FlightConst& fc = a.fc[fid];
-                    const FlightState& st = a.state[fid];
-                    const int gi = fc.genome;
-                    B.fid = fid;
-                    B.nt = a.nt[gi];
-                    const int off = a.off[gi];
-                    B.px = st.pos[0]; B.py = st.pos[1]; B.pz = st.pos[2];
-                    B.qw = st.q[0]; B.qx = st.q[1]; B.qy = st.q[2]; B.qz = st.q[3];
-                    B.vx = st.vel[0]; B.vy = st.vel[1]; B.vz = st.vel[2];
-                    B.wx = st.wb[0]; B.wy = st.wb[1]; B.wz = st.wb[2];
-                    B.step = st.step;
-                    B.t = B.step == 0 ? 0.0f : (float)B.step * PR_DT;  // chunk clock rule (sim.cu:113)
-                    B.end = min(B.step + a.chunk, a.max_steps);
-                    B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
-                    B.I0 = fc.I[0]; B.I1 = fc.I[1]; B.I2 = fc.I[2]; B.I3 = fc.I[3]; B.I4 = fc.I[4]; B.I5 = fc.I[5];
-                    B.v0 = fc.iv[0]; B.v1 = fc.iv[1]; B.v2 = fc.iv[2]; B.v4 = fc.iv[3]; B.v5 = fc.iv[4]; B.v8 = fc.iv[5];
-                    B.cf = fc.cf;
-                    B.bx = fc.base[0]; B.by = fc.base[1]; B.bz = fc.base[2];
-                    const float phase = fc.phase, gust = fc.gust, hr = fc.half_rho;
-                    const vec3 shift = V(fc.shift[0], fc.shift[1], fc.shift[2]);
-                    const float wp = fc.warp_p, wr = fc.warp_r;
+    __device__ __forceinline__ void setup(const FlyArgs& a, int fid, int j) {
+        const FlightConst& fc = a.fc[fid];
+        const FlightState& st = a.state[fid];
+        const int gi = fc.genome;
+        B.fid = fid;
+        B.nt = a.nt[gi];
+        const int off = a.off[gi];
+        B.px = st.pos[0]; B.py = st.pos[1]; B.pz = st.pos[2];
+        B.qw = st.q[0]; B.qx = st.q[1]; B.qy = st.q[2]; B.qz = st.q[3];
+        B.vx = st.vel[0]; B.vy = st.vel[1]; B.vz = st.vel[2];
+        B.wx = st.wb[0]; B.wy = st.wb[1]; B.wz = st.wb[2];
+        B.step = st.step;
+        B.t = B.step == 0 ? 0.0f : (float)B.step * PR_DT;  // chunk clock rule (sim.cu:113)
+        B.end = min(B.step + a.chunk, a.max_steps);
+        B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
+        B.I0 = fc.I[0]; B.I1 = fc.I[1]; B.I2 = fc.I[2]; B.I3 = fc.I[3]; B.I4 = fc.I[4]; B.I5 = fc.I[5];
+        B.v0 = fc.iv[0]; B.v1 = fc.iv[1]; B.v2 = fc.iv[2]; B.v4 = fc.iv[3]; B.v5 = fc.iv[4]; B.v8 = fc.iv[5];
+        B.cf = fc.cf;
+        B.bx = fc.base[0]; B.by = fc.base[1]; B.bz = fc.base[2];
+        const float phase = fc.phase, gust = fc.gust, hr = fc.half_rho;
+        const vec3 shift = V(fc.shift[0], fc.shift[1], fc.shift[2]);
+        const float wp = fc.warp_p, wr = fc.warp_r;
 #pragma unroll
-                    for (int m = 0; m < WindLane<L>::NT; m++) {
-                        int axis, comp;
-                        float ca, cb, cc, cw;
-                        wind_term(j + m * L, axis, ca, cb, cc, cw, comp);
-                        W.ax[m] = axis == 0 ? ca : 0.0f; W.ay[m] = axis == 1 ? ca : 0.0f; W.az[m] = axis == 2 ? ca : 0.0f;
-                        W.bt[m] = cb; W.cs[m] = cc * phase; W.wg[m] = cw * gust; W.comp[m] = comp;
-                    }
+        for (int m = 0; m < WindLane<L>::NT; m++) {
+            int axis, comp;
+            float ca, cb, cc, cw;
+            wind_term(j + m * L, axis, ca, cb, cc, cw, comp);
+            W.ax[m] = axis == 0 ? ca : 0.0f; W.ay[m] = axis == 1 ? ca : 0.0f; W.az[m] = axis == 2 ? ca : 0.0f;
+            W.bt[m] = cb; W.cs[m] = cc * phase; W.wg[m] = cw * gust; W.comp[m] = comp;
+        }
 #pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        const int t = j + p * L;
-                        if (t < B.nt) {
-                            const size_t o = (size_t)off + t;
-                            vec3 rb = vsub(V(a.pcx[o], a.pcy[o], a.pcz[o]), shift);
-                            vec3 nb = warped_normal(rb, V(a.pnx[o], a.pny[o], a.pnz[o]), wp, wr);
-                            rx[p] = rb.x; ry[p] = rb.y; rz[p] = rb.z;
-                            nx[p] = nb.x; ny[p] = nb.y; nz[p] = nb.z;
-                            kk[p] = hr * a.pA[o];
-                        } else {
-                            rx[p] = 0; ry[p] = 0; rz[p] = 0; nx[p] = 0; ny[p] = 0; nz[p] = 1; kk[p] = 0;
-                        }
-                    }
-                    active = true;
-                }
+        for (int p = 0; p < P; p++) {
+            const int t = j + p * L;
+            if (t < B.nt) {
+                const size_t o = (size_t)off + t;
+                vec3 rb = vsub(V(a.pcx[o], a.pcy[o], a.pcz[o]), shift);
+                vec3 nb = warped_normal(rb, V(a.pnx[o], a.pny[o], a.pnz[o]), wp, wr);
+                rx[p] = rb.x; ry[p] = rb.y; rz[p] = rb.z;
+                nx[p] = nb.x; ny[p] = nb.y; nz[p] = nb.z;
+                kk[p] = hr * a.pA[o];
+            } else {
+                rx[p] = 0; ry[p] = 0; rz[p] = 0; nx[p] = 0; ny[p] = 0; nz[p] = 1; kk[p] = 0;
             }
         }
-        if (!__ballot(active)) break;
-        if (!active) continue;
+    }
 
-        // ---------------------------------------------------------------- one 600 Hz step
+    // One step (sim.cu:129-205) for the group; returns true when the flight ended (ground,
+    // NaN, out of bounds or the step cap).
+    __device__ __forceinline__ bool step(int max_steps) {
         const float qw = B.qw, qx = B.qx, qy = B.qy, qz = B.qz;
         const float r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
         const float r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
@@ -401,7 +407,7 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
             const float h = 0.95f * sq + fabsf(sn);
             const float qs = qd * sn;
             const float c1 = qs * (cf - 2.0f * h);     // coefficient on n: -qd cn + qd cf s
-            const float c2 = -(qd * inv) * cf;           // coefficient on v_r: -qd cf / |v_r|
+            const float c2 = -(qd * inv) * cf;         // coefficient on v_r: -qd cf / |v_r|
             const float fx = c1 * nx[p] + c2 * vrx;
             const float fy = c1 * ny[p] + c2 * vry;
             const float fz = c1 * nz[p] + c2 * vrz;
@@ -448,29 +454,61 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
         B.px = npx; B.py = npy; B.pz = npz;
         B.t += PR_DT;
         B.step += 1;
-        fsteps += 1;
-        psteps += (unsigned long long)B.nt;
         // termination (sim.cu:198-204)
-        const bool bad = !(npx == npx && nvx == nvx);
+        bad = !(npx == npx && nvx == nvx);
         const bool ended = npz <= 0.0f || bad || fabsf(npy) > 300.f || npx < -50.f;
-        const bool capped = B.step >= a.max_steps;
-        if (ended || capped || B.step >= B.end) {
+        return ended || B.step >= max_steps;
+    }
+
+    __device__ __forceinline__ FlightState state(bool fin) const {
+        FlightState st;
+        st.pos[0] = B.px; st.pos[1] = B.py; st.pos[2] = B.pz;
+        st.q[0] = B.qw; st.q[1] = B.qx; st.q[2] = B.qy; st.q[3] = B.qz;
+        st.vel[0] = B.vx; st.vel[1] = B.vy; st.vel[2] = B.vz;
+        st.wb[0] = B.wx; st.wb[1] = B.wy; st.wb[2] = B.wz;
+        st.step = B.step; st.done = fin ? 1 : 0;
+        st.fitness = fin ? (bad ? -10.0f : B.px) : 0.0f;
+        return st;
+    }
+};
+
+template <int L, int P>
+__global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & (L - 1);
+    const int gbase = lane & ~(L - 1);
+    const int n_in = *a.count_in;
+    Flyer<L, P> F;
+    bool active = false, exhausted = false;
+    unsigned long long fsteps = 0, psteps = 0;
+    F.B.nt = 0;
+
+    for (;;) {
+        const bool need = !active && !exhausted;
+        if (__ballot(need)) {
+            if (need) {
+                int k = 0;
+                if (j == 0) k = atomicAdd(a.head, 1);
+                k = __shfl(k, gbase);
+                if (k >= n_in) exhausted = true;
+                else { F.setup(a, a.list_in[k], j); active = true; }
+            }
+        }
+        if (!__ballot(active)) break;
+        if (!active) continue;
+        const bool fin = F.step(a.max_steps);
+        fsteps += 1;
+        psteps += (unsigned long long)F.B.nt;
+        if (fin || F.B.step >= F.B.end) {
             if (j == 0) {
-                const bool fin = ended || capped;
-                FlightState st;
-                st.pos[0] = B.px; st.pos[1] = B.py; st.pos[2] = B.pz;
-                st.q[0] = B.qw; st.q[1] = B.qx; st.q[2] = B.qy; st.q[3] = B.qz;
-                st.vel[0] = B.vx; st.vel[1] = B.vy; st.vel[2] = B.vz;
-                st.wb[0] = B.wx; st.wb[1] = B.wy; st.wb[2] = B.wz;
-                st.step = B.step; st.done = fin ? 1 : 0;
-                st.fitness = fin ? (bad ? -10.0f : npx) : 0.0f;
-                a.state[B.fid] = st;
+                const FlightState st = F.state(fin);
+                a.state[F.B.fid] = st;
                 if (fin) {
-                    a.fitness[B.fid] = st.fitness;
-                    a.steps[B.fid] = B.step;
+                    a.fitness[F.B.fid] = st.fitness;
+                    a.steps[F.B.fid] = F.B.step;
                 } else {
                     int pos = atomicAdd(a.count_out, 1);
-                    a.list_out[pos] = B.fid;
+                    a.list_out[pos] = F.B.fid;
                 }
             }
             active = false;
@@ -485,6 +523,45 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     if (lane == 0 && fsteps) {
         atomicAdd(a.stats + 0, fsteps);
         atomicAdd(a.stats + 1, psteps);
+    }
+}
+
+// Pose recording (sim.cu:117-128): one group per flight over the whole horizon.  Frames
+// {pos, q, alive} at steps with step % rec_every == 0, index step0/rec_every + k within
+// each chunk (the reference's per-chunk index rule), capped at rec_cap; ended flights keep
+// writing their final pose with alive = 0.
+template <int L, int P>
+__global__ __launch_bounds__(64) void k_fly_rec(FlyArgs a, int n_flights, float* rec, int rec_every, int rec_cap) {
+    const int lane = threadIdx.x & 63;
+    const int j = lane & (L - 1);
+    const int fid = blockIdx.x * (64 / L) + lane / L;
+    if (fid >= n_flights) return;  // whole groups leave together
+    Flyer<L, P> F;
+    F.setup(a, fid, j);
+    bool done = a.state[fid].done != 0;
+    int rec_i = 0;
+    for (int st = 0; st < a.max_steps; st++) {
+        if (st % a.chunk == 0) { F.B.t = (float)st * PR_DT; rec_i = st / rec_every; }
+        if (st % rec_every == 0 && rec_i < rec_cap) {
+            if (j == 0) {
+                float* r = rec + ((size_t)fid * rec_cap + rec_i) * 8;
+                r[0] = F.B.px; r[1] = F.B.py; r[2] = F.B.pz;
+                r[3] = F.B.qw; r[4] = F.B.qx; r[5] = F.B.qy; r[6] = F.B.qz;
+                r[7] = done ? 0.0f : 1.0f;
+            }
+            rec_i++;
+        }
+        if (done) continue;
+        const bool fin = F.step(a.max_steps);
+        if (fin) {
+            done = true;
+            if (j == 0) {
+                const FlightState s = F.state(true);
+                a.state[fid] = s;
+                a.fitness[fid] = s.fitness;
+                a.steps[fid] = F.B.step;
+            }
+        }
     }
 }
 
@@ -516,10 +593,15 @@ hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, i
     return hipGetLastError();
 }
 
-hipError_t launch_prepare(const PrepareArgs& a, int n, hipStream_t s) {
+hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    size_t lds = sizeof(float) * (3 * (size_t)a.nv_max + 12 * 64);
-    hipLaunchKernelGGL(k_prepare, dim3(n), dim3(64), lds, s, a);
+    if (fp64) {
+        size_t lds = sizeof(double) * (3 * (size_t)a.nv_max + 12 * 64);
+        hipLaunchKernelGGL(k_prepare<double>, dim3(n), dim3(64), lds, s, a);
+    } else {
+        size_t lds = sizeof(float) * (3 * (size_t)a.nv_max + 12 * 64);
+        hipLaunchKernelGGL(k_prepare<float>, dim3(n), dim3(64), lds, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -532,6 +614,11 @@ hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, f
 hipError_t launch_queue(const QueueArgs& a, hipStream_t s) {
     if (a.n_flights <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_init, dim3((a.n_flights + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_pair(int* p, int* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_pair, dim3(1), dim3(64), 0, s, p, q);
     return hipGetLastError();
 }
 
@@ -572,6 +659,16 @@ hipError_t launch_fly(int L, int P, const FlyArgs& a, int grid, hipStream_t s) {
 #define DISP(LL, PP) if (L == LL && P == PP) return fly_one<LL, PP>(a, grid, s);
     PR_VARIANTS(DISP)
 #undef DISP
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fly_rec(int L, int P, const FlyArgs& a, int n_flights, float* rec, int rec_every, int rec_cap,
+                          hipStream_t s) {
+    const int grid = (n_flights * L + 63) / 64;
+#define DISPR(LL, PP) \
+    if (L == LL && P == PP) { hipLaunchKernelGGL((k_fly_rec<LL, PP>), dim3(grid), dim3(64), 0, s, a, n_flights, rec, rec_every, rec_cap); return hipGetLastError(); }
+    PR_VARIANTS(DISPR)
+#undef DISPR
     return hipErrorInvalidValue;
 }
 

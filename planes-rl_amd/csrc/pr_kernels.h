@@ -15,6 +15,7 @@ struct PrepareArgs {
     Mass* mass;            // [n]
     int* nt;               // [n] triangles flown (0 if outside the print volume)
     int* nv;               // [n]
+    float* ext;            // [n*3] bounding-box extent of the mesh
     float *pcx, *pcy, *pcz, *pA, *pnx, *pny, *pnz;  // panel SoA at off[g] + t
 };
 
@@ -30,7 +31,8 @@ struct QueueArgs {
     FlightConst* fc;
     int n_buckets;
     const int* bucket_nt;  // device [n_buckets] upper triangle count of each bucket
-    int* count;            // [n_buckets]
+    int* count;            // bucket b's list count at count[b * count_stride]
+    int count_stride;
     int* list;             // [n_buckets * list_stride]
     int list_stride;
     FlightState* state;
@@ -57,12 +59,15 @@ struct FlyArgs {
 };
 
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
-hipError_t launch_prepare(const PrepareArgs& a, int n, hipStream_t s);
+hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);
 hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, float* out, hipStream_t s);
 hipError_t launch_queue(const QueueArgs& a, hipStream_t s);
 hipError_t launch_zero(int* p, int n, hipStream_t s);
+hipError_t launch_zero_pair(int* p, int* q, hipStream_t s);
 hipError_t launch_fly(int L, int P, const FlyArgs& a, int grid, hipStream_t s);
 hipError_t fly_occupancy(int L, int P, int* blocks_per_cu);
+hipError_t launch_fly_rec(int L, int P, const FlyArgs& a, int n_flights, float* rec, int rec_every, int rec_cap,
+                          hipStream_t s);
 int fly_variant_count();
 int fly_variant_info(int i, int* L, int* P);
 hipError_t launch_scores(const float* samples, int pop, int nseeds, float* scores, hipStream_t s);

@@ -36,29 +36,38 @@ enum Field {
 #define PR_DT (1.0f / 600.0f)
 #define PR_GRAV 9.81f
 
-struct vec3 { float x, y, z; };
+template <class T>
+struct V3T { T x, y, z; };
+typedef V3T<float> vec3;
 PR_HD vec3 V(float x, float y, float z) { vec3 a; a.x = x; a.y = y; a.z = z; return a; }
+template <class T>
+PR_HD V3T<T> VT(T x, T y, T z) { V3T<T> a; a.x = x; a.y = y; a.z = z; return a; }
 // Contraction is decided per function body, so every helper used by an exact path
 // carries its own pragma (the library is built with -ffp-contract=fast-honor-pragmas).
-PR_HD vec3 vadd(vec3 a, vec3 b) {
+template <class T>
+PR_HD V3T<T> vadd(V3T<T> a, V3T<T> b) {
 #pragma clang fp contract(off)
-    return V(a.x + b.x, a.y + b.y, a.z + b.z);
+    return VT<T>(a.x + b.x, a.y + b.y, a.z + b.z);
 }
-PR_HD vec3 vsub(vec3 a, vec3 b) {
+template <class T>
+PR_HD V3T<T> vsub(V3T<T> a, V3T<T> b) {
 #pragma clang fp contract(off)
-    return V(a.x - b.x, a.y - b.y, a.z - b.z);
+    return VT<T>(a.x - b.x, a.y - b.y, a.z - b.z);
 }
-PR_HD vec3 vmul(vec3 a, float s) {
+template <class T>
+PR_HD V3T<T> vmul(V3T<T> a, T s) {
 #pragma clang fp contract(off)
-    return V(a.x * s, a.y * s, a.z * s);
+    return VT<T>(a.x * s, a.y * s, a.z * s);
 }
-PR_HD float vdot(vec3 a, vec3 b) {
+template <class T>
+PR_HD T vdot(V3T<T> a, V3T<T> b) {
 #pragma clang fp contract(off)
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
-PR_HD vec3 vcross(vec3 a, vec3 b) {
+template <class T>
+PR_HD V3T<T> vcross(V3T<T> a, V3T<T> b) {
 #pragma clang fp contract(off)
-    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+    return VT<T>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
 struct quat { float w, x, y, z; };
@@ -476,109 +485,136 @@ PR_HD void tri_of(const MeshLayout& L, int t, int* a3) {
     }
 }
 
-// exact: camber line (common.h:124-129)
-PR_HD float camber_line(const float* g, float xc) {
+// ------------------------------------------------------------------ geometry in T (float or double)
+// The float instantiation reproduces the reference's float rounding; the double one is
+// the FP64 prepare mode (model constants stay the reference's float literals).
+PR_HD float tsin(float x) { return sinf_cr(x); }
+PR_HD float tcos(float x) { return cosf_cr(x); }
+PR_HD float ttan(float x) { return tanf_cr(x); }
+PR_HD double tsin(double x) { double s, c; sincos_d(x, &s, &c); return s; }
+PR_HD double tcos(double x) { double s, c; sincos_d(x, &s, &c); return c; }
+PR_HD double ttan(double x) {
 #pragma clang fp contract(off)
-    float te = g[G_ELEVATOR] * 0.22f;
-    float u = xc, v = 1.0f - u;
-    return 3 * v * v * u * g[G_CAMBER1] + 3 * v * u * u * g[G_CAMBER2] + u * u * u * te;
+    double s, c;
+    sincos_d(x, &s, &c);
+    return s / c;
+}
+PR_HD float tsqrt(float x) { return sqrtf(x); }
+PR_HD double tsqrt(double x) { return sqrt(x); }
+PR_HD float tmin(float a, float b) { return fminf(a, b); }
+PR_HD double tmin(double a, double b) { return fmin(a, b); }
+PR_HD float tmax(float a, float b) { return fmaxf(a, b); }
+PR_HD double tmax(double a, double b) { return fmax(a, b); }
+PR_HD float tabs(float a) { return fabsf(a); }
+PR_HD double tabs(double a) { return fabs(a); }
+
+// exact: camber line (common.h:124-129)
+template <class T>
+PR_HD T camber_line(const T* g, T xc) {
+#pragma clang fp contract(off)
+    T te = g[G_ELEVATOR] * T(0.22f);
+    T u = xc, v = T(1) - u;
+    return T(3) * v * v * u * g[G_CAMBER1] + T(3) * v * u * u * g[G_CAMBER2] + u * u * u * te;
 }
 
 // exact: wing surface point (common.h:132-172)
-PR_HD vec3 wing_point(const float* g, float u, float xc, float zoff, float scale) {
+template <class T>
+PR_HD V3T<T> wing_point(const T* g, T u, T xc, T zoff, T scale) {
 #pragma clang fp contract(off)
-    float half = g[G_SPAN] * 0.5f * scale;
-    float t = fabsf(u) * half;
+    T half = g[G_SPAN] * T(0.5f) * scale;
+    T t = tabs(u) * half;
     // dihedral polyline (common.h:132-141)
-    float hs = g[G_SPAN] * 0.5f;
-    float b = g[G_BODY_FRAC] * hs, k = g[G_KINK] * hs;
+    T hs = g[G_SPAN] * T(0.5f);
+    T b = g[G_BODY_FRAC] * hs, k = g[G_KINK] * hs;
     if (k < b) k = b;
-    float yy = 0, zz = 0, seg;
-    seg = fminf(t, b); yy += seg;
-    seg = fminf(t, k) - b;
-    if (seg > 0) { yy += seg * cosf_cr(g[G_DIH_IN]); zz += seg * sinf_cr(g[G_DIH_IN]); }
+    T yy = 0, zz = 0, seg;
+    seg = tmin(t, b); yy += seg;
+    seg = tmin(t, k) - b;
+    if (seg > 0) { yy += seg * tcos(g[G_DIH_IN]); zz += seg * tsin(g[G_DIH_IN]); }
     seg = t - k;
-    if (seg > 0) { yy += seg * cosf_cr(g[G_DIH_OUT]); zz += seg * sinf_cr(g[G_DIH_OUT]); }
-    float tf = fabsf(u);
-    float v = 1.0f - tf;
-    float cr = 1.0f + g[G_CENTER_CHORD];
-    float ct = g[G_TAPER];
-    float c1 = 1.0f + g[G_CHORD_B1], c2 = ct + g[G_CHORD_B2];
-    float ratio = v * v * v * cr + 3 * v * v * tf * c1 + 3 * v * tf * tf * c2 + tf * tf * tf * ct;
-    float ell = sqrtf(fmaxf(0.08f, 1.0f - 0.92f * tf * tf));
-    ratio *= 1.0f - g[G_TIP_ROUND] + g[G_TIP_ROUND] * ell;
-    ratio = fmaxf(0.035f, fminf(1.9f, ratio));
-    float c = g[G_CHORD] * scale * ratio;
-    float xr = 0.5f * g[G_CHORD] * scale * cr;
-    float xt = 0.5f * g[G_CHORD] * scale - tanf_cr(g[G_SWEEP]) * half;
-    float xb1 = xr + g[G_LE_B1] * g[G_CHORD] * scale;
-    float xb2 = xt + g[G_LE_B2] * g[G_CHORD] * scale;
-    float xle = v * v * v * xr + 3 * v * v * tf * xb1 + 3 * v * tf * tf * xb2 + tf * tf * tf * xt;
-    float x = xle - xc * c;
-    float zc = zz * scale + camber_line(g, xc) * c;
-    float xq = xle - 0.25f * c;
-    float tw = g[G_WASHOUT] * tf;
+    if (seg > 0) { yy += seg * tcos(g[G_DIH_OUT]); zz += seg * tsin(g[G_DIH_OUT]); }
+    T tf = tabs(u);
+    T v = T(1) - tf;
+    T cr = T(1) + g[G_CENTER_CHORD];
+    T ct = g[G_TAPER];
+    T c1 = T(1) + g[G_CHORD_B1], c2 = ct + g[G_CHORD_B2];
+    T ratio = v * v * v * cr + T(3) * v * v * tf * c1 + T(3) * v * tf * tf * c2 + tf * tf * tf * ct;
+    T ell = tsqrt(tmax(T(0.08f), T(1) - T(0.92f) * tf * tf));
+    ratio *= T(1) - g[G_TIP_ROUND] + g[G_TIP_ROUND] * ell;
+    ratio = tmax(T(0.035f), tmin(T(1.9f), ratio));
+    T c = g[G_CHORD] * scale * ratio;
+    T xr = T(0.5f) * g[G_CHORD] * scale * cr;
+    T xt = T(0.5f) * g[G_CHORD] * scale - ttan(g[G_SWEEP]) * half;
+    T xb1 = xr + g[G_LE_B1] * g[G_CHORD] * scale;
+    T xb2 = xt + g[G_LE_B2] * g[G_CHORD] * scale;
+    T xle = v * v * v * xr + T(3) * v * v * tf * xb1 + T(3) * v * tf * tf * xb2 + tf * tf * tf * xt;
+    T x = xle - xc * c;
+    T zc = zz * scale + camber_line(g, xc) * c;
+    T xq = xle - T(0.25f) * c;
+    T tw = g[G_WASHOUT] * tf;
     zc += tw * (x - xq);
-    return V(x, (u < 0 ? -yy : yy), zc + zoff);
+    return VT<T>(x, (u < 0 ? -yy : yy), zc + zoff);
 }
 
 // exact: vertex k (0..3) of a fin rooted at (y0, z0) (common.h:197-213)
-PR_HD vec3 fin_vertex(const float* g, float y0, float z0, int side, int k) {
+template <class T>
+PR_HD V3T<T> fin_vertex(const T* g, T y0, T z0, int side, int k) {
 #pragma clang fp contract(off)
-    float fc = g[G_CHORD] * g[G_FIN_CHORD];
-    float sw = fc * g[G_FIN_SWEEP];
-    float xr = -0.32f * g[G_CHORD] - g[G_TAIL_CHORD] * g[G_CHORD] * 0.82f;
-    float tipc = fc * g[G_FIN_TAPER];
-    float cant = side == 0 ? 0.0f : g[G_FIN_CANT];
-    float ya = y0 + (float)side * sinf_cr(cant) * g[G_FIN_HEIGHT];
-    float za = z0 + cosf_cr(cant) * g[G_FIN_HEIGHT];
-    if (k == 0) return V(xr + 0.5f * fc, y0, z0);
-    if (k == 1) return V(xr - 0.5f * fc, y0, z0);
-    if (k == 2) return V(xr - sw + 0.5f * tipc, ya, za);
-    return V(xr - sw - 0.5f * tipc, ya, za);
+    T fc = g[G_CHORD] * g[G_FIN_CHORD];
+    T sw = fc * g[G_FIN_SWEEP];
+    T xr = T(-0.32f) * g[G_CHORD] - g[G_TAIL_CHORD] * g[G_CHORD] * T(0.82f);
+    T tipc = fc * g[G_FIN_TAPER];
+    T cant = side == 0 ? T(0) : g[G_FIN_CANT];
+    T ya = y0 + T(side) * tsin(cant) * g[G_FIN_HEIGHT];
+    T za = z0 + tcos(cant) * g[G_FIN_HEIGHT];
+    if (k == 0) return VT<T>(xr + T(0.5f) * fc, y0, z0);
+    if (k == 1) return VT<T>(xr - T(0.5f) * fc, y0, z0);
+    if (k == 2) return VT<T>(xr - sw + T(0.5f) * tipc, ya, za);
+    return VT<T>(xr - sw - T(0.5f) * tipc, ya, za);
 }
 
 // exact: vertex v of the mesh (build_mesh emission order, common.h:257-292)
-PR_HD vec3 mesh_vertex(const float* g, const MeshLayout& L, int v) {
+template <class T>
+PR_HD V3T<T> mesh_vertex(const T* g, const MeshLayout& L, int v) {
 #pragma clang fp contract(off)
     if (v < L.v_tail || (L.bi && v >= L.v_deck2 && v < L.v_keel)) {
         bool top = v >= L.v_deck2;
         int k = top ? v - L.v_deck2 : v;
         int i = k / (L.ch + 1), j = k - i * (L.ch + 1);
-        float u = (float)(i - L.sp) / (float)L.sp;
-        float xc = (float)j / (float)L.ch;
-        return top ? wing_point(g, u, xc, g[G_DECK_GAP], g[G_DECK_SCALE]) : wing_point(g, u, xc, 0.0f, 1.0f);
+        T u = T((float)(i - L.sp) / (float)L.sp);
+        T xc = T((float)j / (float)L.ch);
+        return top ? wing_point(g, u, xc, g[G_DECK_GAP], g[G_DECK_SCALE]) : wing_point(g, u, xc, T(0), T(1));
     }
     if (v < L.v_fin) {  // horizontal tail (common.h:215-236)
         int k = v - L.v_tail, i = k / 3, j = k - 3 * i;
-        float hs = 0.5f * g[G_SPAN] * g[G_TAIL_SPAN];
-        float hc = g[G_CHORD] * g[G_TAIL_CHORD];
-        float x0 = -0.32f * g[G_CHORD];
-        float z0 = camber_line(g, 0.82f) * g[G_CHORD];
-        float u = (float)(i - 1), au = fabsf(u);
-        float tc = hc * (1.0f + (g[G_TAIL_TAPER] - 1.0f) * au);
-        float xle = x0 - g[G_TAIL_SWEEP] * hc * au;
-        float xc = 0.5f * (float)j;
-        float x = xle - xc * tc;
-        float z = z0 + g[G_TAIL_ANGLE] * (x - xle) + au * hs * sinf_cr(g[G_TAIL_DIHEDRAL]);
-        return V(x, u * hs * cosf_cr(g[G_TAIL_DIHEDRAL]), z);
+        T hs = T(0.5f) * g[G_SPAN] * g[G_TAIL_SPAN];
+        T hc = g[G_CHORD] * g[G_TAIL_CHORD];
+        T x0 = T(-0.32f) * g[G_CHORD];
+        T z0 = camber_line(g, T(0.82f)) * g[G_CHORD];
+        T u = T((float)(i - 1)), au = tabs(u);
+        T tc = hc * (T(1) + (g[G_TAIL_TAPER] - T(1)) * au);
+        T xle = x0 - g[G_TAIL_SWEEP] * hc * au;
+        T xc = T(0.5f * (float)j);
+        T x = xle - xc * tc;
+        T z = z0 + g[G_TAIL_ANGLE] * (x - xle) + au * hs * tsin(g[G_TAIL_DIHEDRAL]);
+        return VT<T>(x, u * hs * tcos(g[G_TAIL_DIHEDRAL]), z);
     }
     if (v < L.v_deck2) {  // fins (common.h:238-254)
         int k = v - L.v_fin, f = k >> 2, c = k & 3;
-        float hs = 0.5f * g[G_SPAN] * g[G_TAIL_SPAN];
-        float z0 = camber_line(g, 0.82f) * g[G_CHORD];
-        float y0, zr;
+        T hs = T(0.5f) * g[G_SPAN] * g[G_TAIL_SPAN];
+        T z0 = camber_line(g, T(0.82f)) * g[G_CHORD];
+        T y0, zr;
         int side;
         if (L.fin_kind == 3) {
-            float yo = hs * 0.94f * cosf_cr(g[G_TAIL_DIHEDRAL]);
-            float zo = z0 + hs * 0.94f * sinf_cr(g[G_TAIL_DIHEDRAL]);
+            T yo = hs * T(0.94f) * tcos(g[G_TAIL_DIHEDRAL]);
+            T zo = z0 + hs * T(0.94f) * tsin(g[G_TAIL_DIHEDRAL]);
             side = f == 0 ? -1 : 1; y0 = f == 0 ? -yo : yo; zr = zo;
         } else {
             int pf = L.fin_kind == 2 ? f - 1 : (L.fin_kind == 1 ? f : -1);  // -1: centre fin
-            if (pf < 0) { side = 0; y0 = 0.0f; zr = z0; }
+            if (pf < 0) { side = 0; y0 = T(0); zr = z0; }
             else {
-                float yp = hs * g[G_FIN_SPAN_POS] * cosf_cr(g[G_TAIL_DIHEDRAL]);
-                float zp = z0 + hs * g[G_FIN_SPAN_POS] * sinf_cr(g[G_TAIL_DIHEDRAL]);
+                T yp = hs * g[G_FIN_SPAN_POS] * tcos(g[G_TAIL_DIHEDRAL]);
+                T zp = z0 + hs * g[G_FIN_SPAN_POS] * tsin(g[G_TAIL_DIHEDRAL]);
                 side = pf == 0 ? -1 : 1; y0 = pf == 0 ? -yp : yp; zr = zp;
             }
         }
@@ -586,25 +622,26 @@ PR_HD vec3 mesh_vertex(const float* g, const MeshLayout& L, int v) {
     }
     if (v < L.v_strut) {  // keel sheet (common.h:264-277)
         int k = v - L.v_keel, j = k >> 1;
-        float xc = (float)j / 3;
-        vec3 top = wing_point(g, 0.0f, xc, 0.0f, 1.0f);
+        T xc = T((float)j / 3.0f);
+        V3T<T> top = wing_point(g, T(0), xc, T(0), T(1));
         if (!(k & 1)) return top;
-        return V(top.x, 0.0f, top.z - g[G_KEEL_H] * (0.35f + 0.65f * xc));
+        return VT<T>(top.x, T(0), top.z - g[G_KEEL_H] * (T(0.35f) + T(0.65f) * xc));
     }
     // biplane struts at the kink stations (common.h:279-290)
     int k = v - L.v_strut, s = k >> 2, c = k & 3;
-    float u = (float)(s == 0 ? -1 : 1) * g[G_KINK];
-    float xc = (c & 1) ? 0.55f : 0.25f;
-    return c < 2 ? wing_point(g, u, xc, 0.0f, 1.0f) : wing_point(g, u, xc, g[G_DECK_GAP], g[G_DECK_SCALE]);
+    T u = T(s == 0 ? -1 : 1) * g[G_KINK];
+    T xc = (c & 1) ? T(0.55f) : T(0.25f);
+    return c < 2 ? wing_point(g, u, xc, T(0), T(1)) : wing_point(g, u, xc, g[G_DECK_GAP], g[G_DECK_SCALE]);
 }
 
 // exact: first-pass terms of one triangle (common.h:313-319): mass and mass-weighted centroid
-PR_HD void tri_mass_terms(vec3 a, vec3 b, vec3 d, float* mi, vec3* mct) {
+template <class T>
+PR_HD void tri_mass_terms(V3T<T> a, V3T<T> b, V3T<T> d, T* mi, V3T<T>* mct) {
 #pragma clang fp contract(off)
-    vec3 cr = vcross(vsub(b, a), vsub(d, a));
-    float A = 0.5f * sqrtf(vdot(cr, cr));
-    float m = A * PR_AREAL;
-    vec3 ct = vmul(vadd(vadd(a, b), d), 1.0f / 3);
+    V3T<T> cr = vcross(vsub(b, a), vsub(d, a));
+    T A = T(0.5f) * tsqrt(vdot(cr, cr));
+    T m = A * T(PR_AREAL);
+    V3T<T> ct = vmul(vadd(vadd(a, b), d), T(1.0f / 3));
     *mi = m;
     *mct = vmul(ct, m);
 }
@@ -612,39 +649,52 @@ PR_HD void tri_mass_terms(vec3 a, vec3 b, vec3 d, float* mi, vec3* mct) {
 // exact: second-pass inertia terms of one triangle about c (common.h:329-351).
 // out[0..5]: lamina terms (added to xx,yy,zz / subtracted from xy,xz,yz),
 // out[6..11]: through-thickness terms, same signs.
-PR_HD void tri_inertia_terms(vec3 a, vec3 b, vec3 d, vec3 c, float* out) {
+template <class T>
+PR_HD void tri_inertia_terms(V3T<T> a, V3T<T> b, V3T<T> d, V3T<T> c, T* out) {
 #pragma clang fp contract(off)
-    vec3 cr = vcross(vsub(b, a), vsub(d, a));
-    float A = 0.5f * sqrtf(vdot(cr, cr));
-    float mi = A * PR_AREAL;
-    vec3 ra = vsub(a, c), rb = vsub(b, c), rd = vsub(d, c);
-    float exx = (ra.x * ra.x + rb.x * rb.x + rd.x * rd.x + ra.x * rb.x + ra.x * rd.x + rb.x * rd.x) / 6.0f;
-    float eyy = (ra.y * ra.y + rb.y * rb.y + rd.y * rd.y + ra.y * rb.y + ra.y * rd.y + rb.y * rd.y) / 6.0f;
-    float ezz = (ra.z * ra.z + rb.z * rb.z + rd.z * rd.z + ra.z * rb.z + ra.z * rd.z + rb.z * rd.z) / 6.0f;
-    float exy = (2 * (ra.x * ra.y + rb.x * rb.y + rd.x * rd.y) + ra.x * rb.y + rb.x * ra.y + ra.x * rd.y +
-                 rd.x * ra.y + rb.x * rd.y + rd.x * rb.y) / 12.0f;
-    float exz = (2 * (ra.x * ra.z + rb.x * rb.z + rd.x * rd.z) + ra.x * rb.z + rb.x * ra.z + ra.x * rd.z +
-                 rd.x * ra.z + rb.x * rd.z + rd.x * rb.z) / 12.0f;
-    float eyz = (2 * (ra.y * ra.z + rb.y * rb.z + rd.y * rd.z) + ra.y * rb.z + rb.y * ra.z + ra.y * rd.z +
-                 rd.y * ra.z + rb.y * rd.z + rd.y * rb.z) / 12.0f;
+    V3T<T> cr = vcross(vsub(b, a), vsub(d, a));
+    T A = T(0.5f) * tsqrt(vdot(cr, cr));
+    T mi = A * T(PR_AREAL);
+    V3T<T> ra = vsub(a, c), rb = vsub(b, c), rd = vsub(d, c);
+    T exx = (ra.x * ra.x + rb.x * rb.x + rd.x * rd.x + ra.x * rb.x + ra.x * rd.x + rb.x * rd.x) / T(6);
+    T eyy = (ra.y * ra.y + rb.y * rb.y + rd.y * rd.y + ra.y * rb.y + ra.y * rd.y + rb.y * rd.y) / T(6);
+    T ezz = (ra.z * ra.z + rb.z * rb.z + rd.z * rd.z + ra.z * rb.z + ra.z * rd.z + rb.z * rd.z) / T(6);
+    T exy = (T(2) * (ra.x * ra.y + rb.x * rb.y + rd.x * rd.y) + ra.x * rb.y + rb.x * ra.y + ra.x * rd.y +
+             rd.x * ra.y + rb.x * rd.y + rd.x * rb.y) / T(12);
+    T exz = (T(2) * (ra.x * ra.z + rb.x * rb.z + rd.x * rd.z) + ra.x * rb.z + rb.x * ra.z + ra.x * rd.z +
+             rd.x * ra.z + rb.x * rd.z + rd.x * rb.z) / T(12);
+    T eyz = (T(2) * (ra.y * ra.z + rb.y * rb.z + rd.y * rd.z) + ra.y * rb.z + rb.y * ra.z + ra.y * rd.z +
+             rd.y * ra.z + rb.y * rd.z + rd.y * rb.z) / T(12);
     out[0] = mi * (eyy + ezz); out[1] = mi * (exx + ezz); out[2] = mi * (exx + eyy);
     out[3] = mi * exy; out[4] = mi * exz; out[5] = mi * eyz;
-    float l = sqrtf(vdot(cr, cr));
-    vec3 nn = l > 1e-12f ? vmul(cr, 1.0f / l) : V(0, 0, 1);
-    float th = mi * PR_WALL * PR_WALL / 12.0f;
-    out[6] = th * (1.0f - nn.x * nn.x); out[7] = th * (1.0f - nn.y * nn.y); out[8] = th * (1.0f - nn.z * nn.z);
+    T l = tsqrt(vdot(cr, cr));
+    V3T<T> nn = l > T(1e-12f) ? vmul(cr, T(1) / l) : VT<T>(0, 0, 1);
+    T th = mi * T(PR_WALL) * T(PR_WALL) / T(12);
+    out[6] = th * (T(1) - nn.x * nn.x); out[7] = th * (T(1) - nn.y * nn.y); out[8] = th * (T(1) - nn.z * nn.z);
     out[9] = th * nn.x * nn.y; out[10] = th * nn.x * nn.z; out[11] = th * nn.y * nn.z;
 }
 
 // exact: panel record of one triangle relative to the shell COM (sim.cu:41-47)
-PR_HD void panel_of(vec3 va, vec3 vb, vec3 vc, vec3 com, vec3* cen, float* area, vec3* nrm) {
+template <class T>
+PR_HD void panel_of(V3T<T> va, V3T<T> vb, V3T<T> vc, V3T<T> com, V3T<T>* cen, T* area, V3T<T>* nrm) {
 #pragma clang fp contract(off)
-    vec3 a = vsub(va, com), b = vsub(vb, com), c = vsub(vc, com);
-    vec3 n = vcross(vsub(b, a), vsub(c, a));
-    float A = 0.5f * sqrtf(vdot(n, n));
-    *cen = vmul(vadd(vadd(a, b), c), 1.0f / 3);
+    V3T<T> a = vsub(va, com), b = vsub(vb, com), c = vsub(vc, com);
+    V3T<T> n = vcross(vsub(b, a), vsub(c, a));
+    T A = T(0.5f) * tsqrt(vdot(n, n));
+    *cen = vmul(vadd(vadd(a, b), c), T(1.0f / 3));
     *area = A;
-    *nrm = A > 1e-12f ? vmul(n, 0.5f / A) : V(0, 0, 1);
+    *nrm = A > T(1e-12f) ? vmul(n, T(0.5f) / A) : VT<T>(0, 0, 1);
+}
+
+// exact: symmetric 3x3 inverse in T (common.h:356-362 / 429-436)
+template <class T>
+PR_HD void invert_sym3_t(T xx, T yy, T zz, T xy, T xz, T yz, bool floor_det, T* inv) {
+#pragma clang fp contract(off)
+    T det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+    T id = T(1) / (floor_det ? tmax(det, T(1e-24f)) : det);
+    inv[0] = (yy * zz - yz * yz) * id; inv[1] = (xz * yz - xy * zz) * id; inv[2] = (xy * yz - xz * yy) * id;
+    inv[3] = inv[1]; inv[4] = (xx * zz - xz * xz) * id; inv[5] = (xy * xz - xx * yz) * id;
+    inv[6] = inv[2]; inv[7] = inv[5]; inv[8] = (xx * yy - xy * xy) * id;
 }
 
 }  // namespace pr

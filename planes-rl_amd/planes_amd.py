@@ -34,9 +34,23 @@ class SimParams(C.Structure):
         return cls(1, gust_rms_max, np.float32(np.deg2rad(warp_max_deg)), wind_sd, max_steps, chunk, 0.0)
 
 
+NB_MAX = 8
+
+
 class Stats(C.Structure):
     _fields_ = [("flight_steps", C.c_ulonglong), ("panel_steps", C.c_ulonglong), ("fly_ms", C.c_double),
-                ("fly_launches", C.c_int), ("prep_ms", C.c_double)]
+                ("fly_launches", C.c_int), ("prep_ms", C.c_double), ("n_buckets", C.c_int),
+                ("b_nt_max", C.c_int * NB_MAX), ("b_L", C.c_int * NB_MAX), ("b_P", C.c_int * NB_MAX),
+                ("b_launches", C.c_int * NB_MAX), ("b_flight_steps", C.c_ulonglong * NB_MAX),
+                ("b_panel_steps", C.c_ulonglong * NB_MAX), ("b_fly_ms", C.c_double * NB_MAX)]
+
+    def as_dict(self):
+        nb = self.n_buckets
+        d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms")}
+        d["buckets"] = [{"nt_max": self.b_nt_max[i], "L": self.b_L[i], "P": self.b_P[i],
+                         "launches": self.b_launches[i], "flight_steps": self.b_flight_steps[i],
+                         "panel_steps": self.b_panel_steps[i], "fly_ms": self.b_fly_ms[i]} for i in range(nb)]
+        return d
 
 
 class PlanesError(RuntimeError):
@@ -68,6 +82,32 @@ def _load():
     lib.pr_h2d.argtypes = [VP, VP, C.c_size_t]
     lib.pr_d2h.argtypes = [VP, VP, C.c_size_t]
     lib.pr_device_name.argtypes = [C.c_char_p, C.c_int]
+    lib.pr_fly_record.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, VP, C.c_int, C.c_int]
+    lib.pr_read_extent.argtypes = [VP, C.c_int, F32P]
+    lib.pr_set_graph.argtypes = [VP, C.c_int]
+    lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
+    lib.pr_gene_bounds.argtypes = [F32P, F32P]
+    lib.pr_gene_bounds.restype = None
+    lib.pr_ga_new.argtypes = [C.c_uint]
+    lib.pr_ga_new.restype = VP
+    lib.pr_ga_free.argtypes = [VP]
+    lib.pr_ga_free.restype = None
+    lib.pr_ga_random.argtypes = [VP, F32P, C.c_int]
+    lib.pr_ga_random.restype = None
+    lib.pr_ga_sanitize.argtypes = [F32P, C.c_int]
+    lib.pr_ga_sanitize.restype = None
+    lib.pr_ga_rank.argtypes = [F32P, C.c_int, I32P]
+    lib.pr_ga_rank.restype = None
+    lib.pr_ga_breed.argtypes = [VP, F32P, F32P, I32P, C.c_int, C.c_int, C.c_int, F32P]
+    lib.pr_ga_breed.restype = None
+    lib.pr_robust_score_host.argtypes = [F32P, C.c_int]
+    lib.pr_robust_score_host.restype = C.c_float
+    lib.pr_gen_create.argtypes = [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.pr_gen_destroy.argtypes = [VP]
+    lib.pr_gen_destroy.restype = None
+    lib.pr_gen_evaluate.argtypes = [VP, VP, VP, C.c_int, I32P, C.c_int, C.c_int, C.POINTER(SimParams), F32P, I32P,
+                                    VP, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+    lib.pr_gen_stats.argtypes = [VP, C.POINTER(Stats), C.POINTER(Stats)]
     return lib
 
 
@@ -196,6 +236,21 @@ class Planes:
         nf = self.fly_device(layout, n_genomes, n_scen, params)
         return self.d_fit.download(np.float32, nf), self.d_steps.download(np.int32, nf)
 
+    def fly_record(self, layout, n_genomes, n_scen, params, rec_every, rec_cap):
+        """gpu_sim with d_rec (sim.cu:117-128): returns fitness, steps, frames[n, rec_cap, 8]."""
+        nf = n_genomes * n_scen if layout == 0 else n_genomes
+        rec = DeviceBuffer(4 * 8 * max(1, nf * rec_cap))
+        _ck(LIB.pr_fly_record(self.h, layout, n_genomes, n_scen, C.byref(params), self.d_fit.ptr, self.d_steps.ptr,
+                              rec.ptr, rec_every, rec_cap), "pr_fly_record")
+        frames = rec.download(np.float32, nf * rec_cap * 8).reshape(nf, rec_cap, 8)
+        return self.d_fit.download(np.float32, nf), self.d_steps.download(np.int32, nf), frames
+
+    def read_extent(self, n=None):
+        n = self.n_prepared if n is None else n
+        out = np.zeros(3 * n, np.float32)
+        _ck(LIB.pr_read_extent(self.h, n, out), "pr_read_extent")
+        return out.reshape(n, 3)
+
     def init_flights(self, layout, n_genomes, n_scen, params=None):
         params = params or SimParams.reference()
         _ck(LIB.pr_init_flights(self.h, layout, n_genomes, n_scen, C.byref(params), self.d_fit.ptr,
@@ -222,6 +277,14 @@ class Planes:
         _ck(LIB.pr_scores(self.h, buf.ptr, pop, nseeds, self.d_scores.ptr), "pr_scores")
         return self.d_scores.download(np.float32, pop)
 
+    def set_prepare_precision(self, bits):
+        """32: bit-exact reference float meshing/mass; 64: FP64 prepare rounded once."""
+        _ck(LIB.pr_set_prepare_precision(self.h, bits), "pr_set_prepare_precision")
+
+    def set_graph(self, on=True):
+        """Capture pr_fly's launch sequence as a hipGraph and replay it."""
+        _ck(LIB.pr_set_graph(self.h, int(on)), "pr_set_graph")
+
     def set_buckets(self, nt_max, L, P):
         _ck(LIB.pr_set_buckets(self.h, len(nt_max), np.asarray(nt_max, np.int32), np.asarray(L, np.int32),
                                np.asarray(P, np.int32)), "pr_set_buckets")
@@ -229,7 +292,7 @@ class Planes:
     def stats(self):
         st = Stats()
         _ck(LIB.pr_stats_get(self.h, C.byref(st)), "pr_stats_get")
-        return {k: getattr(st, k) for k, _ in Stats._fields_}
+        return st.as_dict()
 
     # --------------------------------------------------------------- readback
     def read_prepared(self, n=None):
@@ -260,3 +323,223 @@ class Planes:
         out = out.reshape(n, 16)
         return {"state": out[:, :13].copy(), "step": out[:, 13].view(np.int32).copy(),
                 "done": out[:, 14].view(np.int32).copy(), "fitness": out[:, 15].copy()}
+
+
+# ------------------------------------------------------------------ GA host side (C ABI)
+def gene_bounds():
+    lo = np.zeros(N_GENES, np.float32)
+    hi = np.zeros(N_GENES, np.float32)
+    LIB.pr_gene_bounds(lo, hi)
+    return lo, hi
+
+
+class GA:
+    """The GA's random stream (std::mt19937 + its distributions) and operators."""
+
+    def __init__(self, seed=42):
+        self.h = LIB.pr_ga_new(seed)
+
+    def random(self, n):
+        out = np.zeros(n * N_GENES, np.float32)
+        LIB.pr_ga_random(self.h, out, n)
+        return out.reshape(n, N_GENES)
+
+    def breed(self, gs, acc, idx, gen, gens):
+        gs = np.ascontiguousarray(gs, np.float32).reshape(-1)
+        pop = len(acc)
+        nxt = np.zeros(pop * N_GENES, np.float32)
+        LIB.pr_ga_breed(self.h, gs, np.ascontiguousarray(acc, np.float32), np.ascontiguousarray(idx, np.int32), pop,
+                        gen, gens, nxt)
+        return nxt.reshape(pop, N_GENES)
+
+    def __del__(self):
+        try:
+            LIB.pr_ga_free(self.h)
+        except Exception:
+            pass
+
+
+def ga_rank(acc):
+    acc = np.ascontiguousarray(acc, np.float32)
+    idx = np.zeros(len(acc), np.int32)
+    LIB.pr_ga_rank(acc, len(acc), idx)
+    return idx
+
+
+def ga_sanitize(genomes):
+    g = np.ascontiguousarray(genomes, np.float32).copy()
+    LIB.pr_ga_sanitize(g.reshape(-1), len(g))
+    return g
+
+
+class Generation:
+    """pr_gen: one GA generation's batched evaluation (seeds x genomes, scores, ranking,
+    top-8 holdout re-test on 128 scenarios)."""
+
+    def __init__(self, pop, nseeds_max, max_ch=0, max_sp=0):
+        self.h = VP()
+        _ck(LIB.pr_gen_create(C.byref(self.h), pop, nseeds_max, max_ch, max_sp), "pr_gen_create")
+        self.pop = pop
+        self.d_genomes = DeviceBuffer(4 * N_GENES * pop)
+        self.d_res = DeviceBuffer(8 * pop)
+
+    def upload(self, genomes, res=None):
+        self.d_genomes.upload(np.ascontiguousarray(genomes, np.float32))
+        self.has_res = res is not None
+        if res is not None:
+            self.d_res.upload(np.ascontiguousarray(res, np.int32))
+
+    def evaluate(self, pop, seeds, params, holdout_k=8, want_samples=False):
+        seeds = np.ascontiguousarray(seeds, np.int32)
+        acc = np.zeros(pop, np.float32)
+        idx = np.zeros(pop, np.int32)
+        samples = np.zeros(pop * len(seeds), np.float32) if want_samples else None
+        hold = C.c_float()
+        champ = C.c_int()
+        _ck(LIB.pr_gen_evaluate(self.h, self.d_genomes.ptr, self.d_res.ptr if self.has_res else None, pop, seeds,
+                                len(seeds), holdout_k, C.byref(params), acc, idx,
+                                None if samples is None else samples.ctypes.data, C.byref(hold), C.byref(champ)),
+            "pr_gen_evaluate")
+        out = {"acc": acc, "idx": idx, "holdout": hold.value, "champion": champ.value}
+        if samples is not None:
+            out["samples"] = samples.reshape(pop, len(seeds))
+        return out
+
+    def stats(self):
+        a, b = Stats(), Stats()
+        _ck(LIB.pr_gen_stats(self.h, C.byref(a), C.byref(b)), "pr_gen_stats")
+        return a.as_dict(), b.as_dict()
+
+    def __del__(self):
+        try:
+            LIB.pr_gen_destroy(self.h)
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ geometry families (workload synthesis)
+FAMILIES = ("delta", "elliptical", "plank", "generic")
+FIN_LAYOUTS = ("centre", "twin", "triple", "outboard")
+
+
+def mesh_triangles(genome, ch, sp):
+    """Closed-form triangle count of a genome at mesh resolution (ch, sp) (common.h:177-292)."""
+    lay = genome[26]
+    nfin = 1 if lay < 0.25 else (2 if lay < 0.5 else (3 if lay < 0.75 else 2))
+    bi = 1 if genome[13] > np.float32(0.02) else 0
+    return 4 * sp * ch * (1 + bi) + 8 + 2 * nfin + 6 + 4 * bi
+
+
+def family_population(n, seed=0, nt_min=200, nt_max=900, biplane_frac=0.25):
+    """Mixed-family population for the 100k x 12 configuration: planform family (delta,
+    elliptical, plank, generic), mono/biplane, centre/twin/triple/outboard fins, and a
+    per-genome mesh resolution (ch, sp) whose triangle count is log-uniform in
+    [nt_min, nt_max].  Returns (genomes [n,35] f32, res [n,2] int32, family ids)."""
+    lo, hi = gene_bounds()
+    rng = np.random.default_rng(seed)
+    g = (lo + rng.random((n, N_GENES), dtype=np.float32) * (hi - lo)).astype(np.float32)
+    fam = rng.integers(0, len(FAMILIES), n)
+
+    def u(a, b, m):
+        return rng.uniform(a, b, m).astype(np.float32)
+
+    d = fam == 0  # delta: strong sweep, small taper, pointed tips, straight chord blend
+    m = int(d.sum())
+    g[d, 3], g[d, 2], g[d, 33] = u(0.6, 0.9, m), u(0.04, 0.25, m), u(0.0, 0.15, m)
+    g[d, 24], g[d, 25] = u(-0.1, 0.2, m), u(-0.2, 0.1, m)
+    e = fam == 1  # elliptical: rounded tips, moderate taper, little sweep
+    m = int(e.sum())
+    g[e, 33], g[e, 2], g[e, 3] = u(0.7, 1.0, m), u(0.4, 0.8, m), u(0.0, 0.2, m)
+    p = fam == 2  # plank: straight, untapered, reflexed trailing edge
+    m = int(p.sum())
+    g[p, 3], g[p, 2], g[p, 10], g[p, 8] = u(0.0, 0.15, m), u(0.75, 1.0, m), u(0.05, 0.2, m), u(-0.06, 0.0, m)
+    bi = rng.random(n) < biplane_frac
+    g[bi, 13] = u(0.025, 0.06, int(bi.sum()))
+    g[~bi, 13] = u(0.0, 0.019, int((~bi).sum()))
+    fins = rng.integers(0, 4, n)
+    g[:, 26] = (fins * 0.25 + rng.uniform(0.02, 0.23, n)).astype(np.float32)
+    g = np.clip(g, lo, hi).astype(np.float32)
+    # resolution: target triangle count log-uniform in [nt_min, nt_max]
+    target = np.exp(rng.uniform(np.log(nt_min), np.log(nt_max), n))
+    res = np.zeros((n, 2), np.int32)
+    for i in range(n):
+        b = 1 if g[i, 13] > np.float32(0.02) else 0
+        base = mesh_triangles(g[i], 0, 0)
+        tw = max(8.0, (target[i] - base) / (1 + b))  # triangles of one deck = 4 sp ch
+        ch = int(np.clip(round(np.sqrt(tw / 8.0)), 2, 12))
+        sp = int(np.clip(round(tw / (4 * ch)), 2, 24))
+        while mesh_triangles(g[i], ch, sp) > nt_max and sp > 2:
+            sp -= 1
+        while mesh_triangles(g[i], ch, sp) < nt_min and sp < 24:
+            sp += 1
+        res[i] = (ch, sp)
+    return g, res, fam
+
+
+# ------------------------------------------------------------------ autotuner
+def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536, 2048), sample_flights=24576,
+             steps=600, out_json=None, verbose=False):
+    """Pick the flight-kernel mapping per triangle-count range: for each range (lo, hi],
+    every compiled (L lanes per flight, P panels per lane) variant with L*P >= hi flies
+    the same sample of real flights (genomes whose count falls in the range) for one
+    chunk; the variant with the least kernel time per panel-step wins.  Adjacent ranges
+    with the same winner are merged.  Returns (nt_max, L, P) lists; writes JSON if asked."""
+    import json
+
+    g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
+    r = None if res is None else np.ascontiguousarray(res, np.int32).reshape(-1, 2)
+    nt = np.array([mesh_triangles(g[i], *(r[i] if r is not None else (5, 8))) for i in range(len(g))])
+    nt_cap = int(nt.max())
+    ranges = [x for x in ranges if x < nt_cap] + [min(y for y in ranges if y >= nt_cap)]
+    variants = fly_variants()
+    seeds = np.ascontiguousarray(seeds, np.int32)
+    tune_params = SimParams(params.preset, params.gust_rms_max, params.warp_max, params.wind_sd,
+                            min(params.max_steps, steps), params.chunk, 0.0)
+    choice = []
+    lo = 0
+    max_ch = int(r[:, 0].max()) if r is not None else 0
+    max_sp = int(r[:, 1].max()) if r is not None else 0
+    for hi in ranges:
+        sel = np.nonzero((nt > lo) & (nt <= hi))[0]
+        cands = [(L, P) for (L, P) in variants if L * P >= hi]
+        if len(sel) == 0 or not cands:
+            lo = hi
+            continue
+        ng = max(1, min(len(sel), sample_flights // len(seeds)))
+        gi = sel[:ng]
+        pl = Planes(ng, len(seeds), ng * len(seeds), max_ch=max_ch, max_sp=max_sp)
+        pl.prepare(g[gi], None if r is None else r[gi])
+        pl.scenarios(seeds, tune_params)
+        best = None
+        for (L, P) in cands:
+            big = max(variants, key=lambda v: v[0] * v[1])  # catch-all bucket; the sample never reaches it
+            if hi >= big[0] * big[1]:
+                pl.set_buckets([hi], [L], [P])
+            else:
+                pl.set_buckets([hi, big[0] * big[1]], [L, big[0]], [P, big[1]])
+            pl.fly_device(0, ng, len(seeds), tune_params)  # warm
+            pl.fly_device(0, ng, len(seeds), tune_params)
+            st = pl.stats()
+            b0 = st["buckets"][0]
+            cost = b0["fly_ms"] / max(1, b0["panel_steps"])
+            if verbose:
+                print(f"  range ({lo},{hi}] L={L} P={P}: {b0['fly_ms']:.3f} ms, {cost * 1e9:.3f} ps/panel-step")
+            if best is None or cost < best[0]:
+                best = (cost, L, P)
+        pl.close()
+        choice.append((hi, best[1], best[2]))
+        lo = hi
+    merged = []
+    for hi, L, P in choice:
+        if merged and merged[-1][1:] == (L, P):
+            merged[-1] = (hi, L, P)
+        else:
+            merged.append((hi, L, P))
+    merged[-1] = (max(merged[-1][0], nt_cap), merged[-1][1], merged[-1][2])
+    nt_max = [m[0] for m in merged]
+    Ls = [m[1] for m in merged]
+    Ps = [m[2] for m in merged]
+    if out_json:
+        with open(out_json, "w") as f:
+            json.dump({"nt_max": nt_max, "L": Ls, "P": Ps}, f)
+    return nt_max, Ls, Ps

@@ -47,6 +47,9 @@ def _load():
     lib.or_wind_f32.argtypes = [C.c_float] * 4 + [C.c_int, F32P, F32P]
     lib.or_wind_f64.argtypes = [C.c_double] * 4 + [C.c_int, F32P, F64P]
     lib.or_set_libm_float.argtypes = [C.c_int]
+    for fn in ("or_sinf", "or_cosf", "or_tanf"):
+        getattr(lib, fn).argtypes = [C.c_float]
+        getattr(lib, fn).restype = C.c_float
     lib.or_hash32.argtypes = [C.c_uint32]
     lib.or_hash32.restype = C.c_uint32
     lib.or_lane_u01.argtypes = [C.c_int, C.c_int]
