@@ -32,6 +32,10 @@ WORKLOADS = {
                desc="64 genomes x 8 seeded launch scenarios, config preset, 8 s cap"),
     "c2": dict(pop=10000, nseeds=8, max_steps=9000, gust=2.0, warp_deg=1.0, wind_sd=2.0,
                desc="one GA generation: pop 10000 x 8 seeds, config preset, 15 s cap, top-8 x 128 holdout"),
+    "c3": dict(pop=100000, nseeds=12, max_steps=7200, gust=4.0, warp_deg=1.0, wind_sd=2.0, families=True,
+               desc="one GA generation: pop 100000 x 12 environments, mixed families (delta/elliptical/plank/"
+                    "biplane, centre/twin/triple/outboard fins), triangles log-uniform 200..900, gusts <= 4 m/s "
+                    "RMS, 12 s cap, top-8 x 128 holdout"),
 }
 
 
@@ -43,7 +47,7 @@ def seeds_for(gen, nseeds):
     return np.array([gen * nseeds + s + 1 for s in range(nseeds)], np.int32)
 
 
-def cpu_baseline(genomes, w, budget_s):
+def cpu_baseline(genomes, res, w, budget_s):
     """The oracle's FP32 transcription on this host's cores over a bounded sample of the
     same workload (same genome distribution, scenario preset and cap)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -53,6 +57,8 @@ def cpu_baseline(genomes, w, budget_s):
     rng = np.random.default_rng(1)
     kw = dict(prec=32, preset=1, gust=w["gust"], warp=float(np.deg2rad(w["warp_deg"])), wsd=w["wind_sd"],
               max_steps=w["max_steps"], nthreads=nth)
+    if res is not None:  # the oracle takes one resolution per call: group the sample by (ch, sp)
+        return cpu_baseline_res(O, genomes, res, kw, budget_s)
 
     def sample(n):
         gi = rng.integers(0, len(genomes), n).astype(np.int32)
@@ -71,12 +77,86 @@ def cpu_baseline(genomes, w, budget_s):
                       f"(g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {pfs:.0f} plane-flight-s"}
 
 
+def cpu_baseline_res(O, genomes, res, kw, budget_s):
+    rng = np.random.default_rng(1)
+    nth = kw["nthreads"]
+    keys = sorted(set(map(tuple, res.tolist())))
+
+    def run(n):
+        gi = rng.integers(0, len(genomes), n)
+        steps_tot, wall_tot = 0, 0.0
+        for key in keys:
+            sel = gi[(res[gi, 0] == key[0]) & (res[gi, 1] == key[1])]
+            if len(sel) == 0:
+                continue
+            sd = rng.integers(1, 1 + 12 * 1000, len(sel)).astype(np.int32)
+            _, steps, wall = O.fly(genomes, sel.astype(np.int32), sd, ch=key[0], sp=key[1], **kw)
+            steps_tot += int(steps.sum())
+            wall_tot += wall
+        return steps_tot, wall_tot
+
+    s, wall = run(max(256, 16 * nth))
+    n = int(min(200000, max(256, budget_s / max(wall / max(256, 16 * nth), 1e-6))))
+    s, wall = run(n)
+    pfs = s / 600.0
+    return {"value": pfs / wall, "unit": "plane-flight-s/s", "cores": nth, "kind": "port",
+            "sample": f"{n} random (genome, seed) flights of the same workload (per-genome mesh resolution), FP32 "
+                      f"oracle (g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {pfs:.0f} plane-flight-s"}
+
+
+def pool_ab(PA, args):
+    """C4: GA generations/hour with surrogate pre-screening vs the plain batched sim.
+    plain: every generation flies the whole pool of mutants; surrogate: the network scores
+    the pool and only the elite + best-predicted candidates are flown (n_fly); both keep
+    the best simulated genomes as the next elite and report the holdout champion."""
+    n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
+    params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
+    g0, _, _ = PA.family_population(n_elite, seed=0)
+    out = {}
+    for mode in ("plain", "surrogate"):
+        ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
+                       sigma_frac=0.05, seed=7)
+        ga.set_elite(g0)
+        S = PA.Surrogate(max_batch=4096, seed=3) if mode == "surrogate" else None
+        gens, times, t_total = [], [], 0.0
+        if S is not None:
+            # bootstrap: one plain generation provides the first training pairs
+            r = ga.generation(0, params, None)
+            t0 = time.perf_counter()
+            ga.train(S, epochs=4, lr=2e-3)
+            r["wall_ms"] += 1000 * (time.perf_counter() - t0)
+            gens.append(r)
+        for gen in range(len(gens), args.gens):
+            r = ga.generation(gen, params, S, train_epochs=1, lr=1e-3)
+            gens.append(r)
+            log(f"[{mode}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout {r['holdout']:.3f} "
+                f"{r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) flights {r['flights']:.0f}")
+        wall = sum(r["wall_ms"] for r in gens) / 1000.0
+        out[mode] = {"gens": len(gens), "wall_s": round(wall, 3), "gens_per_hour": round(3600.0 * len(gens) / wall, 1),
+                     "final_holdout": round(gens[-1]["holdout"], 4), "final_elite_mean": round(gens[-1]["elite_mean"], 4),
+                     "flights_per_gen": gens[-1]["flights"]}
+    ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
+    line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",
+            "n_gpus": 1, "steps": args.gens, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 sim, bf16 surrogate", "data": "synthetic",
+            "config": {"workload": f"C4: pool {n_pool} Gaussian mutants (sigma 5% of range) of a {n_elite} elite, "
+                                   f"surrogate flies the top {n_fly}; 12 environments, gusts <= 4 m/s RMS, 12 s cap",
+                       "parallelism": "single GPU"},
+            "plain": out["plain"], "surrogate": out["surrogate"], "speedup_gens_per_hour": round(ratio, 2)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--pool", type=int, default=1_000_000, help="c4: candidate pool per generation")
+    ap.add_argument("--fly", type=int, default=100_000, help="c4: candidates flown after pre-screening")
+    ap.add_argument("--elite", type=int, default=10_000, help="c4: elite size")
+    ap.add_argument("--gens", type=int, default=4, help="c4: generations per mode")
+    ap.add_argument("--buffer", type=int, default=500_000, help="c4: surrogate replay buffer")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -86,15 +166,24 @@ def main():
 
     import planes_amd as PA
 
+    if args.workload == "c4":
+        log("device:", PA.device_name())
+        return pool_ab(PA, args)
     w = WORKLOADS[args.workload]
     log("device:", PA.device_name())
-    lo, hi = PA.gene_bounds()
-    rng = np.random.default_rng(0)
-    genomes = (lo + rng.random((w["pop"], 35), dtype=np.float32) * (hi - lo)).astype(np.float32)
+    res = None
+    if w.get("families"):
+        genomes, res, _ = PA.family_population(w["pop"], seed=0)
+    else:
+        lo, hi = PA.gene_bounds()
+        rng = np.random.default_rng(0)
+        genomes = (lo + rng.random((w["pop"], 35), dtype=np.float32) * (hi - lo)).astype(np.float32)
     params = PA.SimParams.config(gust_rms_max=w["gust"], warp_max_deg=w["warp_deg"], wind_sd=w["wind_sd"],
                                  max_steps=w["max_steps"])
-    E = PA.Generation(w["pop"], w["nseeds"])
-    E.upload(genomes)
+    max_ch = int(res[:, 0].max()) if res is not None else 0
+    max_sp = int(res[:, 1].max()) if res is not None else 0
+    E = PA.Generation(w["pop"], w["nseeds"], max_ch, max_sp)
+    E.upload(genomes, res)
 
     for k in range(args.warmup):
         E.evaluate(w["pop"], seeds_for(10000 + k, w["nseeds"]), params)
@@ -107,7 +196,7 @@ def main():
         for st in E.stats():
             fsteps += st["flight_steps"]
             for b in st["buckets"]:
-                key = (b["L"], b["P"])
+                key = (b["L"], b["P"], b["tail"])
                 acc = bucket.setdefault(key, {"flight_steps": 0, "panel_steps": 0, "fly_ms": 0.0, "launches": 0})
                 for f in ("flight_steps", "panel_steps", "fly_ms", "launches"):
                     acc[f] += b[f]
@@ -122,18 +211,19 @@ def main():
     key, b = max(bucket.items(), key=lambda kv: kv[1]["fly_ms"])
     flops = FLOP_PANEL * b["panel_steps"] + FLOP_PLANE * b["flight_steps"]
     achieved = flops / (b["fly_ms"] * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": f"k_fly<{key[0]},{key[1]}>", "achieved": round(achieved, 3),
+    roof = {"bound": "valu", "kernel": f"k_fly<{key[0]},{key[1]}>" + (" (tail mapping)" if key[2] else ""),
+            "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": None, "avg_launch_ms": round(b["fly_ms"] / max(1, b["launches"]), 4),
             "flops_per_launch": flops / max(1, b["launches"])}
     for kk, bb in sorted(bucket.items()):
-        log(f"  k_fly<{kk[0]},{kk[1]}>: {bb['launches']} launches, {bb['fly_ms']:.2f} ms, "
+        log(f"  k_fly<{kk[0]},{kk[1]}>{' tail' if kk[2] else ''}: {bb['launches']} launches, {bb['fly_ms']:.2f} ms, "
             f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
 
     cpu = None
     if args.cpu_seconds > 0:
         log("cpu baseline ...")
-        cpu = cpu_baseline(genomes, w, args.cpu_seconds)
+        cpu = cpu_baseline(genomes, res, w, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.1f} pfs/s on {cpu['cores']} threads")
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,

@@ -33,17 +33,17 @@ typedef struct pr_sim_params {
     float spread;        /* showcase launch grid spacing [m], 0 for evaluation (sim.cu:71-78) */
 } pr_sim_params;
 
-#define PR_MAX_BUCKETS 8
+#define PR_MAX_SLOTS 16
 typedef struct pr_stats {
     unsigned long long flight_steps; /* integrated 600 Hz steps of the last pr_fly (= plane-flight-s * 600) */
     unsigned long long panel_steps;  /* sum over those steps of the plane's triangle count */
-    double fly_ms;                   /* GPU time of the flight-kernel launches (HIP events on the context stream) */
+    double fly_ms;                   /* GPU time of the flight-kernel launches (HIP events on the launch streams) */
     int fly_launches;
     double prep_ms;                  /* GPU time of the last pr_prepare */
-    int n_buckets;                   /* per triangle-count bucket (one flight-kernel variant each): */
-    int b_nt_max[PR_MAX_BUCKETS], b_L[PR_MAX_BUCKETS], b_P[PR_MAX_BUCKETS], b_launches[PR_MAX_BUCKETS];
-    unsigned long long b_flight_steps[PR_MAX_BUCKETS], b_panel_steps[PR_MAX_BUCKETS];
-    double b_fly_ms[PR_MAX_BUCKETS];
+    int n_slots;                     /* slot 2b: bucket b's many-flights-per-wave kernel, 2b+1: its tail kernel */
+    int s_nt_max[PR_MAX_SLOTS], s_L[PR_MAX_SLOTS], s_P[PR_MAX_SLOTS], s_launches[PR_MAX_SLOTS];
+    unsigned long long s_flight_steps[PR_MAX_SLOTS], s_panel_steps[PR_MAX_SLOTS];
+    double s_fly_ms[PR_MAX_SLOTS];
 } pr_stats;
 
 /* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).
@@ -107,6 +107,10 @@ int pr_set_graph(pr_ctx* ctx, int on);
  * (and above the previous bucket's) are flown by groups of L[i] lanes holding P[i]
  * panels per lane (L*P >= nt_max).  Available (L, P) pairs: pr_fly_variants. */
 int pr_set_buckets(pr_ctx* ctx, int n, const int* nt_max, const int* L, const int* P);
+/* 1 (default): each bucket also queues a one-flight-per-wave mapping (64 lanes, P = nt/64)
+ * that takes over once the bucket's remaining flights fit one per wave -- the long-flight
+ * tail is then latency- rather than throughput-bound.  0: one mapping throughout. */
+int pr_set_tail_mapping(pr_ctx* ctx, int on);
 int pr_fly_variants(int* n, int* L, int* P, int cap);
 
 /* Readback of intermediate results (tests and tools). */
@@ -140,6 +144,48 @@ int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop
                     int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
                     float* holdout, int* champion);
 int pr_gen_stats(pr_gen* e, pr_stats* main_stats, pr_stats* holdout_stats);
+
+/* ---------------------------------------------------------------- surrogate MLP
+ * BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU) on the matrix cores, FP32 accumulation,
+ * FP32 master weights + Adam (no reference counterpart: a pre-screen for candidate pools;
+ * the flight simulation stays the judge).  Targets per genome: mean distance, CVaR of the
+ * worst quarter, worst distance [m], oversize flag (0/1).  Parameter layout (FP32):
+ * W1[64x256] b1[256] W2[256x256] b2[256] W3[256x256] b3[256] W4[256x32] b4[32]. */
+typedef struct pr_mlp pr_mlp;
+int pr_mlp_param_count(void);
+int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed);
+void pr_mlp_destroy(pr_mlp* m);
+const char* pr_mlp_last_error(void);
+int pr_mlp_get_params(pr_mlp* m, float* out, int cap);
+int pr_mlp_set_params(pr_mlp* m, const float* in, int n);
+/* d_pred [n x 5]: mean, CVaR25, worst [m], oversize probability, predicted robust score */
+int pr_mlp_predict(pr_mlp* m, const float* d_genomes, int n, float* d_pred);
+/* mean loss + gradient of one batch (n <= max_batch); grads host, may be null */
+int pr_mlp_loss_grad(pr_mlp* m, const float* d_genomes, const float* d_targets, int n, float* loss, float* grads);
+/* Adam over contiguous mini-batches of (d_genomes [n x 35], d_targets [n x 4]) */
+int pr_mlp_train(pr_mlp* m, const float* d_genomes, const float* d_targets, int n, int epochs, int batch, float lr,
+                 float* last_loss);
+
+/* ---------------------------------------------------------------- candidate-pool GA
+ * Generation: pool = elite + Gaussian mutants (sigma = sigma_frac x gene range) of random
+ * elite parents; fly the whole pool (model = null) or the elite + the candidates with the
+ * best surrogate-predicted score (n_fly in total); new elite = best simulated scores;
+ * top-8 holdout champion; with a model, the flown pairs join a replay buffer of buf_cap
+ * and the network trains on it.  The simulation alone ranks the elite and the champion.
+ * pr_pool_generation out[10]: best score, elite mean, holdout champion score, wall ms,
+ * flight ms, surrogate ms, flights, plane-flight-s, training loss, fly-set size. */
+typedef struct pr_pool pr_pool;
+int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds, int buf_cap, float sigma_frac,
+                   unsigned seed);
+void pr_pool_destroy(pr_pool* p);
+const char* pr_pool_last_error(void);
+int pr_pool_set_elite(pr_pool* p, const float* genomes, int n);
+int pr_pool_get_elite(pr_pool* p, float* genomes, int n);
+int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
+                       double* out);
+int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* samples, int n);
+int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
+int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 
 /* Device memory helpers so that hosts without a GPU array library can drive the API. */
 int pr_dev_alloc(void** p, size_t bytes);

@@ -37,7 +37,9 @@ static int fail(int code, const char* what) {
         if (!(cond)) { g_err = msg; return -1; } \
     } while (0)
 
-struct Bucket { int nt_max, L, P; };
+// Flight-kernel mapping of one triangle-count bucket: (L, P) while many flights remain,
+// (Lw, Pw) = one flight per wave once they fit one per wave (Lw = 0: none).
+struct Bucket { int nt_max, L, P, Lw, Pw; };
 
 struct pr_ctx {
     hipStream_t stream = nullptr;
@@ -65,6 +67,7 @@ struct pr_ctx {
     unsigned long long* d_stats = nullptr;
     std::vector<Bucket> buckets;
     std::vector<int> grid;    // persistent grid per bucket
+    std::vector<int> grid_w;  // persistent grid of the bucket's one-flight-per-wave mapping
     hipEvent_t ev[2 * 256 + 2] = {};
     int ev_bucket[256] = {};
     hipStream_t bstream[8] = {};     // one stream per triangle-count bucket (its chunk chain)
@@ -72,6 +75,7 @@ struct pr_ctx {
     // hipGraph of the whole flight sequence (optional), keyed by its launch parameters
     int use_graph = 0;
     int prep_fp64 = 0;
+    int tail_wide = 1;
     hipGraphExec_t gexec = nullptr;
     long long gkey[10] = {};
     pr_stats stats = {};
@@ -87,13 +91,24 @@ static int mesh_bounds(int ch, int sp, int* nv, int* nt) {
     return 0;
 }
 
+static int wide_for(int nt_max, int* Lw, int* Pw) {
+    static const int Ps[] = {3, 6, 8, 12, 16, 24, 32};
+    for (int p : Ps)
+        if (64 * p >= nt_max) { *Lw = 64; *Pw = p; return 0; }
+    *Lw = 0; *Pw = 0;
+    return 0;
+}
+
 static int default_buckets(pr_ctx* c) {
-    static const Bucket table[] = {{192, 8, 24}, {352, 16, 22}, {384, 16, 24}, {768, 32, 24},
-                                   {928, 32, 29}, {1536, 64, 24}, {2048, 64, 32}};
+    static const int table[][3] = {{192, 8, 24}, {352, 16, 22}, {512, 32, 16}, {768, 32, 24},
+                                   {1024, 64, 16}, {1536, 64, 24}, {2048, 64, 32}};
     int nv, nt;
     mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
     c->buckets.clear();
-    for (const Bucket& b : table) {
+    for (const auto& t : table) {
+        Bucket b = {t[0], t[1], t[2], 0, 0};
+        wide_for(b.nt_max, &b.Lw, &b.Pw);
+        if (b.Lw == b.L && b.Pw == b.P) b.Lw = 0;
         c->buckets.push_back(b);
         if (b.nt_max >= nt) break;
     }
@@ -102,6 +117,10 @@ static int default_buckets(pr_ctx* c) {
 }
 
 static int upload_buckets(pr_ctx* c) {
+    if (c->gexec) {  // a captured flight graph holds the old mapping
+        HCK(hipGraphExecDestroy(c->gexec));
+        c->gexec = nullptr;
+    }
     int nb = (int)c->buckets.size();
     std::vector<int> h(nb);
     for (int i = 0; i < nb; i++) h[i] = c->buckets[i].nt_max;
@@ -111,11 +130,15 @@ static int upload_buckets(pr_ctx* c) {
     HCK(hipGetDevice(&dev));
     HCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     c->grid.assign(nb, 0);
+    c->grid_w.assign(nb, 0);
     for (int i = 0; i < nb; i++) {
         int occ = 0;
         HCK(fly_occupancy(c->buckets[i].L, c->buckets[i].P, &occ));
-        if (occ < 1) occ = 1;
-        c->grid[i] = occ * ncu;
+        c->grid[i] = (occ < 1 ? 1 : occ) * ncu;
+        if (c->buckets[i].Lw) {
+            HCK(fly_occupancy(c->buckets[i].Lw, c->buckets[i].Pw, &occ));
+            c->grid_w[i] = (occ < 1 ? 1 : occ) * ncu;
+        }
     }
     return 0;
 }
@@ -162,7 +185,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
     ALLOC(c->d_counts, sizeof(int) * 3 * NB_MAX);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
-    ALLOC(c->d_stats, sizeof(unsigned long long) * 2 * NB_MAX);
+    ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream) !=
         hipSuccess) { pr_destroy(c); return fail(-1, "scan workspace query"); }
     ALLOC(c->d_scan_tmp, c->scan_tmp_bytes + 16);
@@ -210,16 +233,34 @@ int pr_set_buckets(pr_ctx* c, int n, const int* nt_max, const int* L, const int*
     ARG(c && n > 0 && n <= NB_MAX, "pr_set_buckets: 1..8 buckets");
     std::vector<Bucket> b(n);
     for (int i = 0; i < n; i++) {
-        b[i] = {nt_max[i], L[i], P[i]};
+        b[i] = {nt_max[i], L[i], P[i], 0, 0};
         ARG(L[i] * P[i] >= nt_max[i], "pr_set_buckets: L*P below the bucket's triangle count");
         ARG(i == 0 || nt_max[i] > nt_max[i - 1], "pr_set_buckets: nt_max must increase");
         int occ = 0;
         ARG(fly_occupancy(L[i], P[i], &occ) == hipSuccess, "pr_set_buckets: (L, P) is not a compiled variant");
+        if (c->tail_wide) {
+            wide_for(nt_max[i], &b[i].Lw, &b[i].Pw);
+            if (b[i].Lw == L[i] && b[i].Pw == P[i]) b[i].Lw = 0;
+        }
     }
     int nv, nt;
     mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
     ARG(b.back().nt_max >= nt, "pr_set_buckets: last bucket below the context's largest triangle count");
     c->buckets = b;
+    return upload_buckets(c);
+}
+
+// 1 (default): buckets switch to one flight per wave once their flights fit that way
+int pr_set_tail_mapping(pr_ctx* c, int on) {
+    ARG(c, "pr_set_tail_mapping: null context");
+    c->tail_wide = on ? 1 : 0;
+    for (Bucket& b : c->buckets) {
+        b.Lw = 0; b.Pw = 0;
+        if (c->tail_wide) {
+            wide_for(b.nt_max, &b.Lw, &b.Pw);
+            if (b.Lw == b.L && b.Pw == b.P) b.Lw = 0;
+        }
+    }
     return upload_buckets(c);
 }
 
@@ -276,6 +317,7 @@ static void fill_fly_args(pr_ctx* c, const pr_sim_params* p, float* d_fitness, i
     a.state = c->d_state; a.fitness = d_fitness; a.steps = d_steps;
     a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
     a.list_in = nullptr; a.count_in = nullptr; a.list_out = nullptr; a.count_out = nullptr; a.head = nullptr;
+    a.run_lo = 0; a.run_hi = 0x7fffffff;
 }
 
 static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
@@ -290,7 +332,7 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     ARG(nf <= c->cap_flights, "pr_fly: flights exceed max_flights");
     const int NB = (int)c->buckets.size();
     HCK(launch_zero(c->d_counts, 3 * NB_MAX, c->stream));
-    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 4 * NB_MAX, c->stream));
+    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 8 * NB_MAX, c->stream));
     QueueArgs q;
     q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
     q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
@@ -323,7 +365,6 @@ static int issue_chains(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
         HCK(hipStreamWaitEvent(s, c->fork, 0));
         int* cb = c->d_counts + 3 * b;
         a.head = cb + 2;
-        a.stats = c->d_stats + 2 * b;
         for (int ck = 0; ck < n_chunks; ck++) {
             const int in = ck & 1, outb = in ^ 1;
             a.list_in = c->d_lists[in] + (size_t)b * c->cap_flights;
@@ -331,10 +372,19 @@ static int issue_chains(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
             a.count_in = cb + in;
             a.count_out = cb + outb;
             HCK(launch_zero_pair(cb + outb, cb + 2, s));
-            const bool tev = timed && ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
-            if (tev) HCK(hipEventRecord(c->ev[ne], s));
-            HCK(launch_fly(c->buckets[b].L, c->buckets[b].P, a, c->grid[b], s));
-            if (tev) { HCK(hipEventRecord(c->ev[ne + 1], s)); c->ev_bucket[ne / 2] = b; ne += 2; }
+            const Bucket& bk = c->buckets[b];
+            for (int w = 0; w < (bk.Lw ? 2 : 1); w++) {
+                // w = 0: many flights per wave while more than grid_w flights remain;
+                // w = 1: one flight per wave (lowest per-step latency) for the tail
+                a.run_lo = (w == 0 && bk.Lw) ? c->grid_w[b] + 1 : 0;
+                a.run_hi = w == 0 ? 0x7fffffff : c->grid_w[b];
+                a.stats = c->d_stats + 2 * (2 * b + w);
+                const bool tev = timed && ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
+                if (tev) HCK(hipEventRecord(c->ev[ne], s));
+                if (w == 0) HCK(launch_fly(bk.L, bk.P, a, c->grid[b], s));
+                else HCK(launch_fly(bk.Lw, bk.Pw, a, c->grid_w[b], s));
+                if (tev) { HCK(hipEventRecord(c->ev[ne + 1], s)); c->ev_bucket[ne / 2] = 2 * b + w; ne += 2; }
+            }
         }
         HCK(hipEventRecord(c->join[b], s));
         HCK(hipStreamWaitEvent(c->stream, c->join[b], 0));
@@ -392,27 +442,32 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
     HCK(hipStreamSynchronize(c->stream));
     pr_stats& S = c->stats;
     S = pr_stats();
-    S.n_buckets = NB;
-    for (int b = 0; b < NB; b++) { S.b_nt_max[b] = c->buckets[b].nt_max; S.b_L[b] = c->buckets[b].L; S.b_P[b] = c->buckets[b].P; }
+    S.n_slots = 2 * NB;
+    for (int b = 0; b < NB; b++) {
+        const Bucket& bk = c->buckets[b];
+        S.s_nt_max[2 * b] = S.s_nt_max[2 * b + 1] = bk.nt_max;
+        S.s_L[2 * b] = bk.L; S.s_P[2 * b] = bk.P;
+        S.s_L[2 * b + 1] = bk.Lw; S.s_P[2 * b + 1] = bk.Pw;
+    }
     for (int k = 0; k < ne; k += 2) {
         float t = 0;
         HCK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
         S.fly_ms += t;
-        S.b_fly_ms[c->ev_bucket[k / 2]] += t;
-        S.b_launches[c->ev_bucket[k / 2]] += 1;
+        S.s_fly_ms[c->ev_bucket[k / 2]] += t;
+        S.s_launches[c->ev_bucket[k / 2]] += 1;
     }
     if (c->use_graph) {
         float t = 0;
         HCK(hipEventElapsedTime(&t, c->g0, c->g1));
         S.fly_ms = t;
     }
-    unsigned long long st[2 * NB_MAX];
+    unsigned long long st[4 * NB_MAX];
     HCK(hipMemcpy(st, c->d_stats, sizeof st, hipMemcpyDeviceToHost));
-    for (int b = 0; b < NB; b++) {
-        S.b_flight_steps[b] = st[2 * b];
-        S.b_panel_steps[b] = st[2 * b + 1];
-        S.flight_steps += st[2 * b];
-        S.panel_steps += st[2 * b + 1];
+    for (int k = 0; k < 2 * NB; k++) {
+        S.s_flight_steps[k] = st[2 * k];
+        S.s_panel_steps[k] = st[2 * k + 1];
+        S.flight_steps += st[2 * k];
+        S.panel_steps += st[2 * k + 1];
     }
     S.fly_launches = ne / 2;
     float pm = 0;
@@ -428,6 +483,8 @@ int pr_fly_record(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim
     if (rc) return rc;
     FlyArgs a;
     fill_fly_args(c, p, d_fitness, d_steps, a);
+    a.run_lo = 0;
+    a.run_hi = 0x7fffffff;
     // every flight flown by the widest-bucket mapping (recording is not throughput work)
     const Bucket& b = c->buckets.back();
     HCK(launch_fly_rec(b.L, b.P, a, (int)nf, d_rec, rec_every, rec_cap, c->stream));

@@ -36,6 +36,17 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+template <int L>
+__device__ __forceinline__ float group_max(float v) {
+    if (L >= 2) v = fmaxf(v, dpp_mov<0xB1>(v));
+    if (L >= 4) v = fmaxf(v, dpp_mov<0x4E>(v));
+    if (L >= 8) v = fmaxf(v, dpp_mov<0x141>(v));
+    if (L >= 16) v = fmaxf(v, dpp_mov<0x140>(v));
+    if (L >= 32) v = fmaxf(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F)));
+    if (L >= 64) v = fmaxf(v, __shfl_xor(v, 32));
+    return v;
+}
+
 
 // ------------------------------------------------------------------ k_layout
 __global__ void k_layout(const float* __restrict__ genomes, const int* __restrict__ res, int def_ch, int def_sp,
@@ -367,6 +378,35 @@ struct Flyer {
         }
     }
 
+    // Force and torque of panel p in body axes (sim.cu:144-158 rotated into the body frame):
+    // v_r = v_b + w x r, s = v_r.n/|v_r|, q = k |v_r|^2 (k = rho A / 2), and
+    // f = -q cn n - q cf (v_r/|v_r| - s n) = c1 n + c2 v_r with cn = 2 s h,
+    // h = 0.95 sqrt(1 - s^2) + |s|, c1 = q s (cf - 2h), c2 = -q cf / |v_r|.
+    // The reference skips panels with |v_r|^2 < 1e-6 (its NaN guard); here rsq(V2 + 1e-30)
+    // keeps V2 = 0 finite with zero force, and 0 < V2 < 1e-6 contributes < 1e-9 N.
+    __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
+                                                float cf, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
+                                                float& Tz) const {
+        const float vrx = __builtin_fmaf(wy, rz[p], __builtin_fmaf(-wz, ry[p], vbx));
+        const float vry = __builtin_fmaf(wz, rx[p], __builtin_fmaf(-wx, rz[p], vby));
+        const float vrz = __builtin_fmaf(wx, ry[p], __builtin_fmaf(-wy, rx[p], vbz));
+        const float V2 = vrx * vrx + vry * vry + vrz * vrz;
+        const float inv = __builtin_amdgcn_rsqf(V2 + 1e-30f);
+        const float sn = (vrx * nx[p] + vry * ny[p] + vrz * nz[p]) * inv;
+        const float qd = kk[p] * V2;
+        const float sq = __builtin_amdgcn_sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
+        const float h = 0.95f * sq + fabsf(sn);
+        const float c1 = (qd * sn) * (cf - 2.0f * h);
+        const float c2 = -(qd * inv) * cf;
+        const float fx = c1 * nx[p] + c2 * vrx;
+        const float fy = c1 * ny[p] + c2 * vry;
+        const float fz = c1 * nz[p] + c2 * vrz;
+        Fx += fx; Fy += fy; Fz += fz;
+        Tx = __builtin_fmaf(ry[p], fz, __builtin_fmaf(-rz[p], fy, Tx));
+        Ty = __builtin_fmaf(rz[p], fx, __builtin_fmaf(-rx[p], fz, Ty));
+        Tz = __builtin_fmaf(rx[p], fy, __builtin_fmaf(-ry[p], fx, Tz));
+    }
+
     // One step (sim.cu:129-205) for the group; returns true when the flight ended (ground,
     // NaN, out of bounds or the step cap).
     __device__ __forceinline__ bool step(int max_steps) {
@@ -393,29 +433,7 @@ struct Flyer {
         const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
         float Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
 #pragma unroll
-        for (int p = 0; p < P; p++) {
-            // relative flow at the panel: v_b + w x r (body axes)
-            const float vrx = vbx + (wy * rz[p] - wz * ry[p]);
-            const float vry = vby + (wz * rx[p] - wx * rz[p]);
-            const float vrz = vbz + (wx * ry[p] - wy * rx[p]);
-            const float V2 = vrx * vrx + vry * vry + vrz * vrz;
-            const float inv = __builtin_amdgcn_rsqf(fmaxf(V2, 1e-6f));
-            const float sn = (vrx * nx[p] + vry * ny[p] + vrz * nz[p]) * inv;
-            const float qd = V2 >= 1e-6f ? kk[p] * V2 : 0.0f;
-            const float sq = __builtin_amdgcn_sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
-            // cn = 1.9 s sqrt(1-s^2) + 2 s|s| = 2 s (0.95 sqrt(1-s^2) + |s|)
-            const float h = 0.95f * sq + fabsf(sn);
-            const float qs = qd * sn;
-            const float c1 = qs * (cf - 2.0f * h);     // coefficient on n: -qd cn + qd cf s
-            const float c2 = -(qd * inv) * cf;         // coefficient on v_r: -qd cf / |v_r|
-            const float fx = c1 * nx[p] + c2 * vrx;
-            const float fy = c1 * ny[p] + c2 * vry;
-            const float fz = c1 * nz[p] + c2 * vrz;
-            Fx += fx; Fy += fy; Fz += fz;
-            Tx += ry[p] * fz - rz[p] * fy;
-            Ty += rz[p] * fx - rx[p] * fz;
-            Tz += rx[p] * fy - ry[p] * fx;
-        }
+        for (int p = 0; p < P; p++) panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
         Tx = group_sum<L>(Tx); Ty = group_sum<L>(Ty); Tz = group_sum<L>(Tz);
         // rigid-body update (sim.cu:177-197); torque is already in body axes
@@ -478,6 +496,9 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
     const int n_in = *a.count_in;
+    // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
+    // flight count decides which one works, so the choice needs no host synchronisation
+    if (n_in < a.run_lo || n_in > a.run_hi) return;
     Flyer<L, P> F;
     bool active = false, exhausted = false;
     unsigned long long fsteps = 0, psteps = 0;
@@ -636,8 +657,8 @@ static hipError_t fly_one(const FlyArgs& a, int grid, hipStream_t s) {
 
 // compiled (L, P) variants; capacity L*P triangles per flight
 #define PR_VARIANTS(X) \
-    X(4, 16) X(4, 24) X(8, 12) X(8, 16) X(8, 20) X(8, 24) X(16, 12) X(16, 16) X(16, 22) X(16, 24) \
-    X(32, 16) X(32, 24) X(32, 29) X(64, 16) X(64, 24) X(64, 32)
+    X(4, 16) X(8, 12) X(8, 16) X(8, 20) X(8, 24) X(16, 12) X(16, 16) X(16, 22) X(32, 16) X(32, 24) \
+    X(64, 3) X(64, 6) X(64, 8) X(64, 12) X(64, 16) X(64, 24) X(64, 32)
 
 int fly_variant_count() {
     int n = 0;

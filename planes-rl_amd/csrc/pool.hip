@@ -1,0 +1,320 @@
+// pool.hip -- candidate-pool GA with optional surrogate pre-screening (no reference
+// counterpart; the benchmark configuration "score 1M mutants, fly the top 100k").
+//
+// One generation, all on the GPU except the host's bookkeeping:
+//   1. pool: the elite (re-flown every generation under new seeds, like the reference's
+//      GA) + Gaussian mutants of random elite parents (sigma = mutation fraction of each
+//      gene's range, clamped to the bounds) -- k_mutate
+//   2. fly set: the whole pool (plain), or the elite + the candidates with the best
+//      surrogate-predicted robust score (pr_mlp_predict, hipCUB radix sort)
+//   3. flights of the fly set under the generation's seeds + robust scores (pr_fly/pr_scores)
+//   4. new elite = the best flown genomes by simulated score; top-8 re-flown on the
+//      128-scenario holdout suite -> champion
+//   5. surrogate mode: the flown (genome, fitness statistics) pairs join a replay buffer
+//      and the network trains on it (pr_mlp_train)
+// The simulation is the only judge of the elite and the champion.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/planes_hip.h"
+#include "pr_physics.h"
+
+namespace {
+
+thread_local std::string g_pool_err;
+
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
+    return pr::mix32(a * 0x9e3779b9u ^ pr::mix32(b * 0x85ebca6bu ^ pr::mix32(c + 0x27d4eb2fu)));
+}
+__device__ __forceinline__ float unit24(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// pool[i] = elite[i] for i < n_elite, else a mutant of a random elite parent
+__global__ void k_mutate(const float* __restrict__ elite, int n_elite, float* __restrict__ pool, int n_pool,
+                         const float* __restrict__ lo, const float* __restrict__ hi, float sigma_frac, uint32_t seed,
+                         uint32_t gen) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pool) return;
+    if (i < n_elite) {
+        for (int j = 0; j < pr::NG; j++) pool[(size_t)i * pr::NG + j] = elite[(size_t)i * pr::NG + j];
+        return;
+    }
+    const int par = (int)(hash3(seed, gen, (uint32_t)i) % (uint32_t)n_elite);
+    for (int j = 0; j < pr::NG; j += 2) {
+        // Box-Muller pair
+        const uint32_t h1 = hash3(seed ^ 0x5bd1e995u, gen * 64u + (uint32_t)j, (uint32_t)i);
+        const uint32_t h2 = pr::mix32(h1 + 0x68e31da4u);
+        const float u1 = ((float)(h1 >> 8) + 1.0f) * (1.0f / 16777216.0f);
+        const float u2 = unit24(h2);
+        const float r = sqrtf(-2.0f * logf(u1));
+        const float a = 6.2831853f * u2;
+        const float z[2] = {r * cosf(a), r * sinf(a)};
+        for (int q = 0; q < 2 && j + q < pr::NG; q++) {
+            const int k = j + q;
+            const float range = hi[k] - lo[k];
+            float v = elite[(size_t)par * pr::NG + k] + sigma_frac * range * z[q];
+            pool[(size_t)i * pr::NG + k] = fminf(hi[k], fmaxf(lo[k], v));
+        }
+    }
+}
+
+__global__ void k_gather(const float* __restrict__ src, const int* __restrict__ idx, int n, float* __restrict__ dst) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * pr::NG) return;
+    const int i = t / pr::NG, j = t - i * pr::NG;
+    dst[t] = src[(size_t)idx[i] * pr::NG + j];
+}
+
+// keys for a descending sort: -score, with the elite (i < n_keep) forced to the front
+__global__ void k_sort_keys(const float* __restrict__ pred, int stride, int col, int n, int n_keep,
+                            float* __restrict__ keys, int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = i < n_keep ? -3.0e38f : -pred[(size_t)i * stride + col];
+    vals[i] = i;
+}
+
+// per-genome fitness statistics of the flown samples: mean (seed order), CVaR of the worst
+// quarter, worst, oversize flag (all samples -100) -- the surrogate's training targets
+__global__ void k_targets(const float* __restrict__ samples, int n, int k, float* __restrict__ tgt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float row[32];
+    float mean = 0.0f;
+    bool over = true;
+    for (int s = 0; s < k; s++) {
+        const float x = samples[(size_t)i * k + s];
+        mean += x / (float)k;
+        over = over && x == -100.0f;
+        int q = s;
+        while (q > 0 && row[q - 1] > x) { row[q] = row[q - 1]; q--; }
+        row[q] = x;
+    }
+    const int tail = (k + 3) / 4;
+    float cv = 0.0f;
+    for (int s = 0; s < tail; s++) cv += row[s] / (float)tail;
+    tgt[(size_t)i * 4 + 0] = mean;
+    tgt[(size_t)i * 4 + 1] = cv;
+    tgt[(size_t)i * 4 + 2] = row[0];
+    tgt[(size_t)i * 4 + 3] = over ? 1.0f : 0.0f;
+}
+
+}  // namespace
+
+struct pr_pool {
+    int n_elite = 0, n_pool = 0, n_fly = 0, nseeds = 0, buf_cap = 0;
+    float sigma = 0.05f;
+    unsigned seed = 1;
+    pr_gen* gen_eval = nullptr;
+    float *d_elite = nullptr, *d_pool = nullptr, *d_fly = nullptr, *d_lo = nullptr, *d_hi = nullptr;
+    float *d_pred = nullptr, *d_keys = nullptr, *d_keys2 = nullptr, *d_samples = nullptr, *d_tgt = nullptr;
+    float *d_buf_g = nullptr, *d_buf_t = nullptr;
+    int *d_vals = nullptr, *d_vals2 = nullptr, *d_idx = nullptr;
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int buf_n = 0, buf_head = 0;
+    std::vector<float> acc, samples, champion;
+    std::vector<int> idx;
+    float champion_score = -1e9f;
+};
+
+#define PCK(x)                                                                                   \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) { g_pool_err = std::string(#x) + ": " + hipGetErrorString(e_); return (int)e_; } \
+    } while (0)
+#define PRC(x)                                                                                   \
+    do {                                                                                         \
+        int r_ = (x);                                                                            \
+        if (r_) { g_pool_err = std::string(#x) + ": " + pr_last_error() + pr_mlp_last_error(); return r_; } \
+    } while (0)
+
+extern "C" {
+
+const char* pr_pool_last_error(void) { return g_pool_err.c_str(); }
+
+int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds, int buf_cap, float sigma_frac,
+                   unsigned seed) {
+    if (!out || n_elite < 8 || n_pool < n_fly || n_fly < n_elite || nseeds < 1 || nseeds > 32) {
+        g_pool_err = "pr_pool_create: need 8 <= n_elite <= n_fly <= n_pool and 1 <= nseeds <= 32";
+        return -1;
+    }
+    pr_pool* p = new pr_pool;
+    p->n_elite = n_elite; p->n_pool = n_pool; p->n_fly = n_fly; p->nseeds = nseeds;
+    p->buf_cap = std::max(buf_cap, n_fly);
+    p->sigma = sigma_frac;
+    p->seed = seed;
+    PRC(pr_gen_create(&p->gen_eval, n_pool, nseeds, 0, 0));
+    const size_t G = sizeof(float) * pr::NG;
+    PCK(hipMalloc(&p->d_elite, G * n_elite));
+    PCK(hipMalloc(&p->d_pool, G * n_pool));
+    PCK(hipMalloc(&p->d_fly, G * n_pool));
+    PCK(hipMalloc(&p->d_lo, G));
+    PCK(hipMalloc(&p->d_hi, G));
+    PCK(hipMalloc(&p->d_pred, sizeof(float) * 5 * (size_t)n_pool));
+    PCK(hipMalloc(&p->d_keys, sizeof(float) * n_pool));
+    PCK(hipMalloc(&p->d_keys2, sizeof(float) * n_pool));
+    PCK(hipMalloc(&p->d_vals, sizeof(int) * n_pool));
+    PCK(hipMalloc(&p->d_vals2, sizeof(int) * n_pool));
+    PCK(hipMalloc(&p->d_idx, sizeof(int) * n_pool));
+    PCK(hipMalloc(&p->d_samples, sizeof(float) * (size_t)n_pool * nseeds));
+    PCK(hipMalloc(&p->d_tgt, sizeof(float) * 4 * (size_t)n_pool));
+    PCK(hipMalloc(&p->d_buf_g, G * p->buf_cap));
+    PCK(hipMalloc(&p->d_buf_t, sizeof(float) * 4 * (size_t)p->buf_cap));
+    PCK(hipcub::DeviceRadixSort::SortPairs(nullptr, p->sort_tmp_bytes, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
+                                           n_pool));
+    PCK(hipMalloc(&p->d_sort_tmp, p->sort_tmp_bytes + 16));
+    float lo[pr::NG], hi[pr::NG];
+    pr_gene_bounds(lo, hi);
+    PCK(hipMemcpy(p->d_lo, lo, G, hipMemcpyHostToDevice));
+    PCK(hipMemcpy(p->d_hi, hi, G, hipMemcpyHostToDevice));
+    p->champion.assign(pr::NG, 0.0f);
+    *out = p;
+    return 0;
+}
+
+void pr_pool_destroy(pr_pool* p) {
+    if (!p) return;
+    pr_gen_destroy(p->gen_eval);
+    void* bufs[] = {p->d_elite, p->d_pool, p->d_fly, p->d_lo, p->d_hi, p->d_pred, p->d_keys, p->d_keys2,
+                    p->d_vals, p->d_vals2, p->d_idx, p->d_samples, p->d_tgt, p->d_buf_g, p->d_buf_t, p->d_sort_tmp};
+    for (void* b : bufs) if (b) (void)hipFree(b);
+    delete p;
+}
+
+// initial elite (host genomes, n >= n_elite; the first n_elite are used)
+int pr_pool_set_elite(pr_pool* p, const float* genomes, int n) {
+    if (!p || n < p->n_elite) { g_pool_err = "pr_pool_set_elite: need n >= n_elite"; return -1; }
+    PCK(hipMemcpy(p->d_elite, genomes, sizeof(float) * pr::NG * p->n_elite, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int pr_pool_get_elite(pr_pool* p, float* genomes, int n) {
+    if (!p || n < p->n_elite) { g_pool_err = "pr_pool_get_elite: capacity"; return -1; }
+    PCK(hipMemcpy(genomes, p->d_elite, sizeof(float) * pr::NG * p->n_elite, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Adds (genome, target) pairs (device) to the surrogate's replay buffer (ring).
+static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
+    for (int s = 0; s < n;) {
+        const int k = std::min(n - s, p->buf_cap - p->buf_head);
+        PCK(hipMemcpy(p->d_buf_g + (size_t)p->buf_head * pr::NG, d_g + (size_t)s * pr::NG, sizeof(float) * pr::NG * k,
+                      hipMemcpyDeviceToDevice));
+        PCK(hipMemcpy(p->d_buf_t + (size_t)p->buf_head * 4, d_t + (size_t)s * 4, sizeof(float) * 4 * k,
+                      hipMemcpyDeviceToDevice));
+        p->buf_head = (p->buf_head + k) % p->buf_cap;
+        p->buf_n = std::min(p->buf_cap, p->buf_n + k);
+        s += k;
+    }
+    return 0;
+}
+
+// One generation.  model = null: plain (fly the whole pool); else pre-screen with the
+// surrogate and train it on the flown pairs for train_epochs epochs over the buffer.
+// out[0..9]: best simulated score, elite mean score, holdout score of the champion,
+// wall ms, flight ms, surrogate ms (predict + train), flights flown, plane-flight-s,
+// last training loss, fly-set size.
+int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
+                       double* out) {
+    if (!p || !prm || !out) { g_pool_err = "pr_pool_generation: null argument"; return -1; }
+    auto t0 = std::chrono::steady_clock::now();
+    double mlp_ms = 0;
+    hipLaunchKernelGGL(k_mutate, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_elite, p->n_elite, p->d_pool,
+                       p->n_pool, p->d_lo, p->d_hi, p->sigma, p->seed, (uint32_t)gen);
+    PCK(hipGetLastError());
+    const float* fly = p->d_pool;
+    int nf = p->n_pool;
+    if (model) {
+        PCK(hipDeviceSynchronize());
+        auto m0 = std::chrono::steady_clock::now();
+        PRC(pr_mlp_predict(model, p->d_pool, p->n_pool, p->d_pred));
+        hipLaunchKernelGGL(k_sort_keys, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->n_pool,
+                           p->n_elite, p->d_keys, p->d_vals);
+        PCK(hipGetLastError());
+        size_t tb = p->sort_tmp_bytes;
+        PCK(hipcub::DeviceRadixSort::SortPairs(p->d_sort_tmp, tb, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
+                                               p->n_pool));
+        nf = p->n_fly;
+        hipLaunchKernelGGL(k_gather, dim3((nf * pr::NG + 255) / 256), dim3(256), 0, 0, p->d_pool, p->d_vals2, nf,
+                           p->d_fly);
+        PCK(hipGetLastError());
+        PCK(hipDeviceSynchronize());
+        mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
+        fly = p->d_fly;
+    }
+    PCK(hipDeviceSynchronize());
+    std::vector<int> seeds(p->nseeds);
+    for (int s = 0; s < p->nseeds; s++) seeds[s] = gen * p->nseeds + s + 1;
+    p->acc.resize(nf);
+    p->idx.resize(nf);
+    p->samples.resize((size_t)nf * p->nseeds);
+    float holdout = 0;
+    int champ = 0;
+    PRC(pr_gen_evaluate(p->gen_eval, fly, nullptr, nf, seeds.data(), p->nseeds, 8, prm, p->acc.data(), p->idx.data(),
+                        model ? p->samples.data() : nullptr, &holdout, &champ));
+    pr_stats st_main, st_hold;
+    pr_gen_stats(p->gen_eval, &st_main, &st_hold);
+    // new elite: best simulated scores
+    PCK(hipMemcpy(p->d_idx, p->idx.data(), sizeof(int) * p->n_elite, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_gather, dim3((p->n_elite * pr::NG + 255) / 256), dim3(256), 0, 0, fly, p->d_idx, p->n_elite,
+                       p->d_elite);
+    PCK(hipGetLastError());
+    PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
+    p->champion_score = holdout;
+    float loss = 0;
+    if (model) {
+        auto m0 = std::chrono::steady_clock::now();
+        PCK(hipMemcpy(p->d_samples, p->samples.data(), sizeof(float) * p->samples.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_targets, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_samples, nf, p->nseeds, p->d_tgt);
+        PCK(hipGetLastError());
+        int rc = buffer_add(p, fly, p->d_tgt, nf);
+        if (rc) return rc;
+        if (train_epochs > 0) PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, train_epochs, 4096, lr, &loss));
+        PCK(hipDeviceSynchronize());
+        mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
+    }
+    double mean = 0;
+    for (int i = 0; i < p->n_elite; i++) mean += p->acc[p->idx[i]];
+    out[0] = p->acc[p->idx[0]];
+    out[1] = mean / p->n_elite;
+    out[2] = holdout;
+    out[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    out[4] = st_main.fly_ms + st_hold.fly_ms;
+    out[5] = mlp_ms;
+    out[6] = (double)nf * p->nseeds + 8.0 * 128;
+    out[7] = (double)(st_main.flight_steps + st_hold.flight_steps) / 600.0;
+    out[8] = loss;
+    out[9] = nf;
+    return 0;
+}
+
+// Seeds the surrogate's replay buffer with already-flown pairs (device genomes [n x 35],
+// host samples [n x nseeds]) -- e.g. the flights of earlier generations.
+int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* samples, int n) {
+    if (!p || n < 1 || n > p->n_pool) { g_pool_err = "pr_pool_add_training: 1 <= n <= n_pool"; return -1; }
+    PCK(hipMemcpy(p->d_samples, samples, sizeof(float) * (size_t)n * p->nseeds, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, 0, p->d_samples, n, p->nseeds, p->d_tgt);
+    PCK(hipGetLastError());
+    return buffer_add(p, d_genomes, p->d_tgt, n);
+}
+
+int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss) {
+    if (!p || !model || p->buf_n < 32) { g_pool_err = "pr_pool_train: model and >= 32 buffered pairs needed"; return -1; }
+    PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, epochs, 4096, lr, loss));
+    return 0;
+}
+
+int pr_pool_champion(pr_pool* p, float* genome35, float* holdout) {
+    if (!p) return -1;
+    std::memcpy(genome35, p->champion.data(), sizeof(float) * pr::NG);
+    if (holdout) *holdout = p->champion_score;
+    return 0;
+}
+
+}  // extern "C"

@@ -54,6 +54,7 @@ struct FlyArgs {
     int* count_out;
     int* head;
     int chunk, max_steps;
+    int run_lo, run_hi;    // the launch works only if run_lo <= flights in list_in <= run_hi
     float spread;
     unsigned long long* stats;  // [0] flight-steps, [1] panel-steps
 };

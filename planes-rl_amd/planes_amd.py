@@ -34,22 +34,22 @@ class SimParams(C.Structure):
         return cls(1, gust_rms_max, np.float32(np.deg2rad(warp_max_deg)), wind_sd, max_steps, chunk, 0.0)
 
 
-NB_MAX = 8
+NS_MAX = 16
 
 
 class Stats(C.Structure):
     _fields_ = [("flight_steps", C.c_ulonglong), ("panel_steps", C.c_ulonglong), ("fly_ms", C.c_double),
-                ("fly_launches", C.c_int), ("prep_ms", C.c_double), ("n_buckets", C.c_int),
-                ("b_nt_max", C.c_int * NB_MAX), ("b_L", C.c_int * NB_MAX), ("b_P", C.c_int * NB_MAX),
-                ("b_launches", C.c_int * NB_MAX), ("b_flight_steps", C.c_ulonglong * NB_MAX),
-                ("b_panel_steps", C.c_ulonglong * NB_MAX), ("b_fly_ms", C.c_double * NB_MAX)]
+                ("fly_launches", C.c_int), ("prep_ms", C.c_double), ("n_slots", C.c_int),
+                ("s_nt_max", C.c_int * NS_MAX), ("s_L", C.c_int * NS_MAX), ("s_P", C.c_int * NS_MAX),
+                ("s_launches", C.c_int * NS_MAX), ("s_flight_steps", C.c_ulonglong * NS_MAX),
+                ("s_panel_steps", C.c_ulonglong * NS_MAX), ("s_fly_ms", C.c_double * NS_MAX)]
 
     def as_dict(self):
-        nb = self.n_buckets
         d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms")}
-        d["buckets"] = [{"nt_max": self.b_nt_max[i], "L": self.b_L[i], "P": self.b_P[i],
-                         "launches": self.b_launches[i], "flight_steps": self.b_flight_steps[i],
-                         "panel_steps": self.b_panel_steps[i], "fly_ms": self.b_fly_ms[i]} for i in range(nb)]
+        d["buckets"] = [{"nt_max": self.s_nt_max[i], "L": self.s_L[i], "P": self.s_P[i], "tail": i % 2 == 1,
+                         "launches": self.s_launches[i], "flight_steps": self.s_flight_steps[i],
+                         "panel_steps": self.s_panel_steps[i], "fly_ms": self.s_fly_ms[i]}
+                        for i in range(self.n_slots) if self.s_L[i] > 0]
         return d
 
 
@@ -85,7 +85,28 @@ def _load():
     lib.pr_fly_record.argtypes = [VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, VP, C.c_int, C.c_int]
     lib.pr_read_extent.argtypes = [VP, C.c_int, F32P]
     lib.pr_set_graph.argtypes = [VP, C.c_int]
+    lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
+    lib.pr_mlp_last_error.restype = C.c_char_p
+    lib.pr_pool_last_error.restype = C.c_char_p
+    lib.pr_pool_create.argtypes = [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint]
+    lib.pr_pool_destroy.argtypes = [VP]
+    lib.pr_pool_destroy.restype = None
+    lib.pr_pool_set_elite.argtypes = [VP, F32P, C.c_int]
+    lib.pr_pool_get_elite.argtypes = [VP, F32P, C.c_int]
+    lib.pr_pool_generation.argtypes = [VP, C.c_int, C.POINTER(SimParams), VP, C.c_int, C.c_float,
+                                       np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+    lib.pr_pool_train.argtypes = [VP, VP, C.c_int, C.c_float, C.POINTER(C.c_float)]
+    lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
+    lib.pr_mlp_param_count.restype = C.c_int
+    lib.pr_mlp_create.argtypes = [C.POINTER(VP), C.c_int, C.c_uint]
+    lib.pr_mlp_destroy.argtypes = [VP]
+    lib.pr_mlp_destroy.restype = None
+    lib.pr_mlp_get_params.argtypes = [VP, F32P, C.c_int]
+    lib.pr_mlp_set_params.argtypes = [VP, F32P, C.c_int]
+    lib.pr_mlp_predict.argtypes = [VP, VP, C.c_int, VP]
+    lib.pr_mlp_loss_grad.argtypes = [VP, VP, VP, C.c_int, C.POINTER(C.c_float), F32P]
+    lib.pr_mlp_train.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
     lib.pr_gene_bounds.argtypes = [F32P, F32P]
     lib.pr_gene_bounds.restype = None
     lib.pr_ga_new.argtypes = [C.c_uint]
@@ -280,6 +301,9 @@ class Planes:
     def set_prepare_precision(self, bits):
         """32: bit-exact reference float meshing/mass; 64: FP64 prepare rounded once."""
         _ck(LIB.pr_set_prepare_precision(self.h, bits), "pr_set_prepare_precision")
+
+    def set_tail_mapping(self, on=True):
+        _ck(LIB.pr_set_tail_mapping(self.h, int(on)), "pr_set_tail_mapping")
 
     def set_graph(self, on=True):
         """Capture pr_fly's launch sequence as a hipGraph and replay it."""
@@ -508,6 +532,7 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         ng = max(1, min(len(sel), sample_flights // len(seeds)))
         gi = sel[:ng]
         pl = Planes(ng, len(seeds), ng * len(seeds), max_ch=max_ch, max_sp=max_sp)
+        pl.set_tail_mapping(False)  # time the candidate mapping itself on every flight
         pl.prepare(g[gi], None if r is None else r[gi])
         pl.scenarios(seeds, tune_params)
         best = None
@@ -543,3 +568,132 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         with open(out_json, "w") as f:
             json.dump({"nt_max": nt_max, "L": Ls, "P": Ps}, f)
     return nt_max, Ls, Ps
+
+
+# ------------------------------------------------------------------ surrogate MLP
+class Surrogate:
+    """BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 on the matrix cores (pr_mlp_*), trained on
+    (genome, [mean, CVaR25, worst, oversize]) pairs produced by the flight simulation."""
+
+    def __init__(self, max_batch=16384, seed=1):
+        self.h = VP()
+        rc = LIB.pr_mlp_create(C.byref(self.h), max_batch, seed)
+        if rc:
+            raise PlanesError(f"pr_mlp_create failed ({rc}): {LIB.pr_mlp_last_error().decode()}")
+        self.max_batch = max_batch
+        self.n_params = LIB.pr_mlp_param_count()
+        self._bufs = {}
+
+    def _ck(self, rc, what):
+        if rc:
+            raise PlanesError(f"{what} failed ({rc}): {LIB.pr_mlp_last_error().decode()}")
+
+    def _dev(self, name, arr):
+        arr = np.ascontiguousarray(arr, np.float32)
+        b = self._bufs.get(name)
+        if b is None or b.nbytes < arr.nbytes:
+            b = DeviceBuffer(arr.nbytes)
+            self._bufs[name] = b
+        b.upload(arr)
+        return b
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        self._ck(LIB.pr_mlp_get_params(self.h, out, self.n_params), "pr_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._ck(LIB.pr_mlp_set_params(self.h, p, len(p)), "pr_mlp_set_params")
+
+    def predict(self, genomes):
+        """-> [n, 5]: mean, CVaR25, worst (m), oversize probability, predicted robust score."""
+        g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
+        dg = self._dev("pg", g)
+        out = self._bufs.get("pout")
+        if out is None or out.nbytes < 20 * len(g):
+            out = DeviceBuffer(20 * len(g))
+            self._bufs["pout"] = out
+        self._ck(LIB.pr_mlp_predict(self.h, dg.ptr, len(g), out.ptr), "pr_mlp_predict")
+        return out.download(np.float32, 5 * len(g)).reshape(-1, 5)
+
+    def predict_device(self, d_genomes, n, d_pred):
+        self._ck(LIB.pr_mlp_predict(self.h, d_genomes, n, d_pred), "pr_mlp_predict")
+
+    def loss_grad(self, genomes, targets):
+        g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
+        dg = self._dev("lg", g)
+        dt = self._dev("lt", np.ascontiguousarray(targets, np.float32).reshape(-1, 4))
+        loss = C.c_float()
+        grads = np.zeros(self.n_params, np.float32)
+        self._ck(LIB.pr_mlp_loss_grad(self.h, dg.ptr, dt.ptr, len(g), C.byref(loss), grads), "pr_mlp_loss_grad")
+        return loss.value, grads
+
+    def train(self, genomes, targets, epochs=4, batch=4096, lr=1e-3, shuffle_seed=0):
+        g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 4)
+        if shuffle_seed is not None:
+            perm = np.random.default_rng(shuffle_seed).permutation(len(g))
+            g, t = g[perm], t[perm]
+        dg = self._dev("tg", g)
+        dt = self._dev("tt", t)
+        loss = C.c_float()
+        self._ck(LIB.pr_mlp_train(self.h, dg.ptr, dt.ptr, len(g), epochs, min(batch, self.max_batch), lr,
+                                  C.byref(loss)), "pr_mlp_train")
+        return loss.value
+
+    def __del__(self):
+        try:
+            LIB.pr_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+
+class PoolGA:
+    """Candidate-pool GA on the GPU (pr_pool_*): plain (fly the whole pool) or surrogate
+    pre-screened (fly the elite + the best-predicted candidates)."""
+
+    FIELDS = ("best", "elite_mean", "holdout", "wall_ms", "fly_ms", "mlp_ms", "flights", "pfs", "loss", "n_fly")
+
+    def __init__(self, n_elite, n_pool, n_fly, nseeds, buf_cap=2_000_000, sigma_frac=0.05, seed=1):
+        self.h = VP()
+        rc = LIB.pr_pool_create(C.byref(self.h), n_elite, n_pool, n_fly, nseeds, buf_cap, sigma_frac, seed)
+        if rc:
+            raise PlanesError(f"pr_pool_create failed ({rc}): {LIB.pr_pool_last_error().decode()}")
+        self.n_elite = n_elite
+
+    def _ck(self, rc, what):
+        if rc:
+            raise PlanesError(f"{what} failed ({rc}): {LIB.pr_pool_last_error().decode()}")
+
+    def set_elite(self, genomes):
+        g = np.ascontiguousarray(genomes, np.float32).reshape(-1)
+        self._ck(LIB.pr_pool_set_elite(self.h, g, len(g) // N_GENES), "pr_pool_set_elite")
+
+    def elite(self):
+        out = np.zeros(self.n_elite * N_GENES, np.float32)
+        self._ck(LIB.pr_pool_get_elite(self.h, out, self.n_elite), "pr_pool_get_elite")
+        return out.reshape(-1, N_GENES)
+
+    def generation(self, gen, params, surrogate=None, train_epochs=2, lr=1e-3):
+        out = np.zeros(10, np.float64)
+        self._ck(LIB.pr_pool_generation(self.h, gen, C.byref(params), surrogate.h if surrogate else None,
+                                        train_epochs, lr, out), "pr_pool_generation")
+        return dict(zip(self.FIELDS, out.tolist()))
+
+    def train(self, surrogate, epochs, lr=1e-3):
+        loss = C.c_float()
+        self._ck(LIB.pr_pool_train(self.h, surrogate.h, epochs, lr, C.byref(loss)), "pr_pool_train")
+        return loss.value
+
+    def champion(self):
+        g = np.zeros(N_GENES, np.float32)
+        hs = C.c_float()
+        self._ck(LIB.pr_pool_champion(self.h, g, C.byref(hs)), "pr_pool_champion")
+        return g, hs.value
+
+    def __del__(self):
+        try:
+            LIB.pr_pool_destroy(self.h)
+        except Exception:
+            pass

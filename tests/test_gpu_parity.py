@@ -169,6 +169,7 @@ def test_mapping_invariance(PA):
         if L * P < 344:
             continue
         pl = PA.Planes(len(gs), 2, 64)
+        pl.set_tail_mapping(False)  # 64 flights would otherwise all take the one-per-wave tail mapping
         pl.set_buckets([344], [L], [P])
         fit, steps = pl.evaluate(gs, seeds, params)
         if base is None:
@@ -176,6 +177,23 @@ def test_mapping_invariance(PA):
             continue
         rel = np.abs(fit - base) / np.maximum(np.abs(base), 1e-3)
         assert np.mean(rel < 1e-3) >= 0.95, (L, P)
+
+
+def test_tail_mapping_equivalent(PA):
+    """The one-flight-per-wave tail mapping flies the same flights to the same results as
+    the many-per-wave mapping, up to float reassociation of the group sums."""
+    gs = O.random_genomes(64, 17)
+    seeds = np.arange(1, 9, dtype=np.int32)
+    params = PA.SimParams.reference(max_steps=3000)
+    res = []
+    for tail in (False, True):
+        pl = PA.Planes(64, 8, 512)
+        pl.set_tail_mapping(tail)
+        fit, steps = pl.evaluate(gs, seeds, params)
+        res.append((fit.reshape(-1), steps.reshape(-1)))
+    rel = np.abs(res[0][0] - res[1][0]) / np.maximum(np.abs(res[0][0]), 1e-3)
+    print(f"tail mapping: {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3, max {rel.max():.2e}")
+    assert np.mean(rel < 1e-2) >= 0.97
 
 
 def test_graph_replay_identical(PA):
