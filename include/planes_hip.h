@@ -37,13 +37,16 @@ typedef struct pr_sim_params {
 typedef struct pr_stats {
     unsigned long long flight_steps; /* integrated 600 Hz steps of the last pr_fly (= plane-flight-s * 600) */
     unsigned long long panel_steps;  /* sum over those steps of the plane's triangle count */
-    double fly_ms;                   /* GPU time of the flight-kernel launches (HIP events on the launch streams) */
+    double fly_ms;                   /* GPU time of all flight-kernel launches (HIP events on the context stream) */
     int fly_launches;
     double prep_ms;                  /* GPU time of the last pr_prepare */
     int n_slots;                     /* slot 2b: bucket b's many-flights-per-wave kernel, 2b+1: its tail kernel */
     int s_nt_max[PR_MAX_SLOTS], s_L[PR_MAX_SLOTS], s_P[PR_MAX_SLOTS], s_launches[PR_MAX_SLOTS];
     unsigned long long s_flight_steps[PR_MAX_SLOTS], s_panel_steps[PR_MAX_SLOTS];
-    double s_fly_ms[PR_MAX_SLOTS];
+    double s_fly_ms[PR_MAX_SLOTS];   /* own-launch time of a slot (0 for slots the fused kernel flies) */
+    int s_fused[PR_MAX_SLOTS];       /* 1: the slot is flown inside the fused kernel */
+    double fused_ms;                 /* GPU time of the fused flight-kernel launches */
+    int fused_launches;
 } pr_stats;
 
 /* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).
@@ -97,8 +100,8 @@ int pr_scores(pr_ctx* ctx, const float* d_samples, int pop, int nseeds, float* d
 
 int pr_stats_get(pr_ctx* ctx, pr_stats* out);
 
-/* 1: pr_fly captures its whole launch sequence (flight set-up + every bucket's chain of
- * 1-s chunk launches, buckets on parallel branches) as a hipGraph on first use and
+/* 1: pr_fly captures its whole launch sequence (flight set-up + the chain of 1-s chunk
+ * launches) as a hipGraph on first use and
  * replays it while the launch parameters stay the same; per-launch timings are then
  * replaced by one whole-graph time.  0 (default): direct launches. */
 int pr_set_graph(pr_ctx* ctx, int on);
@@ -111,6 +114,11 @@ int pr_set_buckets(pr_ctx* ctx, int n, const int* nt_max, const int* L, const in
  * that takes over once the bucket's remaining flights fit one per wave -- the long-flight
  * tail is then latency- rather than throughput-bound.  0: one mapping throughout. */
 int pr_set_tail_mapping(pr_ctx* ctx, int on);
+/* 1 (default): every bucket whose mappings are compiled into the fused flight kernel is
+ * flown by it -- one persistent launch per 1-s chunk drains all those buckets' work lists,
+ * largest triangle counts first, each with its own lane mapping.  0: one launch per bucket
+ * and mapping per chunk (per-mapping timings in pr_stats, used by autotuning). */
+int pr_set_fused(pr_ctx* ctx, int on);
 int pr_fly_variants(int* n, int* L, int* P, int cap);
 
 /* Readback of intermediate results (tests and tools). */
@@ -144,6 +152,9 @@ int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop
                     int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
                     float* holdout, int* champion);
 int pr_gen_stats(pr_gen* e, pr_stats* main_stats, pr_stats* holdout_stats);
+/* The evaluator's flight contexts (0: population flights, 1: holdout re-test), so that the
+ * pr_set_* options apply to it; owned by the evaluator. */
+pr_ctx* pr_gen_context(pr_gen* e, int which);
 
 /* ---------------------------------------------------------------- surrogate MLP
  * BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU) on the matrix cores, FP32 accumulation,
@@ -184,6 +195,9 @@ int pr_pool_get_elite(pr_pool* p, float* genomes, int n);
 int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
                        double* out);
 int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* samples, int n);
+/* 1: plain generations also add their flown (genome, targets) pairs to the replay buffer
+ * (bootstrapping the surrogate); 0 (default): only surrogate generations do. */
+int pr_pool_set_collect(pr_pool* p, int on);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 

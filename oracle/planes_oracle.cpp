@@ -15,8 +15,20 @@
 // ("correctly rounded float" in practice).  sqrt and division are IEEE.
 // Build with -ffp-contract=off: no FMA contraction anywhere.
 //
-// Parity status per function (how each is pinned) is listed in DESIGN.md and in
-// tests/test_oracle_*.py.
+// How each function is pinned (tests/test_oracle_golden.py, test_oracle_physics.py):
+//   mesh (build, surf, empennage, fin, deck): bit-exact against oracle/_ref, the
+//     reference's own common.h/designs.cu/stl.cu compiled with g++; closed-form counts,
+//     mirror symmetry, the champion STL's triangle count.
+//   lamina_mass, extent: bit-exact against oracle/_ref; geometry_test.cpp properties.
+//   scenario_ref, hash32, printed, gust_wind: bit-exact against oracle/_ref.
+//   osin/ocos/otan: correctly rounded against an arbitrary-precision evaluation;
+//     libm-float mode bit-exact against oracle/_ref.
+//   launch, step, aero: closed forms (ballistic motion at rho = 0, angular-momentum
+//     conservation without torque, flat-plate special cases, torque = r x F, normal-flip
+//     invariance) and FP32 <-> FP64 convergence.
+//   robust_scores, ga_rank, ga2_breed: hand vectors and the reference's score weights.
+//   scenario_cfg, lane_normal: PARITY UNPINNED against the reference (no counterpart
+//     there); checked only by definition (Box-Muller moments) and CPU <-> GPU bit-exact.
 #include <algorithm>
 #include <atomic>
 #include <chrono>

@@ -62,20 +62,22 @@ struct pr_ctx {
     FlightState* d_state = nullptr;
     FlightConst* d_fc = nullptr;
     int* d_lists[2] = {};
-    int* d_counts = nullptr;  // [2][NB] list counts, [NB] queue heads
+    int* d_counts = nullptr;  // PR_CNT per bucket: 3 rotating list counts, 2 rotating queue heads
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
     std::vector<Bucket> buckets;
     std::vector<int> grid;    // persistent grid per bucket
     std::vector<int> grid_w;  // persistent grid of the bucket's one-flight-per-wave mapping
-    hipEvent_t ev[2 * 256 + 2] = {};
-    int ev_bucket[256] = {};
-    hipStream_t bstream[8] = {};     // one stream per triangle-count bucket (its chunk chain)
-    hipEvent_t fork = nullptr, join[8] = {}, g0 = nullptr, g1 = nullptr;
+    std::vector<int> vid_p, vid_w;  // fused-kernel variant ids per bucket (-1: own k_fly launch)
+    int grid_f = 0;                 // persistent grid of the fused kernel
+    hipEvent_t ev[2 * 512 + 2] = {};
+    int ev_slot[512] = {};          // slot of a timed launch; -1 = the fused kernel
+    hipEvent_t g0 = nullptr, g1 = nullptr;
     // hipGraph of the whole flight sequence (optional), keyed by its launch parameters
     int use_graph = 0;
     int prep_fp64 = 0;
     int tail_wide = 1;
+    int fused = 1;
     hipGraphExec_t gexec = nullptr;
     long long gkey[10] = {};
     pr_stats stats = {};
@@ -131,6 +133,8 @@ static int upload_buckets(pr_ctx* c) {
     HCK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     c->grid.assign(nb, 0);
     c->grid_w.assign(nb, 0);
+    c->vid_p.assign(nb, -1);
+    c->vid_w.assign(nb, -1);
     for (int i = 0; i < nb; i++) {
         int occ = 0;
         HCK(fly_occupancy(c->buckets[i].L, c->buckets[i].P, &occ));
@@ -139,7 +143,17 @@ static int upload_buckets(pr_ctx* c) {
             HCK(fly_occupancy(c->buckets[i].Lw, c->buckets[i].Pw, &occ));
             c->grid_w[i] = (occ < 1 ? 1 : occ) * ncu;
         }
+        // a bucket joins the fused kernel only if both of its mappings are compiled into it
+        const int vp = fused_variant_id(c->buckets[i].L, c->buckets[i].P);
+        const int vw = c->buckets[i].Lw ? fused_variant_id(c->buckets[i].Lw, c->buckets[i].Pw) : -1;
+        if (c->fused && vp >= 0 && (c->buckets[i].Lw == 0 || vw >= 0)) {
+            c->vid_p[i] = vp;
+            c->vid_w[i] = vw;
+        }
     }
+    int occ = 0;
+    HCK(fly_all_occupancy(&occ));
+    c->grid_f = (occ < 1 ? 1 : occ) * ncu;
     return 0;
 }
 
@@ -183,7 +197,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_state, sizeof(FlightState) * max_flights);
     ALLOC(c->d_fc, sizeof(FlightConst) * max_flights);
     for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
-    ALLOC(c->d_counts, sizeof(int) * 3 * NB_MAX);
+    ALLOC(c->d_counts, sizeof(int) * PR_CNT * NB_MAX);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
     ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream) !=
@@ -191,15 +205,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_scan_tmp, c->scan_tmp_bytes + 16);
 #undef ALLOC
     for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
-    for (int b = 0; b < NB_MAX; b++) {
-        if (hipStreamCreateWithFlags(&c->bstream[b], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
-            pr_destroy(c);
-            return fail(-1, "bucket stream");
-        }
-    }
-    if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&c->g0) != hipSuccess ||
-        hipEventCreate(&c->g1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
+    if (hipEventCreate(&c->g0) != hipSuccess || hipEventCreate(&c->g1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
     if (hipEventCreate(&c->prep0) != hipSuccess || hipEventCreate(&c->prep1) != hipSuccess) { pr_destroy(c); return fail(-1, "event"); }
     if ((rc = upload_buckets(c))) { pr_destroy(c); return rc; }
     *out = c;
@@ -217,11 +223,6 @@ void pr_destroy(pr_ctx* c) {
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->prep0) (void)hipEventDestroy(c->prep0);
     if (c->prep1) (void)hipEventDestroy(c->prep1);
-    for (int b = 0; b < NB_MAX; b++) {
-        if (c->join[b]) (void)hipEventDestroy(c->join[b]);
-        if (c->bstream[b]) (void)hipStreamDestroy(c->bstream[b]);
-    }
-    if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->g0) (void)hipEventDestroy(c->g0);
     if (c->g1) (void)hipEventDestroy(c->g1);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -331,12 +332,12 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     long long nf = layout == 0 ? (long long)n_genomes * n_scen : n_genomes;
     ARG(nf <= c->cap_flights, "pr_fly: flights exceed max_flights");
     const int NB = (int)c->buckets.size();
-    HCK(launch_zero(c->d_counts, 3 * NB_MAX, c->stream));
+    HCK(launch_zero(c->d_counts, PR_CNT * NB_MAX, c->stream));
     HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 8 * NB_MAX, c->stream));
     QueueArgs q;
     q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
     q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
-    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.count_stride = 3; q.list = c->d_lists[0];
+    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.count_stride = PR_CNT; q.list = c->d_lists[0];
     q.list_stride = c->cap_flights; q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
     HCK(launch_queue(q, c->stream));
     *nf_out = nf;
@@ -352,26 +353,52 @@ int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_s
     return 0;
 }
 
-// Issue the flight sequence after k_init: each bucket's chain of chunk launches on its own
-// stream, forked from and joined back to the context stream.  Counters of bucket b:
-// d_counts[3b + 0/1] = list counts (double-buffered), d_counts[3b + 2] = queue head.
-static int issue_chains(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed, int* n_events) {
+// Issue the flight sequence after k_init on the context stream, one 1-s chunk at a time:
+// the fused kernel flies every bucket compiled into it (one launch per chunk, which also
+// clears the next chunk's counters); a bucket outside it gets its own k_fly launches.
+// Chunk ck: list_in = lists[ck & 1], counts: in = ck % 3, out = (ck + 1) % 3, cleared =
+// (ck + 2) % 3; queue head ck & 1, cleared (ck + 1) & 1 (k_init fills list 0 / count 0).
+static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed, int* n_events) {
     const int NB = (int)c->buckets.size();
     const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
-    HCK(hipEventRecord(c->fork, c->stream));
+    const int n_ev = (int)(sizeof(c->ev) / sizeof(c->ev[0]));
+    bool any_fused = false, any_own = false;
+    for (int b = 0; b < NB; b++) (c->vid_p[b] >= 0 ? any_fused : any_own) = true;
+    FlyAllArgs F;
+    F.base = a;
+    F.lists0 = c->d_lists[0];
+    F.lists1 = c->d_lists[1];
+    F.list_stride = c->cap_flights;
+    F.counts = c->d_counts;
+    F.nb = NB;
+    F.stats = c->d_stats;
+    for (int b = 0; b < 8; b++) {
+        F.vid_p[b] = b < NB ? c->vid_p[b] : -1;
+        F.vid_w[b] = b < NB ? c->vid_w[b] : -1;
+        F.thr[b] = c->grid_f;  // one flight per wave once they fit one per wave
+    }
     int ne = 0;
-    for (int b = 0; b < NB; b++) {
-        hipStream_t s = c->bstream[b];
-        HCK(hipStreamWaitEvent(s, c->fork, 0));
-        int* cb = c->d_counts + 3 * b;
-        a.head = cb + 2;
-        for (int ck = 0; ck < n_chunks; ck++) {
-            const int in = ck & 1, outb = in ^ 1;
-            a.list_in = c->d_lists[in] + (size_t)b * c->cap_flights;
-            a.list_out = c->d_lists[outb] + (size_t)b * c->cap_flights;
-            a.count_in = cb + in;
-            a.count_out = cb + outb;
-            HCK(launch_zero_pair(cb + outb, cb + 2, s));
+    for (int ck = 0; ck < n_chunks; ck++) {
+        F.in_list = ck & 1;
+        F.c_in = ck % 3; F.c_out = (ck + 1) % 3; F.c_zero = (ck + 2) % 3;
+        F.h_run = ck & 1; F.h_zero = (ck + 1) & 1;
+        if (any_fused) {
+            const bool tev = timed && ne + 2 <= n_ev;
+            if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
+            HCK(launch_fly_all(F, c->grid_f, c->stream));
+            if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; ne += 2; }
+        } else {
+            HCK(launch_zero_chunk(c->d_counts, NB, F.c_zero, F.h_zero, c->stream));
+        }
+        if (!any_own) continue;
+        for (int b = 0; b < NB; b++) {
+            if (c->vid_p[b] >= 0) continue;
+            int* cb = c->d_counts + PR_CNT * b;
+            a.list_in = c->d_lists[ck & 1] + (size_t)b * c->cap_flights;
+            a.list_out = c->d_lists[(ck & 1) ^ 1] + (size_t)b * c->cap_flights;
+            a.count_in = cb + F.c_in;
+            a.count_out = cb + F.c_out;
+            a.head = cb + 3 + F.h_run;
             const Bucket& bk = c->buckets[b];
             for (int w = 0; w < (bk.Lw ? 2 : 1); w++) {
                 // w = 0: many flights per wave while more than grid_w flights remain;
@@ -379,18 +406,24 @@ static int issue_chains(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
                 a.run_lo = (w == 0 && bk.Lw) ? c->grid_w[b] + 1 : 0;
                 a.run_hi = w == 0 ? 0x7fffffff : c->grid_w[b];
                 a.stats = c->d_stats + 2 * (2 * b + w);
-                const bool tev = timed && ne + 2 <= (int)(sizeof(c->ev) / sizeof(c->ev[0]));
-                if (tev) HCK(hipEventRecord(c->ev[ne], s));
-                if (w == 0) HCK(launch_fly(bk.L, bk.P, a, c->grid[b], s));
-                else HCK(launch_fly(bk.Lw, bk.Pw, a, c->grid_w[b], s));
-                if (tev) { HCK(hipEventRecord(c->ev[ne + 1], s)); c->ev_bucket[ne / 2] = 2 * b + w; ne += 2; }
+                const bool tev = timed && ne + 2 <= n_ev;
+                if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
+                if (w == 0) HCK(launch_fly(bk.L, bk.P, a, c->grid[b], c->stream));
+                else HCK(launch_fly(bk.Lw, bk.Pw, a, c->grid_w[b], c->stream));
+                if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = 2 * b + w; ne += 2; }
             }
         }
-        HCK(hipEventRecord(c->join[b], s));
-        HCK(hipStreamWaitEvent(c->stream, c->join[b], 0));
     }
     *n_events = ne;
     return 0;
+}
+
+// 1 (default): buckets whose mappings are compiled into the fused flight kernel are flown
+// by it, one launch per chunk; 0: every bucket by its own k_fly launches (per-mapping timing)
+int pr_set_fused(pr_ctx* c, int on) {
+    ARG(c, "pr_set_fused: null context");
+    c->fused = on ? 1 : 0;
+    return upload_buckets(c);
 }
 
 int pr_set_prepare_precision(pr_ctx* c, int bits) {
@@ -414,13 +447,14 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
     long long nf;
     if (c->use_graph) {
         long long key[10] = {layout, n_genomes, n_scen, p->max_steps, p->chunk, (long long)(size_t)d_fitness,
-                             (long long)(size_t)d_steps, p->preset, (long long)c->buckets.size(), c->n_prepared};
+                             (long long)(size_t)d_steps, p->preset, (long long)c->buckets.size() + 64 * c->fused,
+                             c->n_prepared};
         if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
             if (c->gexec) { HCK(hipGraphExecDestroy(c->gexec)); c->gexec = nullptr; }
             HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
             int unused = 0;
-            if (rc == 0) rc = issue_chains(c, p, a, false, &unused);
+            if (rc == 0) rc = issue_chunks(c, p, a, false, &unused);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(c->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -436,7 +470,7 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
     } else {
         int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
         if (rc) return rc;
-        rc = issue_chains(c, p, a, true, &ne);
+        rc = issue_chunks(c, p, a, true, &ne);
         if (rc) return rc;
     }
     HCK(hipStreamSynchronize(c->stream));
@@ -453,9 +487,11 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
         float t = 0;
         HCK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
         S.fly_ms += t;
-        S.s_fly_ms[c->ev_bucket[k / 2]] += t;
-        S.s_launches[c->ev_bucket[k / 2]] += 1;
+        const int sl = c->ev_slot[k / 2];
+        if (sl < 0) { S.fused_ms += t; S.fused_launches += 1; }
+        else { S.s_fly_ms[sl] += t; S.s_launches[sl] += 1; }
     }
+    for (int b = 0; b < NB; b++) S.s_fused[2 * b] = S.s_fused[2 * b + 1] = c->vid_p[b] >= 0;
     if (c->use_graph) {
         float t = 0;
         HCK(hipEventElapsedTime(&t, c->g0, c->g1));

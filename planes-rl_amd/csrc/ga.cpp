@@ -182,6 +182,10 @@ int pr_gen_create(pr_gen** out, int pop, int nseeds_max, int max_ch, int max_sp)
     return 0;
 }
 
+// The evaluator's flight contexts (0: population flights, 1: holdout re-test), for the
+// pr_set_* options (fused kernel, tail mapping, buckets, graph, prepare precision).
+pr_ctx* pr_gen_context(pr_gen* e, int which) { return !e ? nullptr : which ? e->hold : e->main; }
+
 void pr_gen_destroy(pr_gen* e) {
     if (!e) return;
     pr_destroy(e->main);

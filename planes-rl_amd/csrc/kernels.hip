@@ -288,11 +288,6 @@ __global__ void k_zero_ints(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-__global__ void k_zero_pair(int* p, int* q) {
-    if (threadIdx.x == 0) *p = 0;
-    if (threadIdx.x == 1) *q = 0;
-}
-
 // ------------------------------------------------------------------ k_fly
 // Per-lane copy of one group's flight.  All lanes of a group hold bitwise-identical
 // values (they compute the same expressions on the same inputs).
@@ -490,15 +485,14 @@ struct Flyer {
     }
 };
 
+// Persistent loop over one work list: groups of L lanes take flights with one atomic each,
+// fly them to the end of the chunk (or of the flight), store the state and append the
+// survivors to the next chunk's list.  Returns when the list is drained.
 template <int L, int P>
-__global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
+__device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
     const int lane = threadIdx.x & 63;
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
-    const int n_in = *a.count_in;
-    // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
-    // flight count decides which one works, so the choice needs no host synchronisation
-    if (n_in < a.run_lo || n_in > a.run_hi) return;
     Flyer<L, P> F;
     bool active = false, exhausted = false;
     unsigned long long fsteps = 0, psteps = 0;
@@ -545,6 +539,81 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
         atomicAdd(a.stats + 0, fsteps);
         atomicAdd(a.stats + 1, psteps);
     }
+}
+
+template <int L, int P>
+__global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
+    const int n_in = *a.count_in;
+    // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
+    // flight count decides which one works, so the choice needs no host synchronisation
+    if (n_in < a.run_lo || n_in > a.run_hi) return;
+    fly_list<L, P>(a, n_in);
+}
+
+// Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
+#define PR_FUSED(X) \
+    X(0, 4, 16) X(1, 8, 12) X(2, 8, 16) X(3, 8, 20) X(4, 8, 24) X(5, 16, 12) X(6, 16, 16) X(7, 16, 22) \
+    X(8, 32, 16) X(9, 32, 24) X(10, 64, 3) X(11, 64, 6) X(12, 64, 8) X(13, 64, 12) X(14, 64, 16)
+
+// One chunk of every bucket in one launch: each wave drains bucket 0's list, then bucket
+// 1's, and so on, flying each bucket with its own (L, P) code path -- its many-per-wave
+// mapping, or its one-flight-per-wave mapping when the bucket's flights fit one per wave.
+__global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
+    if (blockIdx.x == 0 && threadIdx.x < A.nb) {
+        A.counts[PR_CNT * threadIdx.x + A.c_zero] = 0;
+        A.counts[PR_CNT * threadIdx.x + 3 + A.h_zero] = 0;
+    }
+    // largest triangle counts first: the longest chunk jobs start first, small ones fill in
+    for (int b = A.nb - 1; b >= 0; b--) {
+        if (A.vid_p[b] < 0) continue;
+        int* cb = A.counts + PR_CNT * b;
+        const int n_in = cb[A.c_in];
+        if (n_in == 0) continue;
+        FlyArgs a = A.base;
+        a.list_in = (A.in_list ? A.lists1 : A.lists0) + (size_t)b * A.list_stride;
+        a.list_out = (A.in_list ? A.lists0 : A.lists1) + (size_t)b * A.list_stride;
+        a.count_in = cb + A.c_in;
+        a.count_out = cb + A.c_out;
+        a.head = cb + 3 + A.h_run;
+        const bool wide = A.vid_w[b] >= 0 && n_in <= A.thr[b];
+        a.stats = A.stats + 2 * (2 * b + (wide ? 1 : 0));
+        switch (wide ? A.vid_w[b] : A.vid_p[b]) {
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in); break;
+            PR_FUSED(CASE)
+#undef CASE
+            default: break;
+        }
+    }
+}
+
+// Counter clearing for a chunk that has no fused launch (every bucket on its own k_fly).
+__global__ void k_zero_chunk(int* counts, int nb, int c_zero, int h_zero) {
+    const int t = threadIdx.x;
+    if (t < nb) {
+        counts[PR_CNT * t + c_zero] = 0;
+        counts[PR_CNT * t + 3 + h_zero] = 0;
+    }
+}
+
+hipError_t launch_zero_chunk(int* counts, int nb, int c_zero, int h_zero, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_chunk, dim3(1), dim3(64), 0, s, counts, nb, c_zero, h_zero);
+    return hipGetLastError();
+}
+
+int fused_variant_id(int L, int P) {
+#define ID(I, LL, PP) if (L == LL && P == PP) return I;
+    PR_FUSED(ID)
+#undef ID
+    return -1;
+}
+
+hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_fly_all, dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t fly_all_occupancy(int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fly_all, 64, 0);
 }
 
 // Pose recording (sim.cu:117-128): one group per flight over the whole horizon.  Frames
@@ -635,11 +704,6 @@ hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, f
 hipError_t launch_queue(const QueueArgs& a, hipStream_t s) {
     if (a.n_flights <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_init, dim3((a.n_flights + 255) / 256), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_zero_pair(int* p, int* q, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_pair, dim3(1), dim3(64), 0, s, p, q);
     return hipGetLastError();
 }
 

@@ -118,6 +118,7 @@ struct pr_pool {
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int buf_n = 0, buf_head = 0;
+    int collect = 0;  // plain generations also feed the replay buffer
     std::vector<float> acc, samples, champion;
     std::vector<int> idx;
     float champion_score = -1e9f;
@@ -256,8 +257,9 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     p->samples.resize((size_t)nf * p->nseeds);
     float holdout = 0;
     int champ = 0;
+    const bool collect = model != nullptr || p->collect;
     PRC(pr_gen_evaluate(p->gen_eval, fly, nullptr, nf, seeds.data(), p->nseeds, 8, prm, p->acc.data(), p->idx.data(),
-                        model ? p->samples.data() : nullptr, &holdout, &champ));
+                        collect ? p->samples.data() : nullptr, &holdout, &champ));
     pr_stats st_main, st_hold;
     pr_gen_stats(p->gen_eval, &st_main, &st_hold);
     // new elite: best simulated scores
@@ -268,14 +270,14 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
     p->champion_score = holdout;
     float loss = 0;
-    if (model) {
+    if (collect) {
         auto m0 = std::chrono::steady_clock::now();
         PCK(hipMemcpy(p->d_samples, p->samples.data(), sizeof(float) * p->samples.size(), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_targets, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_samples, nf, p->nseeds, p->d_tgt);
         PCK(hipGetLastError());
         int rc = buffer_add(p, fly, p->d_tgt, nf);
         if (rc) return rc;
-        if (train_epochs > 0) PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, train_epochs, 4096, lr, &loss));
+        if (model && train_epochs > 0) PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, train_epochs, 4096, lr, &loss));
         PCK(hipDeviceSynchronize());
         mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
     }
@@ -302,6 +304,14 @@ int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* sample
     hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, 0, p->d_samples, n, p->nseeds, p->d_tgt);
     PCK(hipGetLastError());
     return buffer_add(p, d_genomes, p->d_tgt, n);
+}
+
+// 1: plain generations (model = null) also add their flown pairs to the replay buffer,
+// e.g. to bootstrap the surrogate before pre-screening starts.
+int pr_pool_set_collect(pr_pool* p, int on) {
+    if (!p) { g_pool_err = "pr_pool_set_collect: null pool"; return -1; }
+    p->collect = on ? 1 : 0;
+    return 0;
 }
 
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss) {

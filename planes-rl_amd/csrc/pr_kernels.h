@@ -59,12 +59,39 @@ struct FlyArgs {
     unsigned long long* stats;  // [0] flight-steps, [1] panel-steps
 };
 
+// Per-bucket chunk counters (PR_CNT ints per bucket): [0..2] list counts, rotating with the
+// chunk index mod 3 (in, out, cleared for the next chunk), [3..4] queue heads, rotating mod 2.
+#define PR_CNT 5
+
+// All buckets' chunk in one persistent launch (k_fly_all).  Bucket b's lists are
+// lists{0,1} + b * list_stride (list_in = lists0 when in_list == 0); its counters are
+// counts + PR_CNT * b.  vid_p / vid_w: its fused-variant ids (many flights per wave / one
+// per wave, -1 none: the bucket is flown by its own k_fly launch); vid_w takes over once
+// the bucket's count is <= thr.  Block 0 clears the counters of the next chunk for every
+// bucket (c_zero, h_zero), so a chunk is one launch.
+struct FlyAllArgs {
+    FlyArgs base;
+    int* lists0;
+    int* lists1;
+    int list_stride;
+    int in_list;
+    int* counts;
+    int c_in, c_out, c_zero, h_run, h_zero;
+    int nb;
+    int vid_p[8], vid_w[8], thr[8];
+    unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
+};
+
+int fused_variant_id(int L, int P);
+hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
+hipError_t fly_all_occupancy(int* blocks_per_cu);
+hipError_t launch_zero_chunk(int* counts, int nb, int c_zero, int h_zero, hipStream_t s);
+
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);
 hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, float* out, hipStream_t s);
 hipError_t launch_queue(const QueueArgs& a, hipStream_t s);
 hipError_t launch_zero(int* p, int n, hipStream_t s);
-hipError_t launch_zero_pair(int* p, int* q, hipStream_t s);
 hipError_t launch_fly(int L, int P, const FlyArgs& a, int grid, hipStream_t s);
 hipError_t fly_occupancy(int L, int P, int* blocks_per_cu);
 hipError_t launch_fly_rec(int L, int P, const FlyArgs& a, int n_flights, float* rec, int rec_every, int rec_cap,

@@ -42,11 +42,14 @@ class Stats(C.Structure):
                 ("fly_launches", C.c_int), ("prep_ms", C.c_double), ("n_slots", C.c_int),
                 ("s_nt_max", C.c_int * NS_MAX), ("s_L", C.c_int * NS_MAX), ("s_P", C.c_int * NS_MAX),
                 ("s_launches", C.c_int * NS_MAX), ("s_flight_steps", C.c_ulonglong * NS_MAX),
-                ("s_panel_steps", C.c_ulonglong * NS_MAX), ("s_fly_ms", C.c_double * NS_MAX)]
+                ("s_panel_steps", C.c_ulonglong * NS_MAX), ("s_fly_ms", C.c_double * NS_MAX),
+                ("s_fused", C.c_int * NS_MAX), ("fused_ms", C.c_double), ("fused_launches", C.c_int)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms")}
+        d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms",
+                                           "fused_ms", "fused_launches")}
         d["buckets"] = [{"nt_max": self.s_nt_max[i], "L": self.s_L[i], "P": self.s_P[i], "tail": i % 2 == 1,
+                         "fused": bool(self.s_fused[i]),
                          "launches": self.s_launches[i], "flight_steps": self.s_flight_steps[i],
                          "panel_steps": self.s_panel_steps[i], "fly_ms": self.s_fly_ms[i]}
                         for i in range(self.n_slots) if self.s_L[i] > 0]
@@ -86,6 +89,9 @@ def _load():
     lib.pr_read_extent.argtypes = [VP, C.c_int, F32P]
     lib.pr_set_graph.argtypes = [VP, C.c_int]
     lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
+    lib.pr_set_fused.argtypes = [VP, C.c_int]
+    lib.pr_gen_context.argtypes = [VP, C.c_int]
+    lib.pr_gen_context.restype = VP
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
     lib.pr_mlp_last_error.restype = C.c_char_p
     lib.pr_pool_last_error.restype = C.c_char_p
@@ -97,6 +103,7 @@ def _load():
     lib.pr_pool_generation.argtypes = [VP, C.c_int, C.POINTER(SimParams), VP, C.c_int, C.c_float,
                                        np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
     lib.pr_pool_train.argtypes = [VP, VP, C.c_int, C.c_float, C.POINTER(C.c_float)]
+    lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_mlp_param_count.restype = C.c_int
     lib.pr_mlp_create.argtypes = [C.POINTER(VP), C.c_int, C.c_uint]
@@ -279,7 +286,7 @@ class Planes:
         nf = n_genomes * n_scen if layout == 0 else n_genomes
         return self.read_states(0, nf)
 
-    def evaluate(self, genomes, seeds, params=None, layout=0, res=None):
+    def evaluate(self, genomes, seeds, params=None, layout=0, res=None):  # noqa: D417
         """Batched evaluation: every genome under every seed (layout 0) or genome i under
         seed i (layout 1).  Returns fitness[n_genomes, n_seeds] (layout 0) or [n]."""
         params = params or SimParams.reference()
@@ -304,6 +311,10 @@ class Planes:
 
     def set_tail_mapping(self, on=True):
         _ck(LIB.pr_set_tail_mapping(self.h, int(on)), "pr_set_tail_mapping")
+
+    def set_fused(self, on=True):
+        """One fused flight launch per chunk for all buckets (default) or one per bucket."""
+        _ck(LIB.pr_set_fused(self.h, int(on)), "pr_set_fused")
 
     def set_graph(self, on=True):
         """Capture pr_fly's launch sequence as a hipGraph and replay it."""
@@ -434,6 +445,15 @@ class Generation:
         _ck(LIB.pr_gen_stats(self.h, C.byref(a), C.byref(b)), "pr_gen_stats")
         return a.as_dict(), b.as_dict()
 
+    def set_fused(self, on=True):
+        """Fused flight kernel (default) or one launch per bucket, for both flight contexts."""
+        for which in (0, 1):
+            _ck(LIB.pr_set_fused(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_fused")
+
+    def set_graph(self, on=True):
+        for which in (0, 1):
+            _ck(LIB.pr_set_graph(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_graph")
+
     def __del__(self):
         try:
             LIB.pr_gen_destroy(self.h)
@@ -533,6 +553,7 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         gi = sel[:ng]
         pl = Planes(ng, len(seeds), ng * len(seeds), max_ch=max_ch, max_sp=max_sp)
         pl.set_tail_mapping(False)  # time the candidate mapping itself on every flight
+        pl.set_fused(False)  # per-mapping launch timings
         pl.prepare(g[gi], None if r is None else r[gi])
         pl.scenarios(seeds, tune_params)
         best = None
@@ -680,6 +701,10 @@ class PoolGA:
         self._ck(LIB.pr_pool_generation(self.h, gen, C.byref(params), surrogate.h if surrogate else None,
                                         train_epochs, lr, out), "pr_pool_generation")
         return dict(zip(self.FIELDS, out.tolist()))
+
+    def set_collect(self, on=True):
+        """Plain generations also feed the surrogate's replay buffer (bootstrap)."""
+        self._ck(LIB.pr_pool_set_collect(self.h, int(on)), "pr_pool_set_collect")
 
     def train(self, surrogate, epochs, lr=1e-3):
         loss = C.c_float()

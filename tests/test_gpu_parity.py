@@ -1,6 +1,6 @@
 """GPU path (libplanes_hip.so through its C ABI) against the oracle on the same inputs.
 
-Bars (BASELINE north_star): scenario fields, triangle counts, mesh/mass/panels and
+Bars: scenario fields, triangle counts, mesh/mass/panels and
 launch states bit-exact with the FP32 oracle; mass properties within 1e-6 of FP64;
 trajectories within 1e-4 (normwise) of FP64 over the first 0.5 s; per-flight distance
 within 1 % of FP64 (the fraction meeting it is asserted and printed per config); rank
@@ -196,6 +196,36 @@ def test_tail_mapping_equivalent(PA):
     assert np.mean(rel < 1e-2) >= 0.97
 
 
+@pytest.mark.parametrize("buckets", [None, ([192, 1536], [8, 64], [24, 24])])
+def test_fused_kernel_matches_per_bucket_launches(PA, buckets):
+    """The fused flight kernel (every bucket drained by one launch per 1-s chunk) flies
+    each flight with the same lane mapping as per-bucket launches: bit-identical fitness
+    and step counts.  Second case: the (64, 24) bucket is not compiled into the fused
+    kernel and gets its own launches beside it."""
+    gs = O.random_genomes(96, 21)  # triangle counts 176..180 and 340..344: two buckets
+    seeds = np.arange(1, 9, dtype=np.int32)
+    params = PA.SimParams.reference(max_steps=3000)
+    out = []
+    for fused in (False, True):
+        pl = PA.Planes(96, 8, 768)
+        pl.set_tail_mapping(False)
+        if buckets:
+            pl.set_buckets(*buckets)
+        pl.set_fused(fused)
+        fit, steps = pl.evaluate(gs, seeds, params)
+        st = pl.stats()
+        pl.close()
+        out.append((fit.reshape(-1), steps.reshape(-1), st))
+    (f0, s0, st0), (f1, s1, st1) = out
+    assert np.array_equal(f0, f1) and np.array_equal(s0, s1)
+    assert st0["flight_steps"] == st1["flight_steps"] == int(s0.sum())
+    assert st0["fused_launches"] == 0 and st1["fused_launches"] == 5  # one per 1-s chunk
+    assert any(b["fused"] for b in st1["buckets"])
+    if buckets:
+        assert not all(b["fused"] for b in st1["buckets"])
+        assert sum(b["launches"] for b in st1["buckets"]) == 5  # the (64, 24) bucket's own chunks
+
+
 def test_graph_replay_identical(PA):
     """pr_fly captured as a hipGraph and replayed gives bit-identical results to direct
     launches, on every replay, with other eager work in between."""
@@ -304,3 +334,28 @@ def test_full_size_generation_properties(PA):
     st, hold = E.stats()
     print(f"10000 x 8: {st['flight_steps'] / 600:.0f} plane-flight-s in {st['fly_ms']:.1f} ms of flight kernels")
     assert st["flight_steps"] > 0 and hold["flight_steps"] > 0
+
+
+def test_autotuner_picks_valid_mapping_and_results_hold(PA, tmp_path):
+    """The autotuner times every (L, P) variant covering each triangle-count range on real
+    flights of a mixed-resolution population and persists its choice; flights under the
+    tuned mapping match the default mapping."""
+    import json
+    g, res, _ = PA.family_population(600, seed=5, nt_min=200, nt_max=900)
+    seeds = np.array([1, 2], np.int32)
+    params = PA.SimParams.config(gust_rms_max=4.0, max_steps=1200)
+    path = tmp_path / "tune.json"
+    nt_max, Ls, Ps = PA.autotune(g, res, seeds, params, sample_flights=1024, steps=300, out_json=str(path))
+    cfg = json.loads(path.read_text())
+    assert cfg == {"nt_max": nt_max, "L": Ls, "P": Ps}
+    assert all(L * P >= n for n, L, P in zip(nt_max, Ls, Ps)) and nt_max == sorted(nt_max)
+    out = []
+    for tuned in (False, True):
+        pl = PA.Planes(600, 2, 1200, max_ch=int(res[:, 0].max()), max_sp=int(res[:, 1].max()))
+        if tuned:
+            pl.set_buckets(nt_max, Ls, Ps)
+        fit, _ = pl.evaluate(g, seeds, params, res=res)
+        out.append(fit.reshape(-1))
+    rel = np.abs(out[0] - out[1]) / np.maximum(np.abs(out[0]), 1e-3)
+    print("autotuned buckets:", list(zip(nt_max, Ls, Ps)), f"agreement {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3")
+    assert np.mean(rel < 1e-2) >= 0.97

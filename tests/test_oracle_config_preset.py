@@ -1,0 +1,85 @@
+"""The parametric scenario preset (scenario_cfg, lane_normal) has no counterpart in the
+reference, so nothing pins it bit-for-bit against the reference: parity unpinned.  These
+tests pin it by its definition instead -- the distributions each field is documented to
+follow, and the gust-field RMS constant that scales the gust amplitude -- chosen so that a
+wrong lane, scale, offset or dropped sinusoid fails one of them."""
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle_lib as O
+
+S_SPEED, S_PITCH, S_YAW, S_ROLL, S_HEIGHT, S_RR, S_PR, S_YR, S_RHO, S_CF, S_GUST = range(11)
+S_BASE_X, S_BASE_Y, S_FPITCH, S_FYAW = 11, 12, 13, 14
+S_MATERIAL, S_INFILL, S_WARP_P, S_WARP_R, S_BASE_Z = 18, 19, 23, 24, 25
+DEG = 0.0174532925
+
+
+def test_lane_normal_is_standard_normal():
+    """Box-Muller on two hashed 24-bit uniforms: N(0, 1) by KS test and by moments."""
+    L = O.lib()
+    x = np.array([L.or_lane_normal(s, lane) for s in range(1, 20001) for lane in (32, 44, 52)])
+    assert abs(x.mean()) < 0.02 and abs(x.var() - 1.0) < 0.03
+    assert abs(stats.skew(x)) < 0.05 and abs(stats.kurtosis(x)) < 0.1
+    assert stats.kstest(x, "norm").pvalue > 1e-3
+    assert np.isfinite(x).all() and np.abs(x).max() < 6.5  # u1 > 0: no infinite radius
+
+
+def test_config_preset_field_distributions():
+    gust, warp, wsd = 4.0, 1.0 * DEG, 2.0
+    seeds = np.arange(1, 40001, dtype=np.int32)
+    d = O.scenarios(seeds, preset=1, gust_rms_max=gust, warp_max=warp, wind_sd=wsd).astype(np.float64)
+
+    def uniform(col, lo, hi):
+        v = d[:, col]
+        assert v.min() >= lo - 1e-6 and v.max() <= hi + 1e-6, col
+        assert stats.kstest((v - lo) / (hi - lo), "uniform").pvalue > 1e-3, col
+
+    def normal(col, mu, sd):
+        v = d[:, col]
+        assert abs(v.mean() - mu) < 4 * sd / np.sqrt(len(v)), col
+        assert abs(v.std() / sd - 1.0) < 0.02, col
+        assert stats.kstest((v - mu) / sd, "norm").pvalue > 1e-3, col
+
+    uniform(S_SPEED, 7.2, 11.2)
+    uniform(S_HEIGHT, 1.5, 2.0)
+    uniform(S_RHO, 1.15, 1.25)
+    uniform(S_MATERIAL, 0.95, 1.05)
+    uniform(S_INFILL, 0.06, 0.40)
+    uniform(S_WARP_P, -warp, warp)
+    uniform(S_WARP_R, -warp, warp)
+    normal(S_PITCH, 0.10, 5 * DEG)
+    for c in (S_YAW, S_ROLL, S_FPITCH, S_FYAW):
+        normal(c, 0.0, 5 * DEG)
+    for c in (S_RR, S_PR, S_YR):
+        normal(c, 0.0, 30 * DEG)
+    for c in (S_BASE_X, S_BASE_Y, S_BASE_Z):
+        normal(c, 0.0, wsd)
+    # the gust amplitude is drawn so that the field's 3-D RMS is U(0, gust_rms_max)
+    uniform_rms = d[:, S_GUST] * 0.69191763
+    assert uniform_rms.max() <= gust + 1e-5
+    assert stats.kstest(uniform_rms / gust, "uniform").pvalue > 1e-3
+    # independent lanes: no two fields drawn from the same hash lane
+    c = np.corrcoef(d[:, [S_SPEED, S_HEIGHT, S_RHO, S_PITCH, S_YAW, S_BASE_X, S_BASE_Y, S_BASE_Z, S_GUST]].T)
+    assert np.abs(c - np.eye(len(c))).max() < 0.03
+
+
+def test_gust_field_rms_constant():
+    """3-D RMS of the 7-sinusoid gust field = amplitude * sqrt(sum(w^2) / 2) = 0.69192 *
+    amplitude (weights .55 .35 .25 / .50 .30 / .30 .20): the time average over a long
+    window of |wind - base|^2, in the FP64 oracle."""
+    sc = O.scenario(7, preset=1, gust_rms_max=4.0, warp_max=0.0, wind_sd=2.0)
+    sc[S_GUST] = 1.0
+    base = sc[[S_BASE_X, S_BASE_Y, S_BASE_Z]].astype(np.float64)
+    rng = np.random.default_rng(3)
+    acc = 0.0
+    n = 20000
+    for k in range(n):
+        p = rng.uniform(-20, 20, 3)
+        t = rng.uniform(0, 2000)
+        w = O.wind(p, t, 7, sc, prec=64) - base
+        acc += float(w @ w)
+    rms = np.sqrt(acc / n)
+    w2 = np.array([.55, .35, .25, .50, .30, .30, .20]) ** 2
+    assert abs(np.sqrt(w2.sum() / 2) - 0.69191763) < 1e-7
+    assert rms == pytest.approx(0.69191763, rel=0.015)
