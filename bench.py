@@ -148,6 +148,23 @@ def pool_ab(PA, args):
     print(json.dumps(line), flush=True)
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of known size: reads mib MiB and writes mib MiB per
+    dispatch, so FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters' units."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[-16:], src[-16:])
+    log(f"copy probe: 5 x {mib} MiB read + {mib} MiB written")
+    print(json.dumps({"copy_probe_mib": mib, "dispatches": 5, "bytes_read_per_dispatch": n * 4,
+                      "bytes_written_per_dispatch": n * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +178,11 @@ def main():
     ap.add_argument("--buffer", type=int, default=500_000, help="c4: surrogate replay buffer")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               f"(all-gather of fitness samples over RCCL) is not run here", file=sys.stderr)
@@ -234,6 +255,11 @@ def main():
             "traffic": None, "avg_launch_ms": round(ms / max(1, nl), 4), "launches": nl,
             "flops_per_launch": round(flops / max(1, nl))}
     log(f"  k_fly_all: {fused['launches']} launches, {fused['ms']:.2f} ms")
+    last = E.stats()[0]
+    for k, ch in enumerate(last["chunks"]):
+        fl = FLOP_PANEL * ch["panel_steps"] + FLOP_PLANE * ch["flight_steps"]
+        log(f"  last generation chunk {k}: {ch['flights']} flights, {ch['flight_steps']} flight-steps, "
+            f"{ch['ms']:.2f} ms, {fl / max(ch['ms'], 1e-9) / 1e9:.1f} TFLOP/s")
     for kk, bb in sorted(bucket.items()):
         log(f"  {'(fused) ' if bb['fused'] else ''}k_fly<{kk[0]},{kk[1]}>{' tail' if kk[2] else ''}: "
             f"{bb['launches']} own launches, {bb['fly_ms']:.2f} ms, "

@@ -34,6 +34,7 @@ typedef struct pr_sim_params {
 } pr_sim_params;
 
 #define PR_MAX_SLOTS 16
+#define PR_MAX_CHUNKS 64
 typedef struct pr_stats {
     unsigned long long flight_steps; /* integrated 600 Hz steps of the last pr_fly (= plane-flight-s * 600) */
     unsigned long long panel_steps;  /* sum over those steps of the plane's triangle count */
@@ -47,6 +48,12 @@ typedef struct pr_stats {
     int s_fused[PR_MAX_SLOTS];       /* 1: the slot is flown inside the fused kernel */
     double fused_ms;                 /* GPU time of the fused flight-kernel launches */
     int fused_launches;
+    /* per 1-s chunk (chunks past PR_MAX_CHUNKS - 1 accumulate in the last entry): flights
+     * entering it, their integrated steps and panel-steps, flight-kernel time */
+    int n_chunks;
+    int c_flights[PR_MAX_CHUNKS];
+    unsigned long long c_flight_steps[PR_MAX_CHUNKS], c_panel_steps[PR_MAX_CHUNKS];
+    double c_ms[PR_MAX_CHUNKS];
 } pr_stats;
 
 /* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).

@@ -65,6 +65,8 @@ struct pr_ctx {
     int* d_counts = nullptr;  // PR_CNT per bucket: 3 rotating list counts, 2 rotating queue heads
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
+    unsigned long long* d_cstats = nullptr;  // [PR_MAX_CHUNKS][2] per-chunk flight-/panel-steps
+    int* d_chunk_n = nullptr;                // [PR_MAX_CHUNKS][NB_MAX] flights entering each chunk
     std::vector<Bucket> buckets;
     std::vector<int> grid;    // persistent grid per bucket
     std::vector<int> grid_w;  // persistent grid of the bucket's one-flight-per-wave mapping
@@ -72,6 +74,7 @@ struct pr_ctx {
     int grid_f = 0;                 // persistent grid of the fused kernel
     hipEvent_t ev[2 * 512 + 2] = {};
     int ev_slot[512] = {};          // slot of a timed launch; -1 = the fused kernel
+    int ev_chunk[512] = {};         // chunk of a timed launch
     hipEvent_t g0 = nullptr, g1 = nullptr;
     // hipGraph of the whole flight sequence (optional), keyed by its launch parameters
     int use_graph = 0;
@@ -94,7 +97,7 @@ static int mesh_bounds(int ch, int sp, int* nv, int* nt) {
 }
 
 static int wide_for(int nt_max, int* Lw, int* Pw) {
-    static const int Ps[] = {3, 6, 8, 12, 16, 24, 32};
+    static const int Ps[] = {8, 16, 24, 32};
     for (int p : Ps)
         if (64 * p >= nt_max) { *Lw = 64; *Pw = p; return 0; }
     *Lw = 0; *Pw = 0;
@@ -102,8 +105,10 @@ static int wide_for(int nt_max, int* Lw, int* Pw) {
 }
 
 static int default_buckets(pr_ctx* c) {
-    static const int table[][3] = {{192, 8, 24}, {352, 16, 22}, {512, 32, 16}, {768, 32, 24},
-                                   {1024, 64, 16}, {1536, 64, 24}, {2048, 64, 32}};
+    // smallest group whose lanes hold the flight's panels in <= 24 slots each; the kernel
+    // skips the slots a wave does not need, so one bucket per group width suffices
+    static const int table[][3] = {{192, 8, 24}, {384, 16, 24}, {768, 32, 24}, {1024, 64, 16},
+                                   {1536, 64, 24}, {2048, 64, 32}};
     int nv, nt;
     mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
     c->buckets.clear();
@@ -200,6 +205,8 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_counts, sizeof(int) * PR_CNT * NB_MAX);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
     ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
+    ALLOC(c->d_cstats, sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
+    ALLOC(c->d_chunk_n, sizeof(int) * PR_MAX_CHUNKS * NB_MAX);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream) !=
         hipSuccess) { pr_destroy(c); return fail(-1, "scan workspace query"); }
     ALLOC(c->d_scan_tmp, c->scan_tmp_bytes + 16);
@@ -217,7 +224,8 @@ void pr_destroy(pr_ctx* c) {
     // teardown: release everything; failures here have nowhere to be reported
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
-                    c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp};
+                    c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp,
+                    c->d_cstats, c->d_chunk_n};
     for (void* b : bufs) if (b) (void)hipFree(b);
     for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -334,6 +342,8 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     const int NB = (int)c->buckets.size();
     HCK(launch_zero(c->d_counts, PR_CNT * NB_MAX, c->stream));
     HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 8 * NB_MAX, c->stream));
+    HCK(launch_zero(reinterpret_cast<int*>(c->d_cstats), 4 * PR_MAX_CHUNKS, c->stream));
+    HCK(launch_zero(c->d_chunk_n, PR_MAX_CHUNKS * NB_MAX, c->stream));
     QueueArgs q;
     q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
     q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
@@ -379,16 +389,19 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
     }
     int ne = 0;
     for (int ck = 0; ck < n_chunks; ck++) {
+        const int cs = ck < PR_MAX_CHUNKS ? ck : PR_MAX_CHUNKS - 1;  // later chunks share the last slot
         F.in_list = ck & 1;
         F.c_in = ck % 3; F.c_out = (ck + 1) % 3; F.c_zero = (ck + 2) % 3;
         F.h_run = ck & 1; F.h_zero = (ck + 1) & 1;
+        F.base.cstats = a.cstats = c->d_cstats + 2 * cs;
+        F.chunk_n = c->d_chunk_n + NB_MAX * cs;
         if (any_fused) {
             const bool tev = timed && ne + 2 <= n_ev;
             if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
             HCK(launch_fly_all(F, c->grid_f, c->stream));
-            if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; ne += 2; }
+            if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; c->ev_chunk[ne / 2] = cs; ne += 2; }
         } else {
-            HCK(launch_zero_chunk(c->d_counts, NB, F.c_zero, F.h_zero, c->stream));
+            HCK(launch_zero_chunk(c->d_counts, NB, F.c_in, F.c_zero, F.h_zero, F.chunk_n, c->stream));
         }
         if (!any_own) continue;
         for (int b = 0; b < NB; b++) {
@@ -410,7 +423,12 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
                 if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
                 if (w == 0) HCK(launch_fly(bk.L, bk.P, a, c->grid[b], c->stream));
                 else HCK(launch_fly(bk.Lw, bk.Pw, a, c->grid_w[b], c->stream));
-                if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = 2 * b + w; ne += 2; }
+                if (tev) {
+                    HCK(hipEventRecord(c->ev[ne + 1], c->stream));
+                    c->ev_slot[ne / 2] = 2 * b + w;
+                    c->ev_chunk[ne / 2] = cs;
+                    ne += 2;
+                }
             }
         }
     }
@@ -490,6 +508,20 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
         const int sl = c->ev_slot[k / 2];
         if (sl < 0) { S.fused_ms += t; S.fused_launches += 1; }
         else { S.s_fly_ms[sl] += t; S.s_launches[sl] += 1; }
+        S.c_ms[c->ev_chunk[k / 2]] += t;
+    }
+    {
+        const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
+        S.n_chunks = n_chunks < PR_MAX_CHUNKS ? n_chunks : PR_MAX_CHUNKS;
+        unsigned long long cst[2 * PR_MAX_CHUNKS];
+        int cn[PR_MAX_CHUNKS * NB_MAX];
+        HCK(hipMemcpy(cst, c->d_cstats, sizeof cst, hipMemcpyDeviceToHost));
+        HCK(hipMemcpy(cn, c->d_chunk_n, sizeof cn, hipMemcpyDeviceToHost));
+        for (int k = 0; k < S.n_chunks; k++) {
+            S.c_flight_steps[k] = cst[2 * k];
+            S.c_panel_steps[k] = cst[2 * k + 1];
+            for (int b = 0; b < NB; b++) S.c_flights[k] += cn[NB_MAX * k + b];
+        }
     }
     for (int b = 0; b < NB; b++) S.s_fused[2 * b] = S.s_fused[2 * b + 1] = c->vid_p[b] >= 0;
     if (c->use_graph) {

@@ -47,6 +47,13 @@ __device__ __forceinline__ float group_max(float v) {
     return v;
 }
 
+// Largest panel-slot count over the wave's groups, as a wave-uniform (scalar) value; every
+// lane of the wave must be active.
+template <int P>
+__device__ __forceinline__ int wave_slots(int np) {
+    const int m = (int)group_max<64>((float)np);
+    return __builtin_amdgcn_readfirstlane(m < P ? m : P);
+}
 
 // ------------------------------------------------------------------ k_layout
 __global__ void k_layout(const float* __restrict__ genomes, const int* __restrict__ res, int def_ch, int def_sp,
@@ -325,6 +332,7 @@ struct Flyer {
     Body B;
     WindLane<L> W;
     float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];
+    int np;  // panel slots this group uses per lane: ceil(nt / L) <= P
     bool bad;
 
     __device__ __forceinline__ void setup(const FlyArgs& a, int fid, int j) {
@@ -333,6 +341,7 @@ struct Flyer {
         const int gi = fc.genome;
         B.fid = fid;
         B.nt = a.nt[gi];
+        np = (B.nt + L - 1) / L;
         const int off = a.off[gi];
         B.px = st.pos[0]; B.py = st.pos[1]; B.pz = st.pos[2];
         B.qw = st.q[0]; B.qx = st.q[1]; B.qy = st.q[2]; B.qz = st.q[3];
@@ -403,8 +412,10 @@ struct Flyer {
     }
 
     // One step (sim.cu:129-205) for the group; returns true when the flight ended (ground,
-    // NaN, out of bounds or the step cap).
-    __device__ __forceinline__ bool step(int max_steps) {
+    // NaN, out of bounds or the step cap).  npw: wave-uniform panel-slot count (the most any
+    // group of the wave uses); slots beyond it hold only zero-area padding and are skipped
+    // by a scalar branch -- their force is exactly zero, so skipping changes no bit.
+    __device__ __forceinline__ bool step(int max_steps, int npw) {
         const float qw = B.qw, qx = B.qx, qy = B.qy, qz = B.qz;
         const float r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
         const float r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
@@ -428,7 +439,8 @@ struct Flyer {
         const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
         float Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
 #pragma unroll
-        for (int p = 0; p < P; p++) panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+        for (int p = 0; p < P; p++)
+            if (p < npw) panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
         Tx = group_sum<L>(Tx); Ty = group_sum<L>(Ty); Tz = group_sum<L>(Tz);
         // rigid-body update (sim.cu:177-197); torque is already in body axes
@@ -497,6 +509,8 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
     bool active = false, exhausted = false;
     unsigned long long fsteps = 0, psteps = 0;
     F.B.nt = 0;
+    F.np = 0;
+    int npw = 0;
 
     for (;;) {
         const bool need = !active && !exhausted;
@@ -508,10 +522,11 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
                 if (k >= n_in) exhausted = true;
                 else { F.setup(a, a.list_in[k], j); active = true; }
             }
+            npw = wave_slots<P>(active ? F.np : 0);
         }
         if (!__ballot(active)) break;
         if (!active) continue;
-        const bool fin = F.step(a.max_steps);
+        const bool fin = F.step(a.max_steps, npw);
         fsteps += 1;
         psteps += (unsigned long long)F.B.nt;
         if (fin || F.B.step >= F.B.end) {
@@ -538,6 +553,10 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
     if (lane == 0 && fsteps) {
         atomicAdd(a.stats + 0, fsteps);
         atomicAdd(a.stats + 1, psteps);
+        if (a.cstats) {
+            atomicAdd(a.cstats + 0, fsteps);
+            atomicAdd(a.cstats + 1, psteps);
+        }
     }
 }
 
@@ -552,8 +571,7 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
 
 // Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
 #define PR_FUSED(X) \
-    X(0, 4, 16) X(1, 8, 12) X(2, 8, 16) X(3, 8, 20) X(4, 8, 24) X(5, 16, 12) X(6, 16, 16) X(7, 16, 22) \
-    X(8, 32, 16) X(9, 32, 24) X(10, 64, 3) X(11, 64, 6) X(12, 64, 8) X(13, 64, 12) X(14, 64, 16)
+    X(0, 8, 12) X(1, 8, 24) X(2, 16, 12) X(3, 16, 24) X(4, 32, 12) X(5, 32, 24) X(6, 64, 8) X(7, 64, 16)
 
 // One chunk of every bucket in one launch: each wave drains bucket 0's list, then bucket
 // 1's, and so on, flying each bucket with its own (L, P) code path -- its many-per-wave
@@ -562,6 +580,7 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     if (blockIdx.x == 0 && threadIdx.x < A.nb) {
         A.counts[PR_CNT * threadIdx.x + A.c_zero] = 0;
         A.counts[PR_CNT * threadIdx.x + 3 + A.h_zero] = 0;
+        if (A.chunk_n) A.chunk_n[threadIdx.x] = A.counts[PR_CNT * threadIdx.x + A.c_in];
     }
     // largest triangle counts first: the longest chunk jobs start first, small ones fill in
     for (int b = A.nb - 1; b >= 0; b--) {
@@ -587,16 +606,17 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
 }
 
 // Counter clearing for a chunk that has no fused launch (every bucket on its own k_fly).
-__global__ void k_zero_chunk(int* counts, int nb, int c_zero, int h_zero) {
+__global__ void k_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n) {
     const int t = threadIdx.x;
     if (t < nb) {
         counts[PR_CNT * t + c_zero] = 0;
         counts[PR_CNT * t + 3 + h_zero] = 0;
+        if (chunk_n) chunk_n[t] = counts[PR_CNT * t + c_in];
     }
 }
 
-hipError_t launch_zero_chunk(int* counts, int nb, int c_zero, int h_zero, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_chunk, dim3(1), dim3(64), 0, s, counts, nb, c_zero, h_zero);
+hipError_t launch_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_chunk, dim3(1), dim3(64), 0, s, counts, nb, c_in, c_zero, h_zero, chunk_n);
     return hipGetLastError();
 }
 
@@ -642,7 +662,7 @@ __global__ __launch_bounds__(64) void k_fly_rec(FlyArgs a, int n_flights, float*
             rec_i++;
         }
         if (done) continue;
-        const bool fin = F.step(a.max_steps);
+        const bool fin = F.step(a.max_steps, P);
         if (fin) {
             done = true;
             if (j == 0) {
@@ -721,8 +741,7 @@ static hipError_t fly_one(const FlyArgs& a, int grid, hipStream_t s) {
 
 // compiled (L, P) variants; capacity L*P triangles per flight
 #define PR_VARIANTS(X) \
-    X(4, 16) X(8, 12) X(8, 16) X(8, 20) X(8, 24) X(16, 12) X(16, 16) X(16, 22) X(32, 16) X(32, 24) \
-    X(64, 3) X(64, 6) X(64, 8) X(64, 12) X(64, 16) X(64, 24) X(64, 32)
+    X(8, 12) X(8, 24) X(16, 12) X(16, 24) X(32, 12) X(32, 24) X(64, 8) X(64, 16) X(64, 24) X(64, 32)
 
 int fly_variant_count() {
     int n = 0;

@@ -57,6 +57,7 @@ struct FlyArgs {
     int run_lo, run_hi;    // the launch works only if run_lo <= flights in list_in <= run_hi
     float spread;
     unsigned long long* stats;  // [0] flight-steps, [1] panel-steps
+    unsigned long long* cstats; // this chunk's [0] flight-steps, [1] panel-steps (null: none)
 };
 
 // Per-bucket chunk counters (PR_CNT ints per bucket): [0..2] list counts, rotating with the
@@ -80,12 +81,13 @@ struct FlyAllArgs {
     int nb;
     int vid_p[8], vid_w[8], thr[8];
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
+    int* chunk_n;               // [nb] flights entering this chunk per bucket (null: none)
 };
 
 int fused_variant_id(int L, int P);
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
 hipError_t fly_all_occupancy(int* blocks_per_cu);
-hipError_t launch_zero_chunk(int* counts, int nb, int c_zero, int h_zero, hipStream_t s);
+hipError_t launch_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n, hipStream_t s);
 
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);

@@ -35,6 +35,7 @@ class SimParams(C.Structure):
 
 
 NS_MAX = 16
+NC_MAX = 64
 
 
 class Stats(C.Structure):
@@ -43,7 +44,9 @@ class Stats(C.Structure):
                 ("s_nt_max", C.c_int * NS_MAX), ("s_L", C.c_int * NS_MAX), ("s_P", C.c_int * NS_MAX),
                 ("s_launches", C.c_int * NS_MAX), ("s_flight_steps", C.c_ulonglong * NS_MAX),
                 ("s_panel_steps", C.c_ulonglong * NS_MAX), ("s_fly_ms", C.c_double * NS_MAX),
-                ("s_fused", C.c_int * NS_MAX), ("fused_ms", C.c_double), ("fused_launches", C.c_int)]
+                ("s_fused", C.c_int * NS_MAX), ("fused_ms", C.c_double), ("fused_launches", C.c_int),
+                ("n_chunks", C.c_int), ("c_flights", C.c_int * NC_MAX), ("c_flight_steps", C.c_ulonglong * NC_MAX),
+                ("c_panel_steps", C.c_ulonglong * NC_MAX), ("c_ms", C.c_double * NC_MAX)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms",
@@ -53,6 +56,8 @@ class Stats(C.Structure):
                          "launches": self.s_launches[i], "flight_steps": self.s_flight_steps[i],
                          "panel_steps": self.s_panel_steps[i], "fly_ms": self.s_fly_ms[i]}
                         for i in range(self.n_slots) if self.s_L[i] > 0]
+        d["chunks"] = [{"flights": self.c_flights[k], "flight_steps": self.c_flight_steps[k],
+                        "panel_steps": self.c_panel_steps[k], "ms": self.c_ms[k]} for k in range(self.n_chunks)]
         return d
 
 
@@ -553,7 +558,6 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         gi = sel[:ng]
         pl = Planes(ng, len(seeds), ng * len(seeds), max_ch=max_ch, max_sp=max_sp)
         pl.set_tail_mapping(False)  # time the candidate mapping itself on every flight
-        pl.set_fused(False)  # per-mapping launch timings
         pl.prepare(g[gi], None if r is None else r[gi])
         pl.scenarios(seeds, tune_params)
         best = None
@@ -567,9 +571,12 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
             pl.fly_device(0, ng, len(seeds), tune_params)
             st = pl.stats()
             b0 = st["buckets"][0]
-            cost = b0["fly_ms"] / max(1, b0["panel_steps"])
+            # inside the fused kernel (at its occupancy) when the variant is compiled into
+            # it -- the catch-all bucket holds no flights, so the launch time is bucket 0's
+            ms = st["fused_ms"] if b0["fused"] else b0["fly_ms"]
+            cost = ms / max(1, b0["panel_steps"])
             if verbose:
-                print(f"  range ({lo},{hi}] L={L} P={P}: {b0['fly_ms']:.3f} ms, {cost * 1e9:.3f} ps/panel-step")
+                print(f"  range ({lo},{hi}] L={L} P={P}: {ms:.3f} ms, {cost * 1e9:.3f} ps/panel-step")
             if best is None or cost < best[0]:
                 best = (cost, L, P)
         pl.close()
@@ -582,6 +589,16 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         else:
             merged.append((hi, L, P))
     merged[-1] = (max(merged[-1][0], nt_cap), merged[-1][1], merged[-1][2])
+    # a context sized for (max_ch, max_sp) needs its last bucket to cover that resolution's
+    # triangle bound, not just the sample's largest count
+    bound = 8 * max_ch * max_sp + 24 if max_ch and max_sp else 8 * 5 * 8 + 24
+    if merged[-1][0] < bound:
+        hi, L, P = merged[-1]
+        if L * P >= bound:
+            merged[-1] = (bound, L, P)
+        else:
+            cover = min((v for v in variants if v[0] * v[1] >= bound), key=lambda v: v[0] * v[1])
+            merged.append((bound, cover[0], cover[1]))
     nt_max = [m[0] for m in merged]
     Ls = [m[1] for m in merged]
     Ps = [m[2] for m in merged]
