@@ -302,7 +302,7 @@ struct Body {
     float px, py, pz, qw, qx, qy, qz, vx, vy, vz, wx, wy, wz, t;
     float mass, ks, I0, I1, I2, I3, I4, I5, v0, v1, v2, v4, v5, v8;  // ks = invmass * dt
     float cf, bx, by, bz;
-    int step, end, fid, nt;
+    int step, end, fid, nt, s0;  // s0: step at which this chunk's flying started
 };
 
 // wind terms (common.h:462-465): term k = weight * sin(a*p_axis + b*t + c*phase)
@@ -348,6 +348,7 @@ struct Flyer {
         B.vx = st.vel[0]; B.vy = st.vel[1]; B.vz = st.vel[2];
         B.wx = st.wb[0]; B.wy = st.wb[1]; B.wz = st.wb[2];
         B.step = st.step;
+        B.s0 = st.step;
         B.t = B.step == 0 ? 0.0f : (float)B.step * PR_DT;  // chunk clock rule (sim.cu:113)
         B.end = min(B.step + a.chunk, a.max_steps);
         B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
@@ -386,16 +387,18 @@ struct Flyer {
     // v_r = v_b + w x r, s = v_r.n/|v_r|, q = k |v_r|^2 (k = rho A / 2), and
     // f = -q cn n - q cf (v_r/|v_r| - s n) = c1 n + c2 v_r with cn = 2 s h,
     // h = 0.95 sqrt(1 - s^2) + |s|, c1 = q s (cf - 2h), c2 = -q cf / |v_r|.
-    // The reference skips panels with |v_r|^2 < 1e-6 (its NaN guard); here rsq(V2 + 1e-30)
-    // keeps V2 = 0 finite with zero force, and 0 < V2 < 1e-6 contributes < 1e-9 N.
+    // The reference skips panels with |v_r|^2 < 1e-6 (its NaN guard); here rsq(|v_r|^2 + 1e-30)
+    // keeps v_r = 0 finite with zero force, and 0 < |v_r|^2 < 1e-6 contributes < 1e-9 N.
     __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
                                                 float cf, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
                                                 float& Tz) const {
         const float vrx = __builtin_fmaf(wy, rz[p], __builtin_fmaf(-wz, ry[p], vbx));
         const float vry = __builtin_fmaf(wz, rx[p], __builtin_fmaf(-wx, rz[p], vby));
         const float vrz = __builtin_fmaf(wx, ry[p], __builtin_fmaf(-wy, rx[p], vbz));
-        const float V2 = vrx * vrx + vry * vry + vrz * vrz;
-        const float inv = __builtin_amdgcn_rsqf(V2 + 1e-30f);
+        // |v_r|^2 + 1e-30 with the guard folded into the first FMA (bit-identical to adding
+        // it afterwards whenever |v_r|^2 > 1e-22)
+        const float V2 = __builtin_fmaf(vrz, vrz, __builtin_fmaf(vry, vry, __builtin_fmaf(vrx, vrx, 1e-30f)));
+        const float inv = __builtin_amdgcn_rsqf(V2);
         const float sn = (vrx * nx[p] + vry * ny[p] + vrz * nz[p]) * inv;
         const float qd = kk[p] * V2;
         const float sq = __builtin_amdgcn_sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
@@ -438,9 +441,15 @@ struct Flyer {
         const float vbz = r02 * ux + r12 * uy + r22 * uz;
         const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
         float Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
+        // slots in pairs: two independent panel evaluations per branch give the scheduler
+        // instruction-level parallelism to interleave (at most one padded slot evaluated)
 #pragma unroll
-        for (int p = 0; p < P; p++)
-            if (p < npw) panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+        for (int p = 0; p < P; p += 2) {
+            if (p < npw) {
+                panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+                if (p + 1 < P) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+            }
+        }
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
         Tx = group_sum<L>(Tx); Ty = group_sum<L>(Ty); Tz = group_sum<L>(Tz);
         // rigid-body update (sim.cu:177-197); torque is already in body axes
@@ -507,7 +516,6 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
     const int gbase = lane & ~(L - 1);
     Flyer<L, P> F;
     bool active = false, exhausted = false;
-    unsigned long long fsteps = 0, psteps = 0;
     F.B.nt = 0;
     F.np = 0;
     int npw = 0;
@@ -527,10 +535,17 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
         if (!__ballot(active)) break;
         if (!active) continue;
         const bool fin = F.step(a.max_steps, npw);
-        fsteps += 1;
-        psteps += (unsigned long long)F.B.nt;
         if (fin || F.B.step >= F.B.end) {
             if (j == 0) {
+                // algorithmic work counters: integrated steps and panel-steps of this chunk
+                const unsigned long long fs = (unsigned long long)(F.B.step - F.B.s0);
+                const unsigned long long ps = fs * (unsigned long long)F.B.nt;
+                atomicAdd(a.stats + 0, fs);
+                atomicAdd(a.stats + 1, ps);
+                if (a.cstats) {
+                    atomicAdd(a.cstats + 0, fs);
+                    atomicAdd(a.cstats + 1, ps);
+                }
                 const FlightState st = F.state(fin);
                 a.state[F.B.fid] = st;
                 if (fin) {
@@ -542,20 +557,6 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
                 }
             }
             active = false;
-        }
-    }
-    // algorithmic work counters: one lane per group counted each step
-    if (j != 0) { fsteps = 0; psteps = 0; }
-    for (int o = 32; o > 0; o >>= 1) {
-        fsteps += __shfl_xor(fsteps, o);
-        psteps += __shfl_xor(psteps, o);
-    }
-    if (lane == 0 && fsteps) {
-        atomicAdd(a.stats + 0, fsteps);
-        atomicAdd(a.stats + 1, psteps);
-        if (a.cstats) {
-            atomicAdd(a.cstats + 0, fsteps);
-            atomicAdd(a.cstats + 1, psteps);
         }
     }
 }
