@@ -107,8 +107,10 @@ def cpu_baseline_res(O, genomes, res, kw, budget_s):
 def pool_ab(PA, args):
     """C4: GA generations/hour with surrogate pre-screening vs the plain batched sim.
     plain: every generation flies the whole pool of mutants; surrogate: the network scores
-    the pool and only the elite + best-predicted candidates are flown (n_fly); both keep
-    the best simulated genomes as the next elite and report the holdout champion."""
+    the pool and only the elite + best-predicted candidates (plus an exploration sample)
+    are flown (n_fly); both keep the best simulated genomes as the next elite and report
+    the holdout champion. The surrogate arm's first generation is a plain one whose flights
+    bootstrap the network (its time counts). Champion fitness = best holdout score so far."""
     n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
     params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
     g0, _, _ = PA.family_population(n_elite, seed=0)
@@ -118,34 +120,97 @@ def pool_ab(PA, args):
                        sigma_frac=0.05, seed=7)
         ga.set_elite(g0)
         S = PA.Surrogate(max_batch=4096, seed=3) if mode == "surrogate" else None
-        gens, times, t_total = [], [], 0.0
+        gens = []
         if S is not None:
-            # bootstrap: one plain generation provides the first training pairs
+            ga.set_explore(args.explore)
             ga.set_collect(True)
             r = ga.generation(0, params, None)
             ga.set_collect(False)
             t0 = time.perf_counter()
-            ga.train(S, epochs=4, lr=2e-3)
+            r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=2e-3)
             r["wall_ms"] += 1000 * (time.perf_counter() - t0)
             gens.append(r)
+            log(f"[{mode}] gen 0 (bootstrap): best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
+                f"{r['holdout']:.3f} {r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
         for gen in range(len(gens), args.gens):
-            r = ga.generation(gen, params, S, train_epochs=1, lr=1e-3)
+            r = ga.generation(gen, params, S, train_epochs=args.train_epochs, lr=1e-3)
             gens.append(r)
             log(f"[{mode}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout {r['holdout']:.3f} "
-                f"{r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) flights {r['flights']:.0f}")
+                f"(best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) "
+                f"flights {r['flights']:.0f} rank-corr {r['rank_corr']:.3f}")
         wall = sum(r["wall_ms"] for r in gens) / 1000.0
+        rc = [r["rank_corr"] for r in gens if r["rank_corr"] == r["rank_corr"]]
         out[mode] = {"gens": len(gens), "wall_s": round(wall, 3), "gens_per_hour": round(3600.0 * len(gens) / wall, 1),
-                     "final_holdout": round(gens[-1]["holdout"], 4), "final_elite_mean": round(gens[-1]["elite_mean"], 4),
-                     "flights_per_gen": gens[-1]["flights"]}
+                     "champion_holdout": round(gens[-1]["best_holdout"], 4),
+                     "final_elite_mean": round(gens[-1]["elite_mean"], 4), "flights_per_gen": gens[-1]["flights"],
+                     "mean_rank_corr": round(float(np.mean(rc)), 3) if rc else None}
     ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
+    champ = out["surrogate"]["champion_holdout"] / out["plain"]["champion_holdout"]
     line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",
             "n_gpus": 1, "steps": args.gens, "warmup": 0, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 sim, bf16 surrogate", "data": "synthetic",
             "config": {"workload": f"C4: pool {n_pool} Gaussian mutants (sigma 5% of range) of a {n_elite} elite, "
-                                   f"surrogate flies the top {n_fly}; 12 environments, gusts <= 4 m/s RMS, 12 s cap",
+                                   f"surrogate flies the top {n_fly} ({args.explore:.0%} exploration); 12 environments, "
+                                   f"gusts <= 4 m/s RMS, 12 s cap",
                        "parallelism": "single GPU"},
-            "plain": out["plain"], "surrogate": out["surrogate"], "speedup_gens_per_hour": round(ratio, 2)}
+            "plain": out["plain"], "surrogate": out["surrogate"], "speedup_gens_per_hour": round(ratio, 2),
+            "champion_ratio": round(champ, 4)}
     print(json.dumps(line), flush=True)
+
+
+def roofline_of(stats_list):
+    """Dominant-kernel roofline from pr_stats dicts: the fused flight kernel's algorithmic
+    FLOPs (68 per panel-step + 222 per plane-step) over its HIP-event-timed launches."""
+    fl = {"flight_steps": 0, "panel_steps": 0}
+    ms, nl = 0.0, 0
+    for st in stats_list:
+        ms += st["fused_ms"]
+        nl += st["fused_launches"]
+        for b in st["buckets"]:
+            if b["fused"]:
+                fl["flight_steps"] += b["flight_steps"]
+                fl["panel_steps"] += b["panel_steps"]
+    flops = FLOP_PANEL * fl["panel_steps"] + FLOP_PLANE * fl["flight_steps"]
+    achieved = flops / max(ms * 1e-3, 1e-12) / 1e12
+    return {"bound": "valu", "kernel": "k_fly_all", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "avg_launch_ms": round(ms / max(1, nl), 4), "launches": nl, "flops_per_launch": round(flops / max(1, nl))}
+
+
+def audit_bench(PA, args):
+    """C5: robustness audit of a champion + 1 024 designs on 4 096 scenarios each (seed 9001 +
+    s*7919 by scenario), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap; one step = one batched
+    audit (prepare, scenarios, every flight, per-design statistics on the host)."""
+    n_gen, n_scen = 1025, 4096
+    genomes, res, _ = PA.family_population(n_gen, seed=9)
+    params = PA.SimParams.config(gust_rms_max=5.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=12000)
+    pl = PA.Planes(n_gen, n_scen, n_gen * n_scen, max_ch=int(res[:, 0].max()), max_sp=int(res[:, 1].max()))
+    pl.upload_genomes(genomes, res)
+    for _ in range(args.warmup):
+        pl.audit(None, n_scen, 9001, params, upload=False)
+    PA.LIB.pr_device_sync()
+    fsteps, stats = 0, []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, _ = pl.audit(None, n_scen, 9001, params, upload=False)
+        s = pl.stats()
+        stats.append(s)
+        fsteps += s["flight_steps"]
+    wall = time.perf_counter() - t0
+    mean_col = PA.AUDIT_FIELDS.index("mean")
+    best = int(np.argmax(st[:, mean_col]))
+    log(f"{args.steps} audits of {n_gen} x {n_scen} in {wall:.3f} s; best mean {st[best, mean_col]:.2f} m "
+        f"(design {best}), failures {int(st[best, PA.AUDIT_FIELDS.index('failures')])}")
+    value = fsteps / 600.0 / wall
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * wall / args.steps, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "C5: robustness audit, 1025 designs (family mix) x 4096 scenarios (seed 9001 + "
+                                  "s*7919), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap",
+                      "designs": n_gen, "scenarios": n_scen, "max_steps": 12000, "parallelism": "single GPU"},
+           "audits_per_hour": round(3600.0 * args.steps / wall, 1), "roofline": roofline_of(stats),
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
 
 
 def copy_probe(mib):
@@ -170,12 +235,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
     ap.add_argument("--pool", type=int, default=1_000_000, help="c4: candidate pool per generation")
     ap.add_argument("--fly", type=int, default=100_000, help="c4: candidates flown after pre-screening")
     ap.add_argument("--elite", type=int, default=10_000, help="c4: elite size")
-    ap.add_argument("--gens", type=int, default=4, help="c4: generations per mode")
+    ap.add_argument("--gens", type=int, default=6, help="c4: generations per mode")
     ap.add_argument("--buffer", type=int, default=500_000, help="c4: surrogate replay buffer")
+    ap.add_argument("--explore", type=float, default=0.1, help="c4: fraction of the fly set drawn at random")
+    ap.add_argument("--boot-epochs", type=int, default=8, help="c4: surrogate epochs on the bootstrap generation")
+    ap.add_argument("--train-epochs", type=int, default=1, help="c4: surrogate epochs per generation")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -193,6 +261,9 @@ def main():
     if args.workload == "c4":
         log("device:", PA.device_name())
         return pool_ab(PA, args)
+    if args.workload == "c5":
+        log("device:", PA.device_name())
+        return audit_bench(PA, args)
     w = WORKLOADS[args.workload]
     log("device:", PA.device_name())
     res = None

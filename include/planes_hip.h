@@ -159,6 +159,21 @@ int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop
                     int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
                     float* holdout, int* champion);
 int pr_gen_stats(pr_gen* e, pr_stats* main_stats, pr_stats* holdout_stats);
+/* Robustness statistics of one design's n scenario distances (evaluate_design's summary,
+ * sim.cu:291-309): out[PR_NSTAT] indexed by PR_ST_*. Host function. */
+enum {
+    PR_ST_MIN, PR_ST_P05, PR_ST_P10, PR_ST_P25, PR_ST_MEDIAN, PR_ST_MEAN, PR_ST_SD, PR_ST_P75, PR_ST_P90,
+    PR_ST_P95, PR_ST_MAX, PR_ST_CVAR10, PR_ST_FAILURES, PR_NSTAT
+};
+int pr_design_stats(const float* samples, int n, float* out);
+
+/* Batched robustness audit -- evaluate_design (sim.cu:258-312) for n_genomes designs in one
+ * flight batch: each flies the same n_scen scenarios (seed base_seed + s * 7919, by scenario
+ * index).  d_genomes / d_res as pr_prepare; d_fitness, d_steps: device [n_genomes * n_scen];
+ * stats: host [n_genomes * PR_NSTAT]; nthreads host threads for the statistics (0 = all). */
+int pr_audit(pr_ctx* ctx, const float* d_genomes, const int* d_res, int n_genomes, int n_scen, int base_seed,
+             const pr_sim_params* p, float* d_fitness, int* d_steps, float* stats, int nthreads);
+
 /* The evaluator's flight contexts (0: population flights, 1: holdout re-test), so that the
  * pr_set_* options apply to it; owned by the evaluator. */
 pr_ctx* pr_gen_context(pr_gen* e, int which);
@@ -190,8 +205,9 @@ int pr_mlp_train(pr_mlp* m, const float* d_genomes, const float* d_targets, int 
  * best surrogate-predicted score (n_fly in total); new elite = best simulated scores;
  * top-8 holdout champion; with a model, the flown pairs join a replay buffer of buf_cap
  * and the network trains on it.  The simulation alone ranks the elite and the champion.
- * pr_pool_generation out[10]: best score, elite mean, holdout champion score, wall ms,
- * flight ms, surrogate ms, flights, plane-flight-s, training loss, fly-set size. */
+ * pr_pool_generation out[12]: best score, elite mean, holdout champion score, wall ms,
+ * flight ms, surrogate ms, flights, plane-flight-s, training loss, fly-set size, best
+ * holdout score so far, Spearman rho of predicted vs simulated scores on the fly set. */
 typedef struct pr_pool pr_pool;
 int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds, int buf_cap, float sigma_frac,
                    unsigned seed);
@@ -205,6 +221,8 @@ int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* sample
 /* 1: plain generations also add their flown (genome, targets) pairs to the replay buffer
  * (bootstrapping the surrogate); 0 (default): only surrogate generations do. */
 int pr_pool_set_collect(pr_pool* p, int on);
+/* Fraction of the surrogate fly set (beyond the elite) drawn at random from the pool. */
+int pr_pool_set_explore(pr_pool* p, float frac);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 

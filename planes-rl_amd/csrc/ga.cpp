@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/planes_hip.h"
@@ -52,6 +54,7 @@ float robust_row(const float* r, int n, std::vector<float>& row) {
 }
 
 float percentile(const std::vector<float>& sorted, float q) {  // sim.cu:250-256
+#pragma clang fp contract(off)
     if (sorted.empty()) return 0;
     float x = q * (sorted.size() - 1);
     int a = (int)floorf(x), b = (int)ceilf(x);
@@ -369,6 +372,75 @@ void run_ga(int pop, int gens, int nseeds, const char* out_dir, const char* init
 }
 
 // one genome under `scenarios` seeds base + i*7919 (layout 0, one genome)
+// Robustness statistics of one design's scenario distances (sim.cu:291-309).  The moments
+// are accumulated over the samples in ascending order with a division per term, as there,
+// because that order fixes the float result; order statistics interpolate linearly
+// (sim.cu:250-256); CVaR10 averages the worst max(1, n/10); failures count x < 0.
+int pr_design_stats(const float* samples, int n, float* out) {
+#pragma clang fp contract(off)
+    if (!samples || !out || n < 1) return -1;
+    std::vector<float> v(samples, samples + n);
+    std::sort(v.begin(), v.end());
+    const float fn = (float)n;
+    float m1 = 0, m2 = 0;
+    for (int i = 0; i < n; i++) {
+        m1 += v[i] / fn;
+        m2 += v[i] * v[i] / fn;
+    }
+    const int worst = std::max(1, n / 10);
+    float cv = 0;
+    for (int i = 0; i < worst; i++) cv += v[i] / (float)worst;
+    int neg = 0;
+    while (neg < n && v[neg] < 0.0f) neg++;  // sorted: negatives come first
+    out[PR_ST_MIN] = v.front();
+    out[PR_ST_P05] = percentile(v, .05f);
+    out[PR_ST_P10] = percentile(v, .10f);
+    out[PR_ST_P25] = percentile(v, .25f);
+    out[PR_ST_MEDIAN] = percentile(v, .50f);
+    out[PR_ST_MEAN] = m1;
+    out[PR_ST_SD] = sqrtf(fmaxf(0.0f, m2 - m1 * m1));
+    out[PR_ST_P75] = percentile(v, .75f);
+    out[PR_ST_P90] = percentile(v, .90f);
+    out[PR_ST_P95] = percentile(v, .95f);
+    out[PR_ST_MAX] = v.back();
+    out[PR_ST_CVAR10] = cv;
+    out[PR_ST_FAILURES] = (float)neg;
+    return 0;
+}
+
+// Batched robustness audit: evaluate_design (sim.cu:258-312) for many designs at once.
+// Every genome flies the same n_scen scenarios, seed base_seed + s * 7919 indexed by the
+// scenario (32-bit wrap, as the reference's seed arithmetic), in one pr_fly; the distances
+// land in d_fitness [n_genomes x n_scen] (device) and the per-genome statistics in
+// stats [n_genomes x PR_NSTAT] (host), computed on nthreads host threads (0: all cores).
+int pr_audit(pr_ctx* c, const float* d_genomes, const int* d_res, int n_genomes, int n_scen, int base_seed,
+             const pr_sim_params* p, float* d_fitness, int* d_steps, float* stats, int nthreads) {
+    if (!c || !d_genomes || !p || !d_fitness || !d_steps || !stats || n_genomes < 1 || n_scen < 1) return -1;
+    std::vector<int> seeds(n_scen);
+    for (int s = 0; s < n_scen; s++) seeds[s] = (int)((unsigned)base_seed + (unsigned)s * 7919u);
+    DevBuf ds;
+    ds.ensure(sizeof(int) * n_scen);
+    int rc = pr_h2d(ds.p, seeds.data(), sizeof(int) * n_scen);
+    if (!rc) rc = pr_prepare(c, d_genomes, d_res, n_genomes);
+    if (!rc) rc = pr_scenarios(c, (const int*)ds.p, n_scen, p);
+    if (!rc) rc = pr_fly(c, 0, n_genomes, n_scen, p, d_fitness, d_steps);
+    if (rc) return rc;
+    std::vector<float> dist((size_t)n_genomes * n_scen);
+    rc = pr_d2h(dist.data(), d_fitness, sizeof(float) * dist.size());
+    if (rc) return rc;
+    int nth = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nth = std::min(nth, n_genomes);
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; t++)
+        pool.emplace_back([&]() {
+            for (int g; (g = next.fetch_add(1)) < n_genomes;)
+                pr_design_stats(dist.data() + (size_t)g * n_scen, n_scen, stats + (size_t)g * PR_NSTAT);
+        });
+    for (auto& th : pool) th.join();
+    return 0;
+}
+
 static void fly_one_genome(pr_ctx* c, DevBuf& dg, DevBuf& ds, DevBuf& df, DevBuf& dst, const Genome* g, int n,
                            int base_seed, std::vector<float>& out) {
     std::vector<int> seeds(n);
@@ -416,26 +488,19 @@ void evaluate_design(const Genome* g, int scenarios, int base_seed, const char* 
         }
         fclose(f);
     }
-    std::vector<float> sorted = distances;
-    std::sort(sorted.begin(), sorted.end());
-    float mean = 0, sq = 0;
-    for (float x : sorted) { mean += x / scenarios; sq += x * x / scenarios; }
-    float sd = sqrtf(fmaxf(0.0f, sq - mean * mean));
-    int tail_n = std::max(1, scenarios / 10), failures = 0;
-    float cvar10 = 0;
-    for (int i = 0; i < tail_n; i++) cvar10 += sorted[i] / tail_n;
-    for (float x : sorted) if (x < 0.0f) failures++;
+    float st[PR_NSTAT];
+    pr_design_stats(distances.data(), scenarios, st);
     float mass20[20], ext[3];
     int nt, nv, off[2];
     CKP(pr_read_prepared(c, 1, mass20, &nt, &nv, off));
     CKP(pr_read_extent(c, 1, ext));
     printf("robustness audit: %d GPU scenarios -> %s\n", scenarios, csv_path);
-    printf("  min %.2fm  p05 %.2fm  p10 %.2fm  p25 %.2fm  median %.2fm\n", sorted.front(), percentile(sorted, .05f),
-           percentile(sorted, .10f), percentile(sorted, .25f), percentile(sorted, .50f));
-    printf("  mean %.2fm  sd %.2fm  p75 %.2fm  p90 %.2fm  p95 %.2fm  max %.2fm\n", mean, sd, percentile(sorted, .75f),
-           percentile(sorted, .90f), percentile(sorted, .95f), sorted.back());
-    printf("  CVaR10 %.2fm  failures %d/%d  PLA %.1fg  bounds %.0fx%.0fx%.0fmm\n", cvar10, failures, scenarios,
-           mass20[0] * 1000, ext[0] * 1000, ext[1] * 1000, ext[2] * 1000);
+    printf("  min %.2fm  p05 %.2fm  p10 %.2fm  p25 %.2fm  median %.2fm\n", st[PR_ST_MIN], st[PR_ST_P05], st[PR_ST_P10],
+           st[PR_ST_P25], st[PR_ST_MEDIAN]);
+    printf("  mean %.2fm  sd %.2fm  p75 %.2fm  p90 %.2fm  p95 %.2fm  max %.2fm\n", st[PR_ST_MEAN], st[PR_ST_SD],
+           st[PR_ST_P75], st[PR_ST_P90], st[PR_ST_P95], st[PR_ST_MAX]);
+    printf("  CVaR10 %.2fm  failures %d/%d  PLA %.1fg  bounds %.0fx%.0fx%.0fmm\n", st[PR_ST_CVAR10],
+           (int)st[PR_ST_FAILURES], scenarios, mass20[0] * 1000, ext[0] * 1000, ext[1] * 1000, ext[2] * 1000);
     pr_destroy(c);
 }
 

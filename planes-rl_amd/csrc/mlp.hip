@@ -123,12 +123,34 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_n);
 }
 
-// bias gradient = column sums of a bf16 [rows x cols] gradient
-__global__ void k_colsum(const __bf16* __restrict__ G, int rows, int cols, float* __restrict__ out) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+// bias gradient = column sums of a bf16 [rows x cols] gradient, in two deterministic passes:
+// block (x, y) sums columns 64x..64x+63 over row slice y (4 waves on interleaved rows, then
+// an LDS sum in wave order) into part[y][cols]; k_colsum_fin adds the COLSUM_SLICES partials
+// of each column in slice order.
+constexpr int COLSUM_SLICES = 32;
+__global__ __launch_bounds__(256) void k_colsum_part(const __bf16* __restrict__ G, int rows, int cols,
+                                                     float* __restrict__ part) {
+    __shared__ float red[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int w = threadIdx.x >> 6;
+    const int per = (rows + COLSUM_SLICES - 1) / COLSUM_SLICES;
+    const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+    float s = 0.0f;
+    if (c < cols)
+        for (int r = r0 + w; r < r1; r += 4) s += from_bf(G[(size_t)r * cols + c]);
+    red[w][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (w == 0 && c < cols) {
+        const int l = threadIdx.x;
+        part[(size_t)blockIdx.y * cols + c] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+    }
+}
+
+__global__ void k_colsum_fin(const float* __restrict__ part, int cols, float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
     float s = 0.0f;
-    for (int r = 0; r < rows; r++) s += from_bf(G[(size_t)r * cols + c]);
+    for (int y = 0; y < COLSUM_SLICES; y++) s += part[(size_t)y * cols + c];
     out[c] = s;
 }
 
@@ -199,6 +221,7 @@ struct pr_mlp {
     float* loss = nullptr;
     float* lo = nullptr;
     float* inv_range = nullptr;
+    float* colsum_part = nullptr;  // [COLSUM_SLICES x 256] bias-gradient partial sums
     int step = 0;
     hipStream_t s = nullptr;
 };
@@ -269,7 +292,11 @@ static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, 
                            nullptr, nullptr, 0, (void*)(q->g + q->off_w[L]), N);
         MCK(hipGetLastError());
         // db = column sums of dY
-        hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64), dim3(64), 0, q->s, dY, B, N, q->g + q->off_b[L]);
+        hipLaunchKernelGGL(k_colsum_part, dim3((N + 63) / 64, COLSUM_SLICES), dim3(256), 0, q->s, dY, B, N,
+                           q->colsum_part);
+        MCK(hipGetLastError());
+        hipLaunchKernelGGL(k_colsum_fin, dim3((N + 63) / 64), dim3(64), 0, q->s, q->colsum_part, N,
+                           q->g + q->off_b[L]);
         MCK(hipGetLastError());
         if (L > 0) {
             // dXin [B x K] = dY [B x N] * W^T, masked by ReLU'(Xin) (Xin > 0)
@@ -332,6 +359,7 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     MALLOC(q->loss, 4);
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
+    MALLOC(q->colsum_part, 4 * (size_t)COLSUM_SLICES * HID);
 #undef MALLOC
     float lo[NIN], hi[NIN], ir[NIN];
     pr_gene_bounds(lo, hi);
@@ -362,7 +390,7 @@ void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
     void* bufs[] = {q->w, q->g, q->m, q->v, q->wb, q->X, q->out, q->dout, q->dout_f, q->loss, q->lo, q->inv_range,
-                    q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2]};
+                    q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2], q->colsum_part};
     for (void* b : bufs) if (b) (void)hipFree(b);
     if (q->s) (void)hipStreamDestroy(q->s);
     delete q;

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -70,13 +71,46 @@ __global__ void k_gather(const float* __restrict__ src, const int* __restrict__ 
     dst[t] = src[(size_t)idx[i] * pr::NG + j];
 }
 
-// keys for a descending sort: -score, with the elite (i < n_keep) forced to the front
+// keys for a descending sort: -score, with the elite (i < n_keep) and an exploration sample
+// (each other candidate with probability p_explore, hashed from (seed, gen, i)) forced to
+// the front
 __global__ void k_sort_keys(const float* __restrict__ pred, int stride, int col, int n, int n_keep,
-                            float* __restrict__ keys, int* __restrict__ vals) {
+                            float p_explore, uint32_t seed, uint32_t gen, float* __restrict__ keys,
+                            int* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = i < n_keep ? -3.0e38f : -pred[(size_t)i * stride + col];
+    const bool explore = i >= n_keep && unit24(hash3(seed ^ 0x2545f491u, gen, (uint32_t)i)) < p_explore;
+    keys[i] = i < n_keep ? -3.0e38f : explore ? -2.0e38f : -pred[(size_t)i * stride + col];
     vals[i] = i;
+}
+
+__global__ void k_gather_col(const float* __restrict__ pred, int stride, int col, const int* __restrict__ idx, int n,
+                             float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pred[(size_t)idx[i] * stride + col];
+}
+
+// Spearman rank correlation (average ranks are not needed: ties have measure zero here)
+static double spearman(const std::vector<float>& a, const std::vector<float>& b) {
+    const size_t n = a.size();
+    if (n < 3) return 0.0;
+    auto ranks = [n](const std::vector<float>& x) {
+        std::vector<int> o(n);
+        for (size_t i = 0; i < n; i++) o[i] = (int)i;
+        std::stable_sort(o.begin(), o.end(), [&](int p, int q) { return x[p] < x[q]; });
+        std::vector<double> r(n);
+        for (size_t i = 0; i < n; i++) r[o[i]] = (double)i;
+        return r;
+    };
+    const std::vector<double> ra = ranks(a), rb = ranks(b);
+    const double m = (n - 1) / 2.0;
+    double sab = 0, saa = 0, sbb = 0;
+    for (size_t i = 0; i < n; i++) {
+        sab += (ra[i] - m) * (rb[i] - m);
+        saa += (ra[i] - m) * (ra[i] - m);
+        sbb += (rb[i] - m) * (rb[i] - m);
+    }
+    return sab / std::sqrt(saa * sbb);
 }
 
 // per-genome fitness statistics of the flown samples: mean (seed order), CVaR of the worst
@@ -119,6 +153,8 @@ struct pr_pool {
     size_t sort_tmp_bytes = 0;
     int buf_n = 0, buf_head = 0;
     int collect = 0;  // plain generations also feed the replay buffer
+    float explore = 0.0f;  // fraction of the surrogate fly set drawn at random from the pool
+    float best_holdout = -1e9f;
     std::vector<float> acc, samples, champion;
     std::vector<int> idx;
     float champion_score = -1e9f;
@@ -218,9 +254,10 @@ static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
 
 // One generation.  model = null: plain (fly the whole pool); else pre-screen with the
 // surrogate and train it on the flown pairs for train_epochs epochs over the buffer.
-// out[0..9]: best simulated score, elite mean score, holdout score of the champion,
+// out[0..11]: best simulated score, elite mean score, holdout score of the champion,
 // wall ms, flight ms, surrogate ms (predict + train), flights flown, plane-flight-s,
-// last training loss, fly-set size.
+// last training loss, fly-set size, best holdout score so far (the champion record),
+// Spearman correlation of predicted and simulated scores over the fly set (NaN when plain).
 int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
                        double* out) {
     if (!p || !prm || !out) { g_pool_err = "pr_pool_generation: null argument"; return -1; }
@@ -235,8 +272,10 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         PCK(hipDeviceSynchronize());
         auto m0 = std::chrono::steady_clock::now();
         PRC(pr_mlp_predict(model, p->d_pool, p->n_pool, p->d_pred));
+        const float p_explore =
+            p->n_pool > p->n_elite ? p->explore * (float)(p->n_fly - p->n_elite) / (float)(p->n_pool - p->n_elite) : 0.f;
         hipLaunchKernelGGL(k_sort_keys, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->n_pool,
-                           p->n_elite, p->d_keys, p->d_vals);
+                           p->n_elite, p_explore, p->seed, (uint32_t)gen, p->d_keys, p->d_vals);
         PCK(hipGetLastError());
         size_t tb = p->sort_tmp_bytes;
         PCK(hipcub::DeviceRadixSort::SortPairs(p->d_sort_tmp, tb, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
@@ -269,6 +308,17 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     PCK(hipGetLastError());
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
     p->champion_score = holdout;
+    p->best_holdout = std::max(p->best_holdout, holdout);
+    // screening quality: rank correlation of predicted and simulated scores over the fly set
+    double rho = std::nan("");
+    if (model) {
+        hipLaunchKernelGGL(k_gather_col, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->d_vals2, nf,
+                           p->d_keys);
+        PCK(hipGetLastError());
+        std::vector<float> pf(nf);
+        PCK(hipMemcpy(pf.data(), p->d_keys, sizeof(float) * nf, hipMemcpyDeviceToHost));
+        rho = spearman(pf, p->acc);
+    }
     float loss = 0;
     if (collect) {
         auto m0 = std::chrono::steady_clock::now();
@@ -293,6 +343,8 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     out[7] = (double)(st_main.flight_steps + st_hold.flight_steps) / 600.0;
     out[8] = loss;
     out[9] = nf;
+    out[10] = p->best_holdout;
+    out[11] = rho;
     return 0;
 }
 
@@ -304,6 +356,14 @@ int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* sample
     hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, 0, p->d_samples, n, p->nseeds, p->d_tgt);
     PCK(hipGetLastError());
     return buffer_add(p, d_genomes, p->d_tgt, n);
+}
+
+// Fraction (0..1) of the surrogate fly set beyond the elite drawn uniformly from the pool
+// instead of by predicted score: exploration, and unbiased training pairs for the network.
+int pr_pool_set_explore(pr_pool* p, float frac) {
+    if (!p || !(frac >= 0.0f && frac <= 1.0f)) { g_pool_err = "pr_pool_set_explore: fraction in [0, 1]"; return -1; }
+    p->explore = frac;
+    return 0;
 }
 
 // 1: plain generations (model = null) also add their flown pairs to the replay buffer,

@@ -61,6 +61,18 @@ class Stats(C.Structure):
         return d
 
 
+# pr_design_stats / pr_audit columns (evaluate_design's summary, sim.cu:291-309)
+AUDIT_FIELDS = ("min", "p05", "p10", "p25", "median", "mean", "sd", "p75", "p90", "p95", "max", "cvar10", "failures")
+
+
+def design_stats(samples):
+    """evaluate_design's robustness statistics of one design's distances (host C++)."""
+    x = np.ascontiguousarray(samples, np.float32)
+    out = np.zeros(len(AUDIT_FIELDS), np.float32)
+    _ck(LIB.pr_design_stats(x, len(x), out), "pr_design_stats")
+    return dict(zip(AUDIT_FIELDS, out.tolist()))
+
+
 class PlanesError(RuntimeError):
     pass
 
@@ -95,6 +107,8 @@ def _load():
     lib.pr_set_graph.argtypes = [VP, C.c_int]
     lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
     lib.pr_set_fused.argtypes = [VP, C.c_int]
+    lib.pr_design_stats.argtypes = [F32P, C.c_int, F32P]
+    lib.pr_audit.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, F32P, C.c_int]
     lib.pr_gen_context.argtypes = [VP, C.c_int]
     lib.pr_gen_context.restype = VP
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
@@ -109,6 +123,7 @@ def _load():
                                        np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
     lib.pr_pool_train.argtypes = [VP, VP, C.c_int, C.c_float, C.POINTER(C.c_float)]
     lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
+    lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_mlp_param_count.restype = C.c_int
     lib.pr_mlp_create.argtypes = [C.POINTER(VP), C.c_int, C.c_uint]
@@ -301,6 +316,19 @@ class Planes:
         if layout == 0:
             return fit.reshape(self.n_genomes, self.n_scen), steps.reshape(self.n_genomes, self.n_scen)
         return fit, steps
+
+    def audit(self, genomes, n_scen, base_seed, params, res=None, nthreads=0, upload=True):
+        """Batched robustness audit (evaluate_design for every genome at once): all genomes
+        fly the same n_scen scenarios, seed base_seed + s*7919.  Returns (stats[n, 13] in
+        AUDIT_FIELDS order, distances[n, n_scen]); distances stay in d_fit too."""
+        if upload:
+            self.upload_genomes(genomes, res)
+        n = self.n_genomes
+        st = np.zeros((n, len(AUDIT_FIELDS)), np.float32)
+        _ck(LIB.pr_audit(self.h, self.d_genomes.ptr, self.d_res.ptr if self._has_res else None, n, n_scen,
+                         base_seed, C.byref(params), self.d_fit.ptr, self.d_steps.ptr, st, nthreads), "pr_audit")
+        self.n_scen = n_scen
+        return st, self.d_fit.download(np.float32, n * n_scen).reshape(n, n_scen)
 
     def scores(self, samples, nseeds):
         s = np.ascontiguousarray(samples, np.float32).reshape(-1)
@@ -691,7 +719,8 @@ class PoolGA:
     """Candidate-pool GA on the GPU (pr_pool_*): plain (fly the whole pool) or surrogate
     pre-screened (fly the elite + the best-predicted candidates)."""
 
-    FIELDS = ("best", "elite_mean", "holdout", "wall_ms", "fly_ms", "mlp_ms", "flights", "pfs", "loss", "n_fly")
+    FIELDS = ("best", "elite_mean", "holdout", "wall_ms", "fly_ms", "mlp_ms", "flights", "pfs", "loss", "n_fly",
+              "best_holdout", "rank_corr")
 
     def __init__(self, n_elite, n_pool, n_fly, nseeds, buf_cap=2_000_000, sigma_frac=0.05, seed=1):
         self.h = VP()
@@ -714,10 +743,14 @@ class PoolGA:
         return out.reshape(-1, N_GENES)
 
     def generation(self, gen, params, surrogate=None, train_epochs=2, lr=1e-3):
-        out = np.zeros(10, np.float64)
+        out = np.zeros(len(self.FIELDS), np.float64)
         self._ck(LIB.pr_pool_generation(self.h, gen, C.byref(params), surrogate.h if surrogate else None,
                                         train_epochs, lr, out), "pr_pool_generation")
         return dict(zip(self.FIELDS, out.tolist()))
+
+    def set_explore(self, frac):
+        """Fraction of the surrogate fly set drawn at random from the pool."""
+        self._ck(LIB.pr_pool_set_explore(self.h, float(frac)), "pr_pool_set_explore")
 
     def set_collect(self, on=True):
         """Plain generations also feed the surrogate's replay buffer (bootstrap)."""

@@ -359,3 +359,31 @@ def test_autotuner_picks_valid_mapping_and_results_hold(PA, tmp_path):
     rel = np.abs(out[0] - out[1]) / np.maximum(np.abs(out[0]), 1e-3)
     print("autotuned buckets:", list(zip(nt_max, Ls, Ps)), f"agreement {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3")
     assert np.mean(rel < 1e-2) >= 0.97
+
+
+def test_batched_audit_matches_single_design_flights_and_fp64(PA):
+    """pr_audit (evaluate_design for many designs in one flight batch): each genome's
+    distances equal those of flying it alone on the seeds base + s*7919 (bit-identical),
+    the statistics equal pr_design_stats of those distances, and the champion's distances
+    agree with the FP64 oracle within 1 % for >= 95 % of scenarios (C5 preset: gusts <= 5
+    m/s RMS, warp +-1 deg, 20 s cap)."""
+    gs = O.random_genomes(12, 77)
+    n_scen, base = 96, 9001
+    params = PA.SimParams.config(gust_rms_max=5.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=12000)
+    pl = PA.Planes(12, n_scen, 12 * n_scen)
+    st, dist = pl.audit(gs, n_scen, base, params)
+    seeds = (base + 7919 * np.arange(n_scen)).astype(np.int32)
+    one = PA.Planes(1, n_scen, n_scen)
+    for g in (0, 5, 11):
+        fit, _ = one.evaluate(gs[g:g + 1], seeds, params)
+        assert np.array_equal(fit.reshape(-1), dist[g]), g
+    for g in range(12):
+        ref = PA.design_stats(dist[g])
+        assert list(st[g]) == [np.float32(ref[k]) for k in PA.AUDIT_FIELDS], g
+    champ = int(np.argmax(st[:, PA.AUDIT_FIELDS.index("mean")]))
+    kw = dict(preset=1, gust=5.0, warp=float(np.deg2rad(1.0)), wsd=2.0, max_steps=12000)
+    f64, _, _ = O.fly(gs, np.full(n_scen, champ, np.int32), seeds, prec=64, **kw)
+    rel = np.abs(dist[champ] - f64) / np.maximum(np.abs(f64), 0.5)
+    print(f"audit champion {champ}: mean {st[champ, 5]:.2f} m, {np.mean(rel < 0.01) * 100:.1f}% of scenarios "
+          f"within 1% of FP64")
+    assert np.mean(rel < 0.01) >= 0.95

@@ -98,3 +98,56 @@ def test_sanitize_clamps_like_gset():
     g = np.stack([lo - 1, hi + 1, (lo + hi) / 2]).astype(np.float32)
     s = PA.ga_sanitize(g)
     assert np.array_equal(s[0], lo) and np.array_equal(s[1], hi) and np.array_equal(s[2], g[2])
+
+
+def _design_stats_f32(x):
+    """evaluate_design's summary restated in float32 (sim.cu:250-256, 291-299): moments summed
+    over the sorted samples with one division per term, interpolated order statistics."""
+    v = np.sort(np.asarray(x, np.float32))
+    n = len(v)
+    fn = np.float32(n)
+    m1 = np.float32(0)
+    m2 = np.float32(0)
+    for a in v:
+        m1 = np.float32(m1 + a / fn)
+        m2 = np.float32(m2 + np.float32(a * a) / fn)
+
+    def pct(q):
+        x = np.float32(q) * np.float32(n - 1)
+        lo, hi = int(np.floor(x)), int(np.ceil(x))
+        t = np.float32(x - np.float32(lo))
+        return np.float32(v[lo] * (np.float32(1) - t) + v[hi] * t)
+
+    w = max(1, n // 10)
+    cv = np.float32(0)
+    for a in v[:w]:
+        cv = np.float32(cv + a / np.float32(w))
+    sd = np.float32(np.sqrt(np.float32(max(np.float32(0), np.float32(m2 - m1 * m1)))))
+    return [v[0], pct(.05), pct(.10), pct(.25), pct(.5), m1, sd, pct(.75), pct(.9), pct(.95), v[-1], cv,
+            np.float32((v < 0).sum())]
+
+
+@pytest.mark.parametrize("n", [8, 37, 128, 4096])
+def test_design_stats_match_float32_restatement(n):
+    rng = np.random.default_rng(n)
+    x = (rng.normal(8, 4, n) * (rng.random(n) > 0.05) - 2 * (rng.random(n) < 0.03)).astype(np.float32)
+    got = PA.design_stats(x)
+    want = _design_stats_f32(x)
+    for k, w in zip(PA.AUDIT_FIELDS, want):
+        assert np.float32(got[k]) == w, (k, got[k], w)
+
+
+def test_design_stats_closed_forms():
+    """Samples 0..n-1 in any order: min 0, max n-1, quantile q = q (n-1), no failures; with
+    k negatives, failures = k and CVaR10 = the mean of the worst n/10."""
+    n = 1001
+    x = np.random.default_rng(0).permutation(np.arange(n, dtype=np.float32))
+    s = PA.design_stats(x)
+    assert s["min"] == 0 and s["max"] == n - 1 and s["failures"] == 0
+    for q, k in ((.05, "p05"), (.25, "p25"), (.5, "median"), (.9, "p90")):
+        assert s[k] == pytest.approx(q * (n - 1), abs=1e-3)
+    assert s["mean"] == pytest.approx((n - 1) / 2, rel=1e-5)
+    assert s["sd"] == pytest.approx(np.sqrt((n * n - 1) / 12.0), rel=1e-3)
+    assert s["cvar10"] == pytest.approx(np.mean(np.arange(100)), rel=1e-5)
+    x[:7] = -1.0
+    assert PA.design_stats(x)["failures"] == 7
