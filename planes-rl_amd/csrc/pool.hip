@@ -257,7 +257,8 @@ static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
 // out[0..11]: best simulated score, elite mean score, holdout score of the champion,
 // wall ms, flight ms, surrogate ms (predict + train), flights flown, plane-flight-s,
 // last training loss, fly-set size, best holdout score so far (the champion record),
-// Spearman correlation of predicted and simulated scores over the fly set (NaN when plain).
+// Spearman correlation of predicted and simulated scores over the fly set's exploration
+// sample (NaN when plain; 0 when it has fewer than 3 members).
 int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
                        double* out) {
     if (!p || !prm || !out) { g_pool_err = "pr_pool_generation: null argument"; return -1; }
@@ -309,15 +310,21 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
     p->champion_score = holdout;
     p->best_holdout = std::max(p->best_holdout, holdout);
-    // screening quality: rank correlation of predicted and simulated scores over the fly set
+    // screening quality: rank correlation of predicted and simulated scores over the
+    // exploration sample of the fly set (uniform draws from the pool, so unbiased by the
+    // selection; the surrogate-selected part over-represents over-predicted candidates)
     double rho = std::nan("");
     if (model) {
         hipLaunchKernelGGL(k_gather_col, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->d_vals2, nf,
                            p->d_keys);
         PCK(hipGetLastError());
-        std::vector<float> pf(nf);
+        std::vector<float> pf(nf), key(nf);
         PCK(hipMemcpy(pf.data(), p->d_keys, sizeof(float) * nf, hipMemcpyDeviceToHost));
-        rho = spearman(pf, p->acc);
+        PCK(hipMemcpy(key.data(), p->d_keys2, sizeof(float) * nf, hipMemcpyDeviceToHost));
+        std::vector<float> a, b;
+        for (int i = 0; i < nf; i++)
+            if (key[i] == -2.0e38f) { a.push_back(pf[i]); b.push_back(p->acc[i]); }
+        rho = spearman(a, b);
     }
     float loss = 0;
     if (collect) {
