@@ -25,6 +25,9 @@ sys.path.insert(0, os.path.join(ROOT, "planes-rl_amd"))
 METRIC = "plane-flight-sec/s at 600 Hz; GA gens/hour at 100k×12 env, plain vs surrogate"
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 FLOP_PANEL, FLOP_PLANE = 68, 222  # algorithmic FLOPs per panel-step and per plane-step
+# HBM bytes per k_fly_all launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled
+# by a 1 GiB copy's readings) on this workload; the counters cannot be read inside a timed run
+TRAFFIC = {"c3": (4.26e8, "profiles/k_fly_all_c3_hbm_traffic.md")}
 
 WORKLOADS = {
     # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
@@ -127,13 +130,13 @@ def pool_ab(PA, args):
             r = ga.generation(0, params, None)
             ga.set_collect(False)
             t0 = time.perf_counter()
-            r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=2e-3)
+            r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=args.lr)
             r["wall_ms"] += 1000 * (time.perf_counter() - t0)
             gens.append(r)
             log(f"[{mode}] gen 0 (bootstrap): best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
                 f"{r['holdout']:.3f} {r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
         for gen in range(len(gens), args.gens):
-            r = ga.generation(gen, params, S, train_epochs=args.train_epochs, lr=1e-3)
+            r = ga.generation(gen, params, S, train_epochs=args.train_epochs, lr=args.lr)
             gens.append(r)
             log(f"[{mode}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout {r['holdout']:.3f} "
                 f"(best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) "
@@ -240,10 +243,11 @@ def main():
     ap.add_argument("--fly", type=int, default=100_000, help="c4: candidates flown after pre-screening")
     ap.add_argument("--elite", type=int, default=10_000, help="c4: elite size")
     ap.add_argument("--gens", type=int, default=6, help="c4: generations per mode")
-    ap.add_argument("--buffer", type=int, default=500_000, help="c4: surrogate replay buffer")
-    ap.add_argument("--explore", type=float, default=0.1, help="c4: fraction of the fly set drawn at random")
-    ap.add_argument("--boot-epochs", type=int, default=8, help="c4: surrogate epochs on the bootstrap generation")
-    ap.add_argument("--train-epochs", type=int, default=1, help="c4: surrogate epochs per generation")
+    ap.add_argument("--buffer", type=int, default=300_000, help="c4: surrogate replay buffer")
+    ap.add_argument("--explore", type=float, default=0.3, help="c4: fraction of the fly set drawn at random")
+    ap.add_argument("--boot-epochs", type=int, default=16, help="c4: surrogate epochs on the bootstrap generation")
+    ap.add_argument("--train-epochs", type=int, default=4, help="c4: surrogate epochs per generation")
+    ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -325,6 +329,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": None, "avg_launch_ms": round(ms / max(1, nl), 4), "launches": nl,
             "flops_per_launch": round(flops / max(1, nl))}
+    if name == "k_fly_all" and args.workload in TRAFFIC:
+        roof["traffic"], roof["traffic_source"] = TRAFFIC[args.workload]
     log(f"  k_fly_all: {fused['launches']} launches, {fused['ms']:.2f} ms")
     last = E.stats()[0]
     for k, ch in enumerate(last["chunks"]):
