@@ -389,6 +389,8 @@ struct Flyer {
     // h = 0.95 sqrt(1 - s^2) + |s|, c1 = q s (cf - 2h), c2 = -q cf / |v_r|.
     // The reference skips panels with |v_r|^2 < 1e-6 (its NaN guard); here rsq(|v_r|^2 + 1e-30)
     // keeps v_r = 0 finite with zero force, and 0 < |v_r|^2 < 1e-6 contributes < 1e-9 N.
+    // FIRST: the lane's first panel initialises the sums instead of adding to zero.
+    template <bool FIRST = false>
     __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
                                                 float cf, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
                                                 float& Tz) const {
@@ -408,10 +410,17 @@ struct Flyer {
         const float fx = c1 * nx[p] + c2 * vrx;
         const float fy = c1 * ny[p] + c2 * vry;
         const float fz = c1 * nz[p] + c2 * vrz;
-        Fx += fx; Fy += fy; Fz += fz;
-        Tx = __builtin_fmaf(ry[p], fz, __builtin_fmaf(-rz[p], fy, Tx));
-        Ty = __builtin_fmaf(rz[p], fx, __builtin_fmaf(-rx[p], fz, Ty));
-        Tz = __builtin_fmaf(rx[p], fy, __builtin_fmaf(-ry[p], fx, Tz));
+        if (FIRST) {
+            Fx = fx; Fy = fy; Fz = fz;
+            Tx = __builtin_fmaf(ry[p], fz, -rz[p] * fy);
+            Ty = __builtin_fmaf(rz[p], fx, -rx[p] * fz);
+            Tz = __builtin_fmaf(rx[p], fy, -ry[p] * fx);
+        } else {
+            Fx += fx; Fy += fy; Fz += fz;
+            Tx = __builtin_fmaf(ry[p], fz, __builtin_fmaf(-rz[p], fy, Tx));
+            Ty = __builtin_fmaf(rz[p], fx, __builtin_fmaf(-rx[p], fz, Ty));
+            Tz = __builtin_fmaf(rx[p], fy, __builtin_fmaf(-ry[p], fx, Tz));
+        }
     }
 
     // One step (sim.cu:129-205) for the group; returns true when the flight ended (ground,
@@ -440,11 +449,14 @@ struct Flyer {
         const float vby = r01 * ux + r11 * uy + r21 * uz;
         const float vbz = r02 * ux + r12 * uy + r22 * uz;
         const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
-        float Fx = 0, Fy = 0, Fz = 0, Tx = 0, Ty = 0, Tz = 0;
+        float Fx, Fy, Fz, Tx, Ty, Tz;
         // slots in pairs: two independent panel evaluations per branch give the scheduler
-        // instruction-level parallelism to interleave (at most one padded slot evaluated)
+        // instruction-level parallelism to interleave (at most one padded slot evaluated);
+        // the first pair always runs (a slot past the lane's triangles adds exactly zero)
+        panel_force<true>(0, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+        if (P > 1) panel_force(1, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
 #pragma unroll
-        for (int p = 0; p < P; p += 2) {
+        for (int p = 2; p < P; p += 2) {
             if (p < npw) {
                 panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
                 if (p + 1 < P) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
@@ -467,11 +479,14 @@ struct Flyer {
         const float dwy = B.v1 * tqx + B.v4 * tqy + B.v5 * tqz;
         const float dwz = B.v2 * tqx + B.v5 * tqy + B.v8 * tqz;
         float nwx = wx + dwx * PR_DT, nwy = wy + dwy * PR_DT, nwz = wz + dwz * PR_DT;
-        const float wl = sqrtf(nwx * nwx + nwy * nwy + nwz * nwz);
+        // hardware v_sqrt / v_rsq (<= 1 ulp) for the clamps and the renormalisation: the
+        // IEEE-exact forms cost ~13 instructions each, ~50 per step (the reference's GPU
+        // build uses the approximate forms too, via -use_fast_math)
+        const float wl = __builtin_amdgcn_sqrtf(nwx * nwx + nwy * nwy + nwz * nwz);
         if (wl > 150.f) { const float s = 150.f / wl; nwx *= s; nwy *= s; nwz *= s; }
         const float ks = B.ks;
         float nvx = B.vx + Fwx * ks, nvy = B.vy + Fwy * ks, nvz = B.vz + Fwz * ks;
-        const float vl = sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
+        const float vl = __builtin_amdgcn_sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
         if (vl > 60.f) { const float s = 60.f / vl; nvx *= s; nvy *= s; nvz *= s; }
         const float npx = B.px + nvx * PR_DT, npy = B.py + nvy * PR_DT, npz = B.pz + nvz * PR_DT;
         // q += 0.5 dt q (x) (0, w)
@@ -481,7 +496,7 @@ struct Flyer {
         const float dqz = qw * nwz + qx * nwy - qy * nwx;
         const float hd = 0.5f * PR_DT;
         float nqw = qw + dqw * hd, nqx = qx + dqx * hd, nqy = qy + dqy * hd, nqz = qz + dqz * hd;
-        const float il = 1.0f / sqrtf(nqw * nqw + nqx * nqx + nqy * nqy + nqz * nqz);
+        const float il = __builtin_amdgcn_rsqf(nqw * nqw + nqx * nqx + nqy * nqy + nqz * nqz);
         B.qw = nqw * il; B.qx = nqx * il; B.qy = nqy * il; B.qz = nqz * il;
         B.vx = nvx; B.vy = nvy; B.vz = nvz;
         B.wx = nwx; B.wy = nwy; B.wz = nwz;

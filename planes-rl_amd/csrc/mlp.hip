@@ -35,9 +35,10 @@ const int LN[NL] = {HID, HID, HID, NOP};
 __device__ __forceinline__ __bf16 to_bf(float x) { return (__bf16)x; }  // round to nearest even
 __device__ __forceinline__ float from_bf(__bf16 x) { return (float)x; }
 
-enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2, EPI_ATOMIC_F32 = 3 };
+enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2, EPI_PART_F32 = 3 };
 
-// C[M x N] = A(m,k) * B(k,n) over k in [0, K) (split over blockIdx.z for EPI_ATOMIC_F32).
+// C[M x N] = A(m,k) * B(k,n) over k in [0, K) (split over blockIdx.z for EPI_PART_F32, which
+// writes split z's partial product to C + z * ldc * gridDim.x * 16 for an ordered sum after).
 // A(m,k) = TA ? A[k*lda + m] : A[m*lda + k];  B(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n].
 // M, N multiples of 16, K a multiple of 32 (per split).
 template <bool TA, bool TB, int EPI>
@@ -74,7 +75,8 @@ __global__ __launch_bounds__(64) void gemm_bf16(const __bf16* __restrict__ A, in
             const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
             reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = to_bf(v * keep);
         } else {
-            atomicAdd(reinterpret_cast<float*>(C) + (size_t)m * ldc + n, v);
+            const size_t plane = (size_t)gridDim.x * 16 * ldc;
+            reinterpret_cast<float*>(C)[blockIdx.z * plane + (size_t)m * ldc + n] = v;
         }
     }
 }
@@ -128,6 +130,7 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
 // an LDS sum in wave order) into part[y][cols]; k_colsum_fin adds the COLSUM_SLICES partials
 // of each column in slice order.
 constexpr int COLSUM_SLICES = 32;
+constexpr int WG_SPLIT = 512;  // batch rows per weight-gradient partial product
 __global__ __launch_bounds__(256) void k_colsum_part(const __bf16* __restrict__ G, int rows, int cols,
                                                      float* __restrict__ part) {
     __shared__ float red[4][64];
@@ -152,6 +155,15 @@ __global__ void k_colsum_fin(const float* __restrict__ part, int cols, float* __
     float s = 0.0f;
     for (int y = 0; y < COLSUM_SLICES; y++) s += part[(size_t)y * cols + c];
     out[c] = s;
+}
+
+// out[i] = sum over the splits of part[z * count + i], in split order (deterministic)
+__global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int count, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.0f;
+    for (int z = 0; z < nsplit; z++) s += part[(size_t)z * count + i];
+    out[i] = s;
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -222,6 +234,7 @@ struct pr_mlp {
     float* lo = nullptr;
     float* inv_range = nullptr;
     float* colsum_part = nullptr;  // [COLSUM_SLICES x 256] bias-gradient partial sums
+    float* wgrad_part = nullptr;   // [batch splits x 256 x 256] weight-gradient partial products
     int step = 0;
     hipStream_t s = nullptr;
 };
@@ -283,13 +296,16 @@ static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, 
     for (int L = NL - 1; L >= 0; L--) {
         const __bf16* Xin = L == 0 ? q->X : q->H[L - 1];
         const int K = LK[L], N = LN[L];
-        // dW[K x N] = Xin^T [K x B] * dY [B x N], split over the batch
-        const int ksplit = 512;
-        dim3 gw(K / 16, N / 16, (B + ksplit - 1) / ksplit);
-        const int kps = ksplit;
-        // the batch is padded to a multiple of 32 with zero rows (zero activations and gradients)
-        hipLaunchKernelGGL((gemm_bf16<true, false, EPI_ATOMIC_F32>), gw, dim3(64), 0, q->s, Xin, K, dY, ldy, B, kps,
-                           nullptr, nullptr, 0, (void*)(q->g + q->off_w[L]), N);
+        // dW[K x N] = Xin^T [K x B] * dY [B x N]: partial products per batch split of
+        // WG_SPLIT rows, then summed in split order (deterministic, no atomics); the batch is
+        // padded to a multiple of 32 with zero rows (zero activations and gradients)
+        const int nsplit = (B + WG_SPLIT - 1) / WG_SPLIT;
+        dim3 gw(K / 16, N / 16, nsplit);
+        hipLaunchKernelGGL((gemm_bf16<true, false, EPI_PART_F32>), gw, dim3(64), 0, q->s, Xin, K, dY, ldy, B, WG_SPLIT,
+                           nullptr, nullptr, 0, (void*)q->wgrad_part, N);
+        MCK(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_splits, dim3((K * N + 255) / 256), dim3(256), 0, q->s, q->wgrad_part, nsplit, K * N,
+                           q->g + q->off_w[L]);
         MCK(hipGetLastError());
         // db = column sums of dY
         hipLaunchKernelGGL(k_colsum_part, dim3((N + 63) / 64, COLSUM_SLICES), dim3(256), 0, q->s, dY, B, N,
@@ -360,6 +376,7 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
     MALLOC(q->colsum_part, 4 * (size_t)COLSUM_SLICES * HID);
+    MALLOC(q->wgrad_part, 4 * (size_t)((B + WG_SPLIT - 1) / WG_SPLIT) * HID * HID);
 #undef MALLOC
     float lo[NIN], hi[NIN], ir[NIN];
     pr_gene_bounds(lo, hi);
@@ -390,7 +407,8 @@ void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
     void* bufs[] = {q->w, q->g, q->m, q->v, q->wb, q->X, q->out, q->dout, q->dout_f, q->loss, q->lo, q->inv_range,
-                    q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2], q->colsum_part};
+                    q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2], q->colsum_part,
+                    q->wgrad_part};
     for (void* b : bufs) if (b) (void)hipFree(b);
     if (q->s) (void)hipStreamDestroy(q->s);
     delete q;

@@ -119,6 +119,7 @@ def test_mlp_learns_flight_fitness():
     rho = np.corrcoef(ra, rb)[0, 1]
     acc = np.mean((pred[:, 3] > 0.5) == (tg[n_tr:, 3] > 0.5))
     mae = np.mean(np.abs(reg[ok] - true_score[ok]))
-    print(f"surrogate: loss {l0:.3f} -> {l1:.3f}, held-out Spearman {rho:.3f}, oversize accuracy {acc:.3f}, "
-          f"MAE {mae:.2f} m")
-    assert l1 < 0.5 * l0 and rho > 0.45 and acc > 0.85
+    base = max(np.mean(~ok), np.mean(ok))  # accuracy of always guessing the majority class
+    print(f"surrogate: loss {l0:.3f} -> {l1:.3f}, held-out Spearman {rho:.3f}, oversize accuracy {acc:.3f} "
+          f"(majority class {base:.3f}), MAE {mae:.2f} m")
+    assert l1 < 0.5 * l0 and rho > 0.45 and acc > 0.8 and acc > base + 0.1
