@@ -72,6 +72,7 @@ struct pr_ctx {
     std::vector<int> grid_w;  // persistent grid of the bucket's one-flight-per-wave mapping
     std::vector<int> vid_p, vid_w;  // fused-kernel variant ids per bucket (-1: own k_fly launch)
     int grid_f = 0;                 // persistent grid of the fused kernel
+    int simds = 0;                  // SIMDs on the device (4 per CU)
     hipEvent_t ev[2 * 512 + 2] = {};
     int ev_slot[512] = {};          // slot of a timed launch; -1 = the fused kernel
     int ev_chunk[512] = {};         // chunk of a timed launch
@@ -159,6 +160,7 @@ static int upload_buckets(pr_ctx* c) {
     int occ = 0;
     HCK(fly_all_occupancy(&occ));
     c->grid_f = (occ < 1 ? 1 : occ) * ncu;
+    c->simds = 4 * ncu;
     return 0;
 }
 
@@ -368,8 +370,11 @@ int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_s
 // clears the next chunk's counters); a bucket outside it gets its own k_fly launches.
 // Chunk ck: list_in = lists[ck & 1], counts: in = ck % 3, out = (ck + 1) % 3, cleared =
 // (ck + 2) % 3; queue head ck & 1, cleared (ck + 1) & 1 (k_init fills list 0 / count 0).
-static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed, int* n_events) {
+static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long nf, bool timed, int* n_events) {
     const int NB = (int)c->buckets.size();
+    // a batch that fits one flight per SIMD (e.g. the 8 x 128 holdout) launches one wave per
+    // SIMD: every flight then has a SIMD to itself and its steps run at single-wave latency
+    const int grid = nf <= c->simds ? std::min(c->simds, c->grid_f) : c->grid_f;
     const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
     const int n_ev = (int)(sizeof(c->ev) / sizeof(c->ev[0]));
     bool any_fused = false, any_own = false;
@@ -385,7 +390,7 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
     for (int b = 0; b < 8; b++) {
         F.vid_p[b] = b < NB ? c->vid_p[b] : -1;
         F.vid_w[b] = b < NB ? c->vid_w[b] : -1;
-        F.thr[b] = c->grid_f;  // one flight per wave once they fit one per wave
+        F.thr[b] = grid;  // one flight per wave once they fit one per wave
     }
     int ne = 0;
     for (int ck = 0; ck < n_chunks; ck++) {
@@ -398,7 +403,7 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, bool timed
         if (any_fused) {
             const bool tev = timed && ne + 2 <= n_ev;
             if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
-            HCK(launch_fly_all(F, c->grid_f, c->stream));
+            HCK(launch_fly_all(F, grid, c->stream));
             if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; c->ev_chunk[ne / 2] = cs; ne += 2; }
         } else {
             HCK(launch_zero_chunk(c->d_counts, NB, F.c_in, F.c_zero, F.h_zero, F.chunk_n, c->stream));
@@ -472,7 +477,7 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
             HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
             int unused = 0;
-            if (rc == 0) rc = issue_chunks(c, p, a, false, &unused);
+            if (rc == 0) rc = issue_chunks(c, p, a, nf, false, &unused);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(c->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -488,7 +493,7 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
     } else {
         int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
         if (rc) return rc;
-        rc = issue_chunks(c, p, a, true, &ne);
+        rc = issue_chunks(c, p, a, nf, true, &ne);
         if (rc) return rc;
     }
     HCK(hipStreamSynchronize(c->stream));

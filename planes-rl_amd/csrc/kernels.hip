@@ -523,9 +523,11 @@ struct Flyer {
 
 // Persistent loop over one work list: groups of L lanes take flights with one atomic each,
 // fly them to the end of the chunk (or of the flight), store the state and append the
-// survivors to the next chunk's list.  Returns when the list is drained.
+// survivors to the next chunk's list.  A group stops taking flights once the queue head
+// has reached `limit` (<= n_in; the rest is left to a wider mapping); the function returns
+// when every group of the wave has stopped.
 template <int L, int P>
-__device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
+__device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in, int limit) {
     const int lane = threadIdx.x & 63;
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
@@ -540,7 +542,11 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in) {
         if (__ballot(need)) {
             if (need) {
                 int k = 0;
-                if (j == 0) k = atomicAdd(a.head, 1);
+                if (j == 0) {
+                    // check before claiming: a claim that races past `limit` is still flown here
+                    const int h = __atomic_load_n(a.head, __ATOMIC_RELAXED);
+                    k = h < limit ? atomicAdd(a.head, 1) : n_in;
+                }
                 k = __shfl(k, gbase);
                 if (k >= n_in) exhausted = true;
                 else { F.setup(a, a.list_in[k], j); active = true; }
@@ -582,16 +588,18 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
     // flight count decides which one works, so the choice needs no host synchronisation
     if (n_in < a.run_lo || n_in > a.run_hi) return;
-    fly_list<L, P>(a, n_in);
+    fly_list<L, P>(a, n_in, n_in);
 }
 
 // Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
 #define PR_FUSED(X) \
     X(0, 8, 12) X(1, 8, 24) X(2, 16, 12) X(3, 16, 24) X(4, 32, 12) X(5, 32, 24) X(6, 64, 8) X(7, 64, 16)
 
-// One chunk of every bucket in one launch: each wave drains bucket 0's list, then bucket
-// 1's, and so on, flying each bucket with its own (L, P) code path -- its many-per-wave
-// mapping, or its one-flight-per-wave mapping when the bucket's flights fit one per wave.
+// One chunk of every bucket in one launch: each wave drains the buckets' lists, largest
+// triangle counts first, flying each bucket with its own (L, P) code path: the
+// many-flights-per-wave mapping until no more than thr[b] flights (one per wave) are left
+// in the queue, then the one-flight-per-wave mapping for those -- the last flights of a
+// chunk run at the lowest step latency instead of leaving most lanes of a partial round idle.
 __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     if (blockIdx.x == 0 && threadIdx.x < A.nb) {
         A.counts[PR_CNT * threadIdx.x + A.c_zero] = 0;
@@ -610,13 +618,25 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         a.count_in = cb + A.c_in;
         a.count_out = cb + A.c_out;
         a.head = cb + 3 + A.h_run;
-        const bool wide = A.vid_w[b] >= 0 && n_in <= A.thr[b];
-        a.stats = A.stats + 2 * (2 * b + (wide ? 1 : 0));
-        switch (wide ? A.vid_w[b] : A.vid_p[b]) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in); break;
-            PR_FUSED(CASE)
+        const int vw = A.vid_w[b];
+        const int limit = vw >= 0 ? n_in - A.thr[b] : n_in;
+        if (limit > 0) {
+            a.stats = A.stats + 2 * (2 * b);
+            switch (A.vid_p[b]) {
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in, limit); break;
+                PR_FUSED(CASE)
 #undef CASE
-            default: break;
+                default: break;
+            }
+        }
+        if (vw >= 0) {
+            a.stats = A.stats + 2 * (2 * b + 1);
+            switch (vw) {
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in, n_in); break;
+                PR_FUSED(CASE)
+#undef CASE
+                default: break;
+            }
         }
     }
 }
