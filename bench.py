@@ -250,6 +250,8 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="c1-c3: finish each generation's holdout before the next generation starts")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -293,9 +295,23 @@ def main():
     bucket = {}
     fused = {"flight_steps": 0, "panel_steps": 0, "ms": 0.0, "launches": 0}
     t0 = time.perf_counter()
+    pipelined = not args.no_pipeline
     for k in range(args.steps):
-        r = E.evaluate(w["pop"], seeds_for(k, w["nseeds"]), params)
-        for st in E.stats():
+        if pipelined:
+            # generation k's holdout flies on its own stream while generation k+1's
+            # population flies; the last holdout is collected inside the timed region
+            E.evaluate_async(w["pop"], seeds_for(k, w["nseeds"]), params)
+            st_list = [E.stats()[0]]
+            if k > 0:
+                r = E.collect()
+                st_list.append(E.stats()[1])
+            if k == args.steps - 1:
+                r = E.collect()
+                st_list.append(E.stats()[1])
+        else:
+            r = E.evaluate(w["pop"], seeds_for(k, w["nseeds"]), params)
+            st_list = E.stats()
+        for st in st_list:
             fsteps += st["flight_steps"]
             fused["ms"] += st["fused_ms"]
             fused["launches"] += st["fused_launches"]

@@ -88,6 +88,12 @@ int pr_scenarios(pr_ctx* ctx, const int* d_seeds, int n, const pr_sim_params* p)
  * Returns when d_fitness is complete (like gpu_sim, sim.cu:238-248). */
 int pr_fly(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
            int* d_steps);
+/* pr_fly in two halves: pr_fly_async queues the flight batch on the context's stream and
+ * returns; pr_fly_wait waits for it and gathers pr_stats. Results (d_fitness, d_steps,
+ * readbacks, stats) are valid after pr_fly_wait; one batch per context at a time. */
+int pr_fly_async(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                 int* d_steps);
+int pr_fly_wait(pr_ctx* ctx);
 
 /* Only the flight set-up of pr_fly: per-flight constants and launch states (sim.cu:63-109),
  * fitness -100 for unprintable genomes.  pr_read_states then returns the launch states. */
@@ -174,9 +180,19 @@ int pr_design_stats(const float* samples, int n, float* out);
 int pr_audit(pr_ctx* ctx, const float* d_genomes, const int* d_res, int n_genomes, int n_scen, int base_seed,
              const pr_sim_params* p, float* d_fitness, int* d_steps, float* stats, int nthreads);
 
-/* The evaluator's flight contexts (0: population flights, 1: holdout re-test), so that the
- * pr_set_* options apply to it; owned by the evaluator. */
+/* The evaluator's flight contexts (0: population flights, 1 and 2: the two holdout slots),
+ * so that the pr_set_* options apply to it; owned by the evaluator. */
 pr_ctx* pr_gen_context(pr_gen* e, int which);
+
+/* Pipelined generation evaluation: the population part of pr_gen_evaluate (acc, idx,
+ * samples valid on return), with the top-8 holdout batch left flying on its own stream;
+ * pr_gen_collect waits for the oldest such batch and returns its holdout score and
+ * champion. Two batches may be outstanding, so generation g+1's population flights overlap
+ * generation g's holdout. No reference counterpart: run_ga's holdout (sim.cu:446-455)
+ * never feeds breeding, so deferring it changes no result. -2: both slots outstanding. */
+int pr_gen_evaluate_async(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds,
+                          int nseeds, int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples);
+int pr_gen_collect(pr_gen* e, float* holdout_out, int* champion_out);
 
 /* ---------------------------------------------------------------- surrogate MLP
  * BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU) on the matrix cores, FP32 accumulation,

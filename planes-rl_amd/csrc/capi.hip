@@ -82,6 +82,7 @@ struct pr_ctx {
     int prep_fp64 = 0;
     int tail_wide = 1;
     int fused = 1;
+    int pending = 0, pend_events = 0, pend_chunks = 0;  // an asynchronous pr_fly in flight
     hipGraphExec_t gexec = nullptr;
     long long gkey[10] = {};
     pr_stats stats = {};
@@ -461,9 +462,10 @@ int pr_set_graph(pr_ctx* c, int on) {
     return 0;
 }
 
-int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness, int* d_steps) {
+int pr_fly_async(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
+                 int* d_steps) {
     ARG(c && p, "pr_fly: null argument");
-    const int NB = (int)c->buckets.size();
+    ARG(!c->pending, "pr_fly: the previous asynchronous flight batch has not been waited for");
     FlyArgs a;
     fill_fly_args(c, p, d_fitness, d_steps, a);
     int ne = 0;
@@ -496,7 +498,20 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
         rc = issue_chunks(c, p, a, nf, true, &ne);
         if (rc) return rc;
     }
+    c->pending = 1;
+    c->pend_events = ne;
+    c->pend_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
+    return 0;
+}
+
+// Waits for the context's flight batch and gathers its statistics.
+int pr_fly_wait(pr_ctx* c) {
+    ARG(c, "pr_fly_wait: null context");
+    if (!c->pending) return 0;
+    c->pending = 0;
     HCK(hipStreamSynchronize(c->stream));
+    const int NB = (int)c->buckets.size();
+    const int ne = c->pend_events;
     pr_stats& S = c->stats;
     S = pr_stats();
     S.n_slots = 2 * NB;
@@ -516,7 +531,7 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
         S.c_ms[c->ev_chunk[k / 2]] += t;
     }
     {
-        const int n_chunks = (p->max_steps + p->chunk - 1) / p->chunk;
+        const int n_chunks = c->pend_chunks;
         S.n_chunks = n_chunks < PR_MAX_CHUNKS ? n_chunks : PR_MAX_CHUNKS;
         unsigned long long cst[2 * PR_MAX_CHUNKS];
         int cn[PR_MAX_CHUNKS * NB_MAX];
@@ -546,6 +561,11 @@ int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params
     float pm = 0;
     if (hipEventElapsedTime(&pm, c->prep0, c->prep1) == hipSuccess) S.prep_ms = pm;
     return 0;
+}
+
+int pr_fly(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness, int* d_steps) {
+    const int rc = pr_fly_async(c, layout, n_genomes, n_scen, p, d_fitness, d_steps);
+    return rc ? rc : pr_fly_wait(c);
 }
 
 int pr_fly_record(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,

@@ -86,9 +86,18 @@ struct pr_ga {
 // ------------------------------------------------------------------ generation evaluator
 struct pr_gen {
     pr_ctx* main = nullptr;
-    pr_ctx* hold = nullptr;
     int pop = 0, nseeds_max = 0, hold_n = 128;
-    DevBuf seeds, fit, steps, scores, hseeds, hgen, hfit, hsteps, hscores;
+    DevBuf seeds, fit, steps, scores, hseeds;
+    // two holdout slots, so that one generation's 8 x 128 holdout batch can fly on its own
+    // stream while the next generation's population flies (pr_gen_evaluate_async)
+    struct Hold {
+        pr_ctx* ctx = nullptr;
+        DevBuf gen, fit, steps, scores;
+        int pending = 0, k = 0;
+        int idx[8] = {};
+    } hold[2];
+    int next_slot = 0;       // slot the next holdout batch goes to
+    int oldest = -1;         // oldest uncollected slot (-1: none)
     std::vector<float> samples, acc_tmp;
     pr_stats st_main = {}, st_hold = {};
 };
@@ -166,45 +175,43 @@ int pr_gen_create(pr_gen** out, int pop, int nseeds_max, int max_ch, int max_sp)
     e->pop = pop;
     e->nseeds_max = nseeds_max;
     int rc = pr_create(&e->main, pop, 0, nseeds_max, pop * nseeds_max, max_ch, max_sp);
-    if (rc) { delete e; return rc; }
-    rc = pr_create(&e->hold, 8, 0, e->hold_n, 8 * e->hold_n, max_ch, max_sp);
-    if (rc) { pr_destroy(e->main); delete e; return rc; }
+    for (int s = 0; s < 2 && !rc; s++) rc = pr_create(&e->hold[s].ctx, 8, 0, e->hold_n, 8 * e->hold_n, max_ch, max_sp);
+    if (rc) { pr_gen_destroy(e); return rc; }
     e->seeds.ensure(sizeof(int) * nseeds_max);
     e->fit.ensure(sizeof(float) * pop * nseeds_max);
     e->steps.ensure(sizeof(int) * pop * nseeds_max);
     e->scores.ensure(sizeof(float) * pop);
     e->hseeds.ensure(sizeof(int) * e->hold_n);
-    e->hgen.ensure(sizeof(float) * NG * 8);
-    e->hfit.ensure(sizeof(float) * 8 * e->hold_n);
-    e->hsteps.ensure(sizeof(int) * 8 * e->hold_n);
-    e->hscores.ensure(sizeof(float) * 8);
+    for (auto& h : e->hold) {
+        h.gen.ensure(sizeof(float) * NG * 8);
+        h.fit.ensure(sizeof(float) * 8 * e->hold_n);
+        h.steps.ensure(sizeof(int) * 8 * e->hold_n);
+        h.scores.ensure(sizeof(float) * 8);
+    }
     std::vector<int> hs(e->hold_n);
     for (int p = 0; p < e->hold_n; p++) hs[p] = 100003 + p * 7919;  // holdout suite (sim.cu:64,452)
-    CKP(pr_h2d(e->hseeds.p, hs.data(), sizeof(int) * e->hold_n));
+    if ((rc = pr_h2d(e->hseeds.p, hs.data(), sizeof(int) * e->hold_n))) { pr_gen_destroy(e); return rc; }
     *out = e;
     return 0;
 }
 
-// The evaluator's flight contexts (0: population flights, 1: holdout re-test), for the
-// pr_set_* options (fused kernel, tail mapping, buckets, graph, prepare precision).
-pr_ctx* pr_gen_context(pr_gen* e, int which) { return !e ? nullptr : which ? e->hold : e->main; }
+// The evaluator's flight contexts (0: population flights, 1 and 2: the two holdout slots),
+// for the pr_set_* options (fused kernel, tail mapping, buckets, graph, prepare precision).
+pr_ctx* pr_gen_context(pr_gen* e, int which) {
+    return !e || which < 0 || which > 2 ? nullptr : which == 0 ? e->main : e->hold[which - 1].ctx;
+}
 
 void pr_gen_destroy(pr_gen* e) {
     if (!e) return;
     pr_destroy(e->main);
-    pr_destroy(e->hold);
+    for (auto& h : e->hold) pr_destroy(h.ctx);
     delete e;
 }
 
-// One generation's evaluation (sim.cu:432-456), batched: every genome x every seed in one
-// pr_fly, robust scores on the GPU, std::sort ranking on the host, then the top
-// min(holdout_k, pop) genomes re-flown on the 128-scenario holdout suite in one batch.
-// seeds: host [nseeds].  Outputs (host): acc[pop], idx[pop], optional samples
-// [pop*nseeds], holdout score and champion genome index.
-int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds, int nseeds,
-                    int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
-                    float* holdout_out, int* champion_out) {
-    if (!e || pop < 1 || pop > e->pop || nseeds < 1 || nseeds > e->nseeds_max || !acc || !idx) return -1;
+// Population part of a generation (sim.cu:432-441): every genome x every seed in one batch,
+// robust scores, std::sort ranking on the host.
+static int gen_main(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds, int nseeds,
+                    const pr_sim_params* p, float* acc, int* idx, float* samples) {
     int rc;
     if ((rc = pr_h2d(e->seeds.p, seeds, sizeof(int) * nseeds))) return rc;
     if ((rc = pr_prepare(e->main, d_genomes, d_res, pop))) return rc;
@@ -223,29 +230,105 @@ int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop
         if (samples) std::memcpy(samples, e->samples.data(), sizeof(float) * pop * nseeds);
     }
     pr_ga_rank(acc, pop, idx);
+    return 0;
+}
+
+// Holdout re-test (sim.cu:446-455) of the top min(holdout_k, 8) genomes on the fixed
+// 128-scenario suite (seed 100003 with the vary rule), launched on slot s's own stream.
+// The genomes are copied out before returning, so the caller may overwrite d_genomes.
+static int hold_launch(pr_gen* e, int s, const float* d_genomes, const int* idx, int k, const pr_sim_params* p) {
+    pr_gen::Hold& h = e->hold[s];
+    int rc;
+    for (int c = 0; c < k; c++)
+        if ((rc = hipMemcpyAsync((char*)h.gen.p + sizeof(float) * NG * c, d_genomes + (size_t)NG * idx[c],
+                                 sizeof(float) * NG, hipMemcpyDeviceToDevice, 0)))
+            return rc;
+    if ((rc = hipStreamSynchronize(0))) return rc;
+    pr_sim_params hp = *p;
+    hp.spread = 0;
+    if ((rc = pr_prepare(h.ctx, (const float*)h.gen.p, nullptr, k))) return rc;
+    if ((rc = pr_scenarios(h.ctx, (const int*)e->hseeds.p, e->hold_n, &hp))) return rc;
+    if ((rc = pr_fly_async(h.ctx, 0, k, e->hold_n, &hp, (float*)h.fit.p, (int*)h.steps.p))) return rc;
+    h.pending = 1;
+    h.k = k;
+    for (int c = 0; c < k; c++) h.idx[c] = idx[c];
+    return 0;
+}
+
+// Waits for slot s's holdout batch; champion = the first strict maximum of the holdout
+// robust scores (sim.cu:447-450), as a genome index of that generation.
+static int hold_collect(pr_gen* e, int s, float* holdout_out, int* champion_out) {
+    pr_gen::Hold& h = e->hold[s];
+    int rc;
+    if ((rc = pr_fly_wait(h.ctx))) return rc;
+    h.pending = 0;
+    pr_stats_get(h.ctx, &e->st_hold);
+    if ((rc = pr_scores(h.ctx, (const float*)h.fit.p, h.k, e->hold_n, (float*)h.scores.p))) return rc;
+    float hsc[8];
+    if ((rc = pr_d2h(hsc, h.scores.p, sizeof(float) * h.k))) return rc;
+    float best = -1e9f;
+    int champ = h.idx[0];
+    for (int c = 0; c < h.k; c++)
+        if (hsc[c] > best) { best = hsc[c]; champ = h.idx[c]; }
+    if (holdout_out) *holdout_out = best;
+    if (champion_out) *champion_out = champ;
+    return 0;
+}
+
+int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds, int nseeds,
+                    int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples,
+                    float* holdout_out, int* champion_out) {
+    if (!e || pop < 1 || pop > e->pop || nseeds < 1 || nseeds > e->nseeds_max || !acc || !idx || !p) return -1;
+    if (e->oldest >= 0) return -2;  // an asynchronous holdout batch is still uncollected
+    int rc = gen_main(e, d_genomes, d_res, pop, seeds, nseeds, p, acc, idx, samples);
+    if (rc) return rc;
     float holdout = -1e9f;
     int champion = idx[0];
-    int val_k = std::min(holdout_k, pop);
-    if (val_k > 0) {
-        for (int c = 0; c < val_k; c++)
-            if ((rc = hipMemcpyAsync((char*)e->hgen.p + sizeof(float) * NG * c, d_genomes + (size_t)NG * idx[c],
-                                     sizeof(float) * NG, hipMemcpyDeviceToDevice, 0)))
-                return rc;
-        if ((rc = hipDeviceSynchronize())) return rc;
-        pr_sim_params hp = *p;
-        hp.spread = 0;
-        if ((rc = pr_prepare(e->hold, (const float*)e->hgen.p, nullptr, val_k))) return rc;
-        if ((rc = pr_scenarios(e->hold, (const int*)e->hseeds.p, e->hold_n, &hp))) return rc;
-        if ((rc = pr_fly(e->hold, 0, val_k, e->hold_n, &hp, (float*)e->hfit.p, (int*)e->hsteps.p))) return rc;
-        pr_stats_get(e->hold, &e->st_hold);
-        if ((rc = pr_scores(e->hold, (const float*)e->hfit.p, val_k, e->hold_n, (float*)e->hscores.p))) return rc;
-        float hsc[8];
-        if ((rc = pr_d2h(hsc, e->hscores.p, sizeof(float) * val_k))) return rc;
-        for (int c = 0; c < val_k; c++)
-            if (hsc[c] > holdout) { holdout = hsc[c]; champion = idx[c]; }
+    const int k = std::min(std::min(holdout_k, pop), 8);
+    if (k > 0) {
+        if ((rc = hold_launch(e, 0, d_genomes, idx, k, p))) return rc;
+        if ((rc = hold_collect(e, 0, &holdout, &champion))) return rc;
     }
     if (holdout_out) *holdout_out = holdout;
     if (champion_out) *champion_out = champion;
+    return 0;
+}
+
+// Pipelined variant: the population part as pr_gen_evaluate, then the holdout batch is
+// launched on its own stream and the call returns; pr_gen_collect returns its result. At
+// most two holdout batches may be outstanding, so generation g+1's population flights can
+// overlap generation g's holdout. Returns -2 when both slots are still uncollected.
+int pr_gen_evaluate_async(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds,
+                          int nseeds, int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples) {
+    if (!e || pop < 1 || pop > e->pop || nseeds < 1 || nseeds > e->nseeds_max || !acc || !idx || !p) return -1;
+    const int s = e->next_slot;
+    if (e->hold[s].pending) return -2;
+    int rc = gen_main(e, d_genomes, d_res, pop, seeds, nseeds, p, acc, idx, samples);
+    if (rc) return rc;
+    const int k = std::min(std::min(holdout_k, pop), 8);
+    e->hold[s].k = 0;
+    e->hold[s].idx[0] = idx[0];
+    if (k > 0 && (rc = hold_launch(e, s, d_genomes, idx, k, p))) return rc;
+    if (k <= 0) e->hold[s].pending = 2;  // nothing to fly: collect returns -1e9 and idx[0]
+    if (e->oldest < 0) e->oldest = s;
+    e->next_slot = s ^ 1;
+    return 0;
+}
+
+// Result of the oldest outstanding pr_gen_evaluate_async holdout batch (waits for it).
+int pr_gen_collect(pr_gen* e, float* holdout_out, int* champion_out) {
+    if (!e || e->oldest < 0) return -1;
+    const int s = e->oldest;
+    pr_gen::Hold& h = e->hold[s];
+    int rc = 0;
+    if (h.pending == 2) {
+        h.pending = 0;
+        if (holdout_out) *holdout_out = -1e9f;
+        if (champion_out) *champion_out = h.idx[0];
+    } else if ((rc = hold_collect(e, s, holdout_out, champion_out))) {
+        return rc;
+    }
+    e->oldest = e->hold[s ^ 1].pending ? (s ^ 1) : -1;
     return 0;
 }
 

@@ -110,6 +110,9 @@ def _load():
     lib.pr_design_stats.argtypes = [F32P, C.c_int, F32P]
     lib.pr_audit.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, F32P, C.c_int]
     lib.pr_gen_context.argtypes = [VP, C.c_int]
+    lib.pr_gen_evaluate_async.argtypes = [VP, VP, VP, C.c_int, I32P, C.c_int, C.c_int, C.POINTER(SimParams), F32P,
+                                          I32P, VP]
+    lib.pr_gen_collect.argtypes = [VP, C.POINTER(C.c_float), C.POINTER(C.c_int)]
     lib.pr_gen_context.restype = VP
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
     lib.pr_mlp_last_error.restype = C.c_char_p
@@ -479,13 +482,35 @@ class Generation:
         return a.as_dict(), b.as_dict()
 
     def set_fused(self, on=True):
-        """Fused flight kernel (default) or one launch per bucket, for both flight contexts."""
-        for which in (0, 1):
+        """Fused flight kernel (default) or one launch per bucket, for every flight context."""
+        for which in (0, 1, 2):
             _ck(LIB.pr_set_fused(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_fused")
 
     def set_graph(self, on=True):
-        for which in (0, 1):
+        for which in (0, 1, 2):
             _ck(LIB.pr_set_graph(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_graph")
+
+    def evaluate_async(self, pop, seeds, params, holdout_k=8, want_samples=False):
+        """Population part now (acc, idx[, samples]); the holdout batch keeps flying on its
+        own stream until collect()."""
+        seeds = np.ascontiguousarray(seeds, np.int32)
+        acc = np.zeros(pop, np.float32)
+        idx = np.zeros(pop, np.int32)
+        samples = np.zeros(pop * len(seeds), np.float32) if want_samples else None
+        _ck(LIB.pr_gen_evaluate_async(self.h, self.d_genomes.ptr, self.d_res.ptr if self.has_res else None, pop,
+                                      seeds, len(seeds), holdout_k, C.byref(params), acc, idx,
+                                      None if samples is None else samples.ctypes.data), "pr_gen_evaluate_async")
+        out = {"acc": acc, "idx": idx}
+        if samples is not None:
+            out["samples"] = samples.reshape(pop, len(seeds))
+        return out
+
+    def collect(self):
+        """Holdout score and champion of the oldest outstanding evaluate_async."""
+        hold = C.c_float()
+        champ = C.c_int()
+        _ck(LIB.pr_gen_collect(self.h, C.byref(hold), C.byref(champ)), "pr_gen_collect")
+        return {"holdout": hold.value, "champion": champ.value}
 
     def __del__(self):
         try:

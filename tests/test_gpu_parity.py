@@ -387,3 +387,27 @@ def test_batched_audit_matches_single_design_flights_and_fp64(PA):
     print(f"audit champion {champ}: mean {st[champ, 5]:.2f} m, {np.mean(rel < 0.01) * 100:.1f}% of scenarios "
           f"within 1% of FP64")
     assert np.mean(rel < 0.01) >= 0.95
+
+
+def test_pipelined_generation_matches_synchronous(PA):
+    """pr_gen_evaluate_async + pr_gen_collect (the holdout batch of generation g flying on its
+    own stream while generation g+1's population flies) return exactly the synchronous
+    pr_gen_evaluate results, generation by generation."""
+    gs = O.random_genomes(600, 31)
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=3000)
+    seeds = [np.arange(1, 7, dtype=np.int32) + 6 * g for g in range(3)]
+    E = PA.Generation(600, 6)
+    E.upload(gs)
+    sync = [E.evaluate(600, s, params) for s in seeds]
+    A = PA.Generation(600, 6)
+    A.upload(gs)
+    out = [A.evaluate_async(600, seeds[0], params)]
+    for g in (1, 2):
+        out.append(A.evaluate_async(600, seeds[g], params))
+        out[g - 1].update(A.collect())
+    out[2].update(A.collect())
+    for g in range(3):
+        assert np.array_equal(out[g]["acc"], sync[g]["acc"]) and np.array_equal(out[g]["idx"], sync[g]["idx"]), g
+        assert out[g]["holdout"] == sync[g]["holdout"] and out[g]["champion"] == sync[g]["champion"], g
+    with pytest.raises(PA.PlanesError):
+        A.collect()  # nothing outstanding
