@@ -371,7 +371,8 @@ int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_s
 // clears the next chunk's counters); a bucket outside it gets its own k_fly launches.
 // Chunk ck: list_in = lists[ck & 1], counts: in = ck % 3, out = (ck + 1) % 3, cleared =
 // (ck + 2) % 3; queue head ck & 1, cleared (ck + 1) & 1 (k_init fills list 0 / count 0).
-static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long nf, bool timed, int* n_events) {
+static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long nf, bool timed, int* n_events,
+                        bool capturing) {
     const int NB = (int)c->buckets.size();
     // a batch that fits one flight per SIMD (e.g. the 8 x 128 holdout) launches one wave per
     // SIMD: every flight then has a SIMD to itself and its steps run at single-wave latency
@@ -380,6 +381,19 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
     const int n_ev = (int)(sizeof(c->ev) / sizeof(c->ev[0]));
     bool any_fused = false, any_own = false;
     for (int b = 0; b < NB; b++) (c->vid_p[b] >= 0 ? any_fused : any_own) = true;
+    // a bucket outside the fused kernel that received no flights needs no launches (lists
+    // only shrink between chunks); known from k_init's counts, except under graph capture
+    bool own_used[NB_MAX] = {};
+    if (any_own) {
+        int cnt[PR_CNT * NB_MAX];
+        if (!capturing) {
+            HCK(hipMemcpyAsync(cnt, c->d_counts, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+            HCK(hipStreamSynchronize(c->stream));
+        }
+        for (int b = 0; b < NB; b++) own_used[b] = c->vid_p[b] < 0 && (capturing || cnt[PR_CNT * b] > 0);
+        any_own = false;
+        for (int b = 0; b < NB; b++) any_own = any_own || own_used[b];
+    }
     FlyAllArgs F;
     F.base = a;
     F.lists0 = c->d_lists[0];
@@ -411,7 +425,7 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
         }
         if (!any_own) continue;
         for (int b = 0; b < NB; b++) {
-            if (c->vid_p[b] >= 0) continue;
+            if (!own_used[b]) continue;
             int* cb = c->d_counts + PR_CNT * b;
             a.list_in = c->d_lists[ck & 1] + (size_t)b * c->cap_flights;
             a.list_out = c->d_lists[(ck & 1) ^ 1] + (size_t)b * c->cap_flights;
@@ -479,7 +493,7 @@ int pr_fly_async(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_
             HCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
             int unused = 0;
-            if (rc == 0) rc = issue_chunks(c, p, a, nf, false, &unused);
+            if (rc == 0) rc = issue_chunks(c, p, a, nf, false, &unused, true);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(c->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -495,7 +509,7 @@ int pr_fly_async(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_
     } else {
         int rc = init_flights(c, layout, n_genomes, n_scen, p, d_fitness, d_steps, &nf);
         if (rc) return rc;
-        rc = issue_chunks(c, p, a, nf, true, &ne);
+        rc = issue_chunks(c, p, a, nf, true, &ne, false);
         if (rc) return rc;
     }
     c->pending = 1;
