@@ -27,7 +27,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 FLOP_PANEL, FLOP_PLANE = 68, 222  # algorithmic FLOPs per panel-step and per plane-step
 # HBM bytes per k_fly_all launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled
 # by a 1 GiB copy's readings) on this workload; the counters cannot be read inside a timed run
-TRAFFIC = {"c3": (4.26e8, "profiles/k_fly_all_c3_hbm_traffic.md")}
+TRAFFIC = {"c3": (8.61e8, "profiles/k_fly_all_c3_hbm_traffic.md")}  # per population launch
 
 WORKLOADS = {
     # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
@@ -50,61 +50,53 @@ def seeds_for(gen, nseeds):
     return np.array([gen * nseeds + s + 1 for s in range(nseeds)], np.int32)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(genomes, res, w, budget_s):
     """The oracle's FP32 transcription on this host's cores over a bounded sample of the
-    same workload (same genome distribution, scenario preset and cap)."""
+    same workload (same genome distribution and mesh resolutions, scenario preset and cap):
+    random (genome, seed) flights pulled from an atomic counter by all host threads, the
+    sample grown until it holds about budget_s of work; then the 1-thread rate on a
+    sample of about budget_s / 5."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
     nth = os.cpu_count() or 1
     rng = np.random.default_rng(1)
     kw = dict(prec=32, preset=1, gust=w["gust"], warp=float(np.deg2rad(w["warp_deg"])), wsd=w["wind_sd"],
-              max_steps=w["max_steps"], nthreads=nth)
-    if res is not None:  # the oracle takes one resolution per call: group the sample by (ch, sp)
-        return cpu_baseline_res(O, genomes, res, kw, budget_s)
+              max_steps=w["max_steps"])
+    rs = np.ascontiguousarray(res if res is not None else np.tile([[5, 8]], (len(genomes), 1)), np.int32)
 
-    def sample(n):
+    def run(n, threads):
         gi = rng.integers(0, len(genomes), n).astype(np.int32)
-        sd = rng.integers(1, 1 + 8 * 1000, n).astype(np.int32)
-        return gi, sd
+        gi.sort()  # a thread keeps its prepared genome while consecutive flights share it
+        sd = rng.integers(1, 1 + w["nseeds"] * 1000, n).astype(np.int32)
+        _, steps, wall = O.fly_res(genomes, rs, gi, sd, nthreads=threads, **kw)
+        return int(steps.sum()), wall
 
-    gi, sd = sample(max(64, 8 * nth))
-    _, steps, wall = O.fly(genomes, gi, sd, **kw)
-    per = wall / len(gi)
-    n = int(min(200000, max(len(gi), budget_s / max(per, 1e-6))))
-    gi, sd = sample(n)
-    _, steps, wall = O.fly(genomes, gi, sd, **kw)
-    pfs = steps.sum() / 600.0
-    return {"value": pfs / wall, "unit": "plane-flight-s/s", "cores": nth, "kind": "port",
-            "sample": f"{n} random (genome, seed) flights of the same workload, FP32 oracle "
-                      f"(g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {pfs:.0f} plane-flight-s"}
+    def sized(threads, budget):
+        n = max(64, 4 * threads)
+        while True:
+            steps, wall = run(n, threads)
+            if wall >= 0.6 * budget or n >= 4_000_000:
+                return n, steps, wall
+            n = int(n * min(8.0, budget / max(wall, 1e-3)))
 
-
-def cpu_baseline_res(O, genomes, res, kw, budget_s):
-    rng = np.random.default_rng(1)
-    nth = kw["nthreads"]
-    keys = sorted(set(map(tuple, res.tolist())))
-
-    def run(n):
-        gi = rng.integers(0, len(genomes), n)
-        steps_tot, wall_tot = 0, 0.0
-        for key in keys:
-            sel = gi[(res[gi, 0] == key[0]) & (res[gi, 1] == key[1])]
-            if len(sel) == 0:
-                continue
-            sd = rng.integers(1, 1 + 12 * 1000, len(sel)).astype(np.int32)
-            _, steps, wall = O.fly(genomes, sel.astype(np.int32), sd, ch=key[0], sp=key[1], **kw)
-            steps_tot += int(steps.sum())
-            wall_tot += wall
-        return steps_tot, wall_tot
-
-    s, wall = run(max(256, 16 * nth))
-    n = int(min(200000, max(256, budget_s / max(wall / max(256, 16 * nth), 1e-6))))
-    s, wall = run(n)
-    pfs = s / 600.0
-    return {"value": pfs / wall, "unit": "plane-flight-s/s", "cores": nth, "kind": "port",
+    n, steps, wall = sized(nth, budget_s)
+    n1, steps1, wall1 = sized(1, budget_s / 5)
+    return {"value": steps / 600.0 / wall, "unit": "plane-flight-s/s", "cores": nth, "kind": "port",
+            "single_thread_value": steps1 / 600.0 / wall1, "cpu": cpu_model(),
             "sample": f"{n} random (genome, seed) flights of the same workload (per-genome mesh resolution), FP32 "
-                      f"oracle (g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {pfs:.0f} plane-flight-s"}
+                      f"oracle (g++ -O3 -ffp-contract=off), {nth} threads, {wall:.1f} s wall, {steps / 600:.0f} "
+                      f"plane-flight-s; 1 thread: {n1} flights, {wall1:.1f} s"}
 
 
 def pool_ab(PA, args):

@@ -951,6 +951,36 @@ void or_aero_f64(int nt, const double* c, const double* A, const double* n, cons
 FLY_FN(or_fly_f32, float)
 FLY_FN(or_fly_f64, double)
 
+// The same with a mesh resolution per genome: res[2 * g] = ch, res[2 * g + 1] = sp.
+#define FLY_RES_FN(NAME, R)                                                                                  \
+    double NAME(const float* genomes, const int* res, const int* gi, const int* seeds, int n, int preset,      \
+                float gust, float warp, float wsd, int max_steps, int nthreads, R* fitness, int* steps) {      \
+        auto t0 = std::chrono::steady_clock::now();                                                         \
+        ScenarioCfg scfg = mkscfg(preset, gust, warp, wsd);                                                 \
+        SimCfg sc; if (max_steps > 0) sc.max_steps = max_steps;                                             \
+        std::atomic<int> next(0);                                                                           \
+        auto work = [&]() {                                                                                 \
+            int cached = -1; Prepared<R> p;                                                                 \
+            for (;;) {                                                                                      \
+                int i = next.fetch_add(1);                                                                  \
+                if (i >= n) break;                                                                          \
+                if (gi[i] != cached) {                                                                      \
+                    p = prepare<R>(genomes + (size_t)N_GENES * gi[i], mkres(res[2 * gi[i]], res[2 * gi[i] + 1])); \
+                    cached = gi[i];                                                                         \
+                }                                                                                           \
+                Flight<R> F = make_flight(p, seeds[i], scfg);                                               \
+                State<R> s = fly(F, 0, sc);                                                                 \
+                fitness[i] = s.fitness; steps[i] = s.steps;                                                 \
+            }                                                                                               \
+        };                                                                                                  \
+        if (nthreads <= 1) work();                                                                          \
+        else { std::vector<std::thread> th; for (int k = 0; k < nthreads; k++) th.emplace_back(work);       \
+               for (auto& x : th) x.join(); }                                                               \
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();                \
+    }
+FLY_RES_FN(or_fly_res_f32, float)
+FLY_RES_FN(or_fly_res_f64, double)
+
 // Pose recording of one flight (sim.cu:119-128): rec[rec_cap*8]; returns fitness
 float or_record_f32(const float* g, int seed, int plane_index, float spread, int max_steps, int rec_every,
                     int rec_cap, float* rec) {

@@ -43,6 +43,9 @@ def _load():
         getattr(lib, f"or_fly_{fx}").argtypes = [F32P, I32P, I32P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
                                                  C.c_float, C.c_float, C.c_int, C.c_int, P, I32P]
         getattr(lib, f"or_fly_{fx}").restype = C.c_double
+        getattr(lib, f"or_fly_res_{fx}").argtypes = [F32P, I32P, I32P, I32P, C.c_int, C.c_int, C.c_float, C.c_float,
+                                                     C.c_float, C.c_int, C.c_int, P, I32P]
+        getattr(lib, f"or_fly_res_{fx}").restype = C.c_double
     lib.or_aero_f64.argtypes = [C.c_int, F64P, F64P, F64P] + [F64P] * 5 + [C.c_double] * 4 + [C.c_int, F64P, F64P]
     lib.or_wind_f32.argtypes = [C.c_float] * 4 + [C.c_int, F32P, F32P]
     lib.or_wind_f64.argtypes = [C.c_double] * 4 + [C.c_int, F32P, F64P]
@@ -201,6 +204,22 @@ def fly(genomes, gi, seeds, prec=32, preset=0, gust=-1.0, warp=-1.0, wsd=-1.0, m
         nthreads = os.cpu_count() or 1
     wall = getattr(lib(), f"or_fly_f{prec}")(genomes, gi, seeds, n, ch, sp, preset, gust, warp, wsd, max_steps,
                                              nthreads, fit, steps)
+    return fit, steps, wall
+
+
+def fly_res(genomes, res, gi, seeds, prec=32, preset=0, gust=-1.0, warp=-1.0, wsd=-1.0, max_steps=0, nthreads=0):
+    """Batched flights with a mesh resolution per genome (res [n_genomes, 2] = ch, sp)."""
+    genomes = np.ascontiguousarray(genomes, np.float32).reshape(-1)
+    res = np.ascontiguousarray(res, np.int32).reshape(-1)
+    gi = np.ascontiguousarray(gi, np.int32)
+    seeds = np.ascontiguousarray(seeds, np.int32)
+    n = len(gi)
+    fit = np.zeros(n, _dt(prec))
+    steps = np.zeros(n, np.int32)
+    if nthreads <= 0:
+        nthreads = os.cpu_count() or 1
+    wall = getattr(lib(), f"or_fly_res_f{prec}")(genomes, res, gi, seeds, n, preset, gust, warp, wsd, max_steps,
+                                                 nthreads, fit, steps)
     return fit, steps, wall
 
 
