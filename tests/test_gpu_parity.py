@@ -411,3 +411,33 @@ def test_pipelined_generation_matches_synchronous(PA):
         assert out[g]["holdout"] == sync[g]["holdout"] and out[g]["champion"] == sync[g]["champion"], g
     with pytest.raises(PA.PlanesError):
         A.collect()  # nothing outstanding
+
+
+@pytest.mark.parametrize("res", [(12, 12), (15, 16)])
+def test_fine_meshes_largest_buckets_vs_fp64(PA, res):
+    """Fine mesh resolutions (biplanes of 1176 and 1944 triangles, monoplanes of ~600-980)
+    reach the buckets above 1024 triangles, which run outside the fused kernel with 64
+    lanes x 24 or 32 panel slots: distances against the FP64 oracle at the same resolution,
+    and prepare (triangle counts, mass) bit-exact with the FP32 oracle."""
+    ch, sp = res
+    gs = O.random_genomes(24, 41 + ch)
+    r = np.tile(np.array([[ch, sp]], np.int32), (len(gs), 1))
+    seeds = np.arange(1, 5, dtype=np.int32)
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=2400)
+    pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=ch, max_sp=sp)
+    fit, steps = pl.evaluate(gs, seeds, params, res=r)
+    mass, nt, nv, off = pl.read_prepared()
+    for i in range(len(gs)):
+        m_ref, printable, nt_ref, nv_ref = O.mass(gs[i], ch=ch, sp=sp)
+        assert np.array_equal(mass[i], m_ref) and nt[i] == (nt_ref if printable else 0), i
+    assert nt.max() > 1024  # the own-launch buckets are exercised
+    gi = np.repeat(np.arange(len(gs)), 4).astype(np.int32)
+    f64, s64, _ = O.fly(gs, gi, np.tile(seeds, len(gs)), prec=64, preset=1, gust=2.0,
+                        warp=float(params.warp_max), wsd=2.0, max_steps=2400, ch=ch, sp=sp)
+    fit = fit.reshape(-1)
+    rel = np.abs(fit - f64) / np.maximum(np.abs(f64), 1e-3)
+    st = pl.stats()
+    own = [b for b in st["buckets"] if not b["fused"] and b["flight_steps"] > 0]
+    print(f"res {res}: nt {nt.min()}..{nt.max()}, {np.mean(rel < 0.01) * 100:.1f}% within 1% of FP64, "
+          f"own-launch mappings {[(b['L'], b['P']) for b in own]}")
+    assert np.mean(rel < 0.01) >= 0.95 and own
