@@ -100,6 +100,14 @@ int pr_fly_wait(pr_ctx* ctx);
 int pr_init_flights(pr_ctx* ctx, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
                     int* d_steps);
 
+/* Replaces gpu_sim (sim.h:12, sim.cu:238-248) one for one: flies the prepared genomes
+ * from seed (vary = 0: every plane the same scenario; 1: plane p flies seed + p * 7919,
+ * 32-bit wrap), launch-grid offset spread, pose recording into d_rec [pop*rec_cap*8]
+ * (null: none) every rec_every >= 1 steps; d_fitness device [pop]; scenario preset and
+ * flight cap from p. Synchronous, like the reference. */
+int pr_sim(pr_ctx* ctx, int pop, int seed, float* d_fitness, float* d_rec, int rec_every, int rec_cap, float spread,
+           int vary, const pr_sim_params* p);
+
 /* pr_fly with pose recording (gpu_sim's d_rec path, sim.cu:117-128): d_rec device
  * [n_flights * rec_cap * 8] floats, frame k of flight f at (f*rec_cap + k)*8 =
  * {pos.xyz, q.wxyz, alive}; frames every rec_every steps; a flight that has ended keeps

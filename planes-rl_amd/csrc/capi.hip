@@ -66,6 +66,7 @@ struct pr_ctx {
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
     unsigned long long* d_cstats = nullptr;  // [PR_MAX_CHUNKS][2] per-chunk flight-/panel-steps
+    int* d_steps_own = nullptr;              // step counts for callers that do not ask for them (pr_sim)
     int* d_chunk_n = nullptr;                // [PR_MAX_CHUNKS][NB_MAX] flights entering each chunk
     std::vector<Bucket> buckets;
     std::vector<int> grid;    // persistent grid per bucket
@@ -203,6 +204,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_seeds, sizeof(int) * max_scenarios);
     ALLOC(c->d_scen, sizeof(float) * NSF * max_scenarios);
     ALLOC(c->d_state, sizeof(FlightState) * max_flights);
+    ALLOC(c->d_steps_own, sizeof(int) * max_flights);
     ALLOC(c->d_fc, sizeof(FlightConst) * max_flights);
     for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
     ALLOC(c->d_counts, sizeof(int) * PR_CNT * NB_MAX);
@@ -228,7 +230,7 @@ void pr_destroy(pr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
                     c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp,
-                    c->d_cstats, c->d_chunk_n};
+                    c->d_cstats, c->d_chunk_n, c->d_steps_own};
     for (void* b : bufs) if (b) (void)hipFree(b);
     for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -597,6 +599,30 @@ int pr_fly_record(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim
     HCK(launch_fly_rec(b.L, b.P, a, (int)nf, d_rec, rec_every, rec_cap, c->stream));
     HCK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// gpu_sim (sim.h:12, sim.cu:238-248) as a C entry point: flies the prepared genomes from
+// seed `seed` (vary = 0: every plane the same scenario; 1: plane p flies seed + p * 7919),
+// launch grid offset `spread`, optional pose recording into d_rec [pop * rec_cap * 8]
+// every rec_every steps (>= 1 as in the reference, whose step0 / rec_every runs always).
+int pr_sim(pr_ctx* c, int pop, int seed, float* d_fitness, float* d_rec, int rec_every, int rec_cap, float spread,
+           int vary, const pr_sim_params* p) {
+    ARG(c && d_fitness && p, "pr_sim: null argument");
+    ARG(rec_every >= 1, "pr_sim: rec_every must be >= 1");
+    ARG(pop > 0 && pop <= c->n_prepared, "pr_sim: pop exceeds the prepared genomes");
+    const int ns = vary ? pop : 1;
+    ARG(ns <= c->cap_scen, "pr_sim: vary = 1 needs max_scenarios >= pop");
+    std::vector<int> seeds(ns);
+    for (int i = 0; i < ns; i++) seeds[i] = (int)((unsigned)seed + (unsigned)i * 7919u);  // 32-bit wrap
+    HCK(hipMemcpyAsync(c->d_seeds, seeds.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+    ScenarioParams sp;
+    sp.preset = p->preset; sp.gust_rms_max = p->gust_rms_max; sp.warp_max = p->warp_max; sp.wind_sd = p->wind_sd;
+    HCK(launch_scenarios(c->d_seeds, ns, sp, c->d_scen, c->stream));
+    c->n_scen = ns;
+    pr_sim_params q = *p;
+    q.spread = spread;
+    if (d_rec) return pr_fly_record(c, vary ? 1 : 0, pop, ns, &q, d_fitness, c->d_steps_own, d_rec, rec_every, rec_cap);
+    return pr_fly(c, vary ? 1 : 0, pop, ns, &q, d_fitness, c->d_steps_own);
 }
 
 int pr_read_extent(pr_ctx* c, int n, float* ext3) {

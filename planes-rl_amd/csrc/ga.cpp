@@ -650,15 +650,12 @@ void select_finalist(const Genome* candidates, int count, int scenarios, int bas
 struct RefPanelBuf {
     pr_ctx* ctx;
     int pop;
-    DevBuf seeds, steps;
 };
 
 void* panelbuf_alloc(int pop) {
     RefPanelBuf* pb = new RefPanelBuf;
     pb->pop = pop;
     CKP(pr_create(&pb->ctx, pop, 0, pop, pop, 0, 0));
-    pb->seeds.ensure(sizeof(int) * pop);
-    pb->steps.ensure(sizeof(int) * pop);
     return pb;
 }
 
@@ -678,15 +675,5 @@ void gpu_sim(void* h, int pop, int seed, float* d_fitness, float8* d_rec, int re
              int vary_scenarios) {
     RefPanelBuf* pb = (RefPanelBuf*)h;
     pr_sim_params prm = reference_params();
-    prm.spread = spread;
-    std::vector<int> seeds(vary_scenarios ? pop : 1);
-    for (size_t p = 0; p < seeds.size(); p++) seeds[p] = (int)((unsigned)seed + (unsigned)p * 7919u);
-    CKP(pr_h2d(pb->seeds.p, seeds.data(), sizeof(int) * seeds.size()));
-    CKP(pr_scenarios(pb->ctx, (const int*)pb->seeds.p, (int)seeds.size(), &prm));
-    if (d_rec) {
-        CKP(pr_fly_record(pb->ctx, vary_scenarios ? 1 : 0, pop, (int)seeds.size(), &prm, d_fitness,
-                          (int*)pb->steps.p, (float*)d_rec, rec_every, rec_cap));
-    } else {
-        CKP(pr_fly(pb->ctx, vary_scenarios ? 1 : 0, pop, (int)seeds.size(), &prm, d_fitness, (int*)pb->steps.p));
-    }
+    CKP(pr_sim(pb->ctx, pop, seed, d_fitness, (float*)d_rec, rec_every, rec_cap, spread, vary_scenarios, &prm));
 }

@@ -107,6 +107,7 @@ def _load():
     lib.pr_set_graph.argtypes = [VP, C.c_int]
     lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
     lib.pr_set_fused.argtypes = [VP, C.c_int]
+    lib.pr_sim.argtypes = [VP, C.c_int, C.c_int, VP, VP, C.c_int, C.c_int, C.c_float, C.c_int, C.POINTER(SimParams)]
     lib.pr_design_stats.argtypes = [F32P, C.c_int, F32P]
     lib.pr_audit.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, F32P, C.c_int]
     lib.pr_gen_context.argtypes = [VP, C.c_int]
@@ -286,6 +287,16 @@ class Planes:
         params = params or SimParams.reference()
         nf = self.fly_device(layout, n_genomes, n_scen, params)
         return self.d_fit.download(np.float32, nf), self.d_steps.download(np.int32, nf)
+
+    def sim(self, pop, seed, params=None, spread=0.0, vary=0, rec_every=1, rec_cap=0):
+        """gpu_sim (sim.h:12) on the prepared genomes: fitness[pop] and, with rec_cap > 0,
+        poses [pop, rec_cap, 8] (pos, q, alive)."""
+        params = params or SimParams.reference()
+        rec = DeviceBuffer(4 * 8 * pop * rec_cap) if rec_cap > 0 else None
+        _ck(LIB.pr_sim(self.h, pop, seed, self.d_fit.ptr, rec.ptr if rec else None, rec_every, rec_cap, spread, vary,
+                       C.byref(params)), "pr_sim")
+        fit = self.d_fit.download(np.float32, pop)
+        return (fit, rec.download(np.float32, 8 * pop * rec_cap).reshape(pop, rec_cap, 8)) if rec else fit
 
     def fly_record(self, layout, n_genomes, n_scen, params, rec_every, rec_cap):
         """gpu_sim with d_rec (sim.cu:117-128): returns fitness, steps, frames[n, rec_cap, 8]."""

@@ -441,3 +441,27 @@ def test_fine_meshes_largest_buckets_vs_fp64(PA, res):
     print(f"res {res}: nt {nt.min()}..{nt.max()}, {np.mean(rel < 0.01) * 100:.1f}% within 1% of FP64, "
           f"own-launch mappings {[(b['L'], b['P']) for b in own]}")
     assert np.mean(rel < 0.01) >= 0.95 and own
+
+
+def test_pr_sim_is_gpu_sim(PA):
+    """pr_sim, the C entry point behind gpu_sim (sim.h:12): vary = 0 flies every plane under
+    the one seed, vary = 1 plane p under seed + p * 7919 (32-bit wrap), identical to the
+    batched layouts; with recording, the poses' last alive frame matches the flight."""
+    gs = O.random_genomes(40, 61)
+    params = PA.SimParams.reference(max_steps=1800)
+    pl = PA.Planes(40, 40, 40)
+    pl.prepare(gs)
+    f0 = pl.sim(40, 77, params)
+    ref0, _ = PA.Planes(40, 1, 40).evaluate(gs, np.array([77], np.int32), params)
+    assert np.array_equal(f0, ref0.reshape(-1))
+    seed = 2 ** 31 - 5000  # the vary rule wraps past int32
+    f1 = pl.sim(40, seed, params, vary=1)
+    seeds = ((seed + 7919 * np.arange(40, dtype=np.int64) + 2 ** 31) % 2 ** 32 - 2 ** 31).astype(np.int32)
+    ref1, _ = PA.Planes(40, 40, 40).evaluate(gs, seeds, params, layout=1)
+    assert np.array_equal(f1, ref1.reshape(-1))
+    f2, rec = pl.sim(40, 77, params, rec_every=60, rec_cap=30)
+    assert np.allclose(f2, f0, rtol=1e-5, atol=1e-5) and rec.shape == (40, 30, 8)
+    alive = rec[:, :, 7]
+    assert np.all(np.diff(alive, axis=1) <= 0)  # once ended, a flight stays ended
+    with pytest.raises(PA.PlanesError):
+        pl.sim(40, 1, params, rec_every=0)
