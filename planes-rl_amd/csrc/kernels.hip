@@ -437,7 +437,7 @@ struct Flyer {
 #pragma unroll
         for (int m = 0; m < WindLane<L>::NT; m++) {
             float arg = W.ax[m] * B.px + W.ay[m] * B.py + W.az[m] * B.pz + W.bt[m] * B.t + W.cs[m];
-            float sv = W.wg[m] * sin_fast(arg);
+            float sv = W.wg[m] * sin_gust(arg);
             gx += W.comp[m] == 0 ? sv : 0.0f;
             gy += W.comp[m] == 1 ? sv : 0.0f;
             gz += W.comp[m] == 2 ? sv : 0.0f;
