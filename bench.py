@@ -114,7 +114,7 @@ def pool_ab(PA, args):
         ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
                        sigma_frac=0.05, seed=7)
         ga.set_elite(g0)
-        S = PA.Surrogate(max_batch=4096, seed=3) if mode == "surrogate" else None
+        S = PA.Surrogate(max_batch=args.mlp_batch, seed=3) if mode == "surrogate" else None
         gens = []
         if S is not None:
             ga.set_explore(args.explore)
@@ -240,6 +240,7 @@ def main():
     ap.add_argument("--boot-epochs", type=int, default=16, help="c4: surrogate epochs on the bootstrap generation")
     ap.add_argument("--train-epochs", type=int, default=4, help="c4: surrogate epochs per generation")
     ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
+    ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",

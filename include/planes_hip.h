@@ -219,6 +219,8 @@ int pr_mlp_set_params(pr_mlp* m, const float* in, int n);
 int pr_mlp_predict(pr_mlp* m, const float* d_genomes, int n, float* d_pred);
 /* mean loss + gradient of one batch (n <= max_batch); grads host, may be null */
 int pr_mlp_loss_grad(pr_mlp* m, const float* d_genomes, const float* d_targets, int n, float* loss, float* grads);
+/* Largest batch of the network (rounded up to 32); pr_pool trains with batches of this size. */
+int pr_mlp_max_batch(pr_mlp* m);
 /* Adam over contiguous mini-batches of (d_genomes [n x 35], d_targets [n x 4]) */
 int pr_mlp_train(pr_mlp* m, const float* d_genomes, const float* d_targets, int n, int epochs, int batch, float lr,
                  float* last_loss);

@@ -83,8 +83,9 @@ __global__ __launch_bounds__(64) void gemm_bf16(const __bf16* __restrict__ A, in
 
 // genomes [n x 35] f32 -> X [n_pad x 64] bf16, genes scaled to [0, 1] by the bounds
 __global__ void k_embed(const float* __restrict__ g, int n, int n_pad, const float* __restrict__ lo,
-                        const float* __restrict__ inv_range, __bf16* __restrict__ X) {
+                        const float* __restrict__ inv_range, __bf16* __restrict__ X, float* __restrict__ loss) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (loss && i == 0) *loss = 0.0f;  // the batch's loss accumulator (k_loss adds to it)
     if (i >= n_pad * K0) return;
     int r = i / K0, k = i - r * K0;
     float v = 0.0f;
@@ -166,35 +167,30 @@ __global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int cou
     out[i] = s;
 }
 
+// Adam step on the FP32 master parameters, writing their BF16 copy too. Padding parameters
+// (W1 rows >= 35, W4 / b4 columns >= 4) take no step, so they stay zero.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+                       __bf16* __restrict__ wb, int n, int off_w1, int off_w4, int off_b4, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const bool pad = (i >= off_w1 + NIN * HID && i < off_w1 + K0 * HID) ||
+                     (i >= off_w4 && i < off_w4 + HID * NOP && (i - off_w4) % NOP >= NOUT) ||
+                     (i >= off_b4 && i < off_b4 + NOP && i - off_b4 >= NOUT);
+    if (pad) return;
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
-}
-
-// gradients of the padding (W1 rows >= 35, W4/b4 columns >= 4) are zeroed so they stay 0
-__global__ void k_zero_pad_grads(float* g, int off_w1, int off_w4, int off_b4) {
-    for (int i = threadIdx.x; i < (K0 - NIN) * HID; i += blockDim.x) g[off_w1 + NIN * HID + i] = 0.0f;
-    for (int i = threadIdx.x; i < HID * NOP; i += blockDim.x)
-        if (i % NOP >= NOUT) g[off_w4 + i] = 0.0f;
-    for (int i = threadIdx.x; i < NOP; i += blockDim.x)
-        if (i >= NOUT) g[off_b4 + i] = 0.0f;
+    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    wb[i] = to_bf(wi);
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict__ y) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = to_bf(x[i]);
-}
-
-__global__ void k_zero_f(float* p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
 }
 
 // robust score from the three regression heads; -100 when the oversize head says so
@@ -209,6 +205,7 @@ __global__ void k_head_scores(const float* __restrict__ out, int n, float* __res
     pred[(size_t)r * 5 + 3] = 1.0f / (1.0f + expf(-o[3]));
     pred[(size_t)r * 5 + 4] = o[3] > 0.0f ? -100.0f : s;
 }
+
 
 thread_local std::string g_mlp_err;
 
@@ -281,12 +278,11 @@ static int forward(pr_mlp* q, int B, bool keep_out) {
 static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, float* loss_out) {
     const int B = pad32(n);
     hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes, n, B, q->lo, q->inv_range,
-                       q->X);
+                       q->X, q->loss);
     MCK(hipGetLastError());
     int rc = forward(q, B, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_zero_f, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->g, (int)q->n_params);
-    hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, q->s, q->loss, 1);
+    // every gradient entry is written below (weights by k_sum_splits, biases by k_colsum_fin)
     hipLaunchKernelGGL(k_loss, dim3((B + 63) / 64), dim3(64), 0, q->s, q->out, d_targets, n, B, 1.0f / n, q->dout,
                        q->dout_f, q->loss);
     MCK(hipGetLastError());
@@ -414,6 +410,8 @@ void pr_mlp_destroy(pr_mlp* q) {
     delete q;
 }
 
+int pr_mlp_max_batch(pr_mlp* q) { return q ? q->max_batch : 0; }
+
 int pr_mlp_get_params(pr_mlp* q, float* out, int cap) {
     if (!q || cap < (int)q->n_params) { g_mlp_err = "pr_mlp_get_params: capacity"; return -1; }
     MCK(hipStreamSynchronize(q->s));
@@ -438,7 +436,7 @@ int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
         const int nb = std::min(q->max_batch, n - s0);
         const int B = pad32(nb);
         hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes + (size_t)s0 * NIN, nb,
-                           B, q->lo, q->inv_range, q->X);
+                           B, q->lo, q->inv_range, q->X, nullptr);
         MCK(hipGetLastError());
         int rc = forward(q, B, false);
         if (rc) return rc;
@@ -473,17 +471,13 @@ int pr_mlp_train(pr_mlp* q, const float* d_genomes, const float* d_targets, int 
             const int nb = std::min(batch, n - s0) / 32 * 32;
             int rc = loss_grad(q, d_genomes + (size_t)s0 * NIN, d_targets + (size_t)s0 * NOUT, nb, nullptr);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_zero_pad_grads, dim3(1), dim3(256), 0, q->s, q->g, (int)q->off_w[0], (int)q->off_w[3],
-                               (int)q->off_b[3]);
-            MCK(hipGetLastError());
             q->step++;
             const float c1 = 1.0f / (1.0f - std::pow(b1, (float)q->step));
             const float c2 = 1.0f / (1.0f - std::pow(b2, (float)q->step));
             hipLaunchKernelGGL(k_adam, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->w, q->g, q->m,
-                               q->v, (int)q->n_params, lr, b1, b2, eps, c1, c2);
+                               q->v, q->wb, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3], (int)q->off_b[3], lr,
+                               b1, b2, eps, c1, c2);
             MCK(hipGetLastError());
-            rc = refresh_bf16(q);
-            if (rc) return rc;
         }
     }
     if (last_loss) {

@@ -334,7 +334,8 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         PCK(hipGetLastError());
         int rc = buffer_add(p, fly, p->d_tgt, nf);
         if (rc) return rc;
-        if (model && train_epochs > 0) PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, train_epochs, 4096, lr, &loss));
+        if (model && train_epochs > 0)
+            PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, train_epochs, pr_mlp_max_batch(model), lr, &loss));
         PCK(hipDeviceSynchronize());
         mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
     }
@@ -383,7 +384,7 @@ int pr_pool_set_collect(pr_pool* p, int on) {
 
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss) {
     if (!p || !model || p->buf_n < 32) { g_pool_err = "pr_pool_train: model and >= 32 buffered pairs needed"; return -1; }
-    PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, epochs, 4096, lr, loss));
+    PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, epochs, pr_mlp_max_batch(model), lr, loss));
     return 0;
 }
 
