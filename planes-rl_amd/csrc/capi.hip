@@ -2,9 +2,10 @@
 //
 // One pr_ctx owns the HBM buffers of one evaluation slot (the reference's PanelBuf,
 // sim.cu:15-21, without its MAX_T padding) and one HIP stream.  pr_fly schedules the
-// flights as 1-s chunks: each chunk launches the persistent flight kernel once per
-// triangle-count bucket over that bucket's work list; survivors are appended to the
-// next chunk's list by the kernel itself, so the host never synchronises between chunks.
+// flights as 1-s chunks: each chunk flies every triangle-count bucket's work list with
+// the persistent flight kernel, which marks the flights that go on; a stable compaction
+// turns the marks into the next chunk's lists on the device, so the host never
+// synchronises between chunks and the lists keep a data-determined order.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -62,7 +63,9 @@ struct pr_ctx {
     FlightState* d_state = nullptr;
     FlightConst* d_fc = nullptr;
     int* d_lists[2] = {};
-    int* d_counts = nullptr;  // PR_CNT per bucket: 3 rotating list counts, 2 rotating queue heads
+    int* d_counts = nullptr;  // PR_CNT per bucket: 2 alternating list counts, 2 claim counters
+    int* d_tcount = nullptr;  // [NB_MAX][max_tiles] survivors per compaction tile
+    int max_tiles = 0;
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
     unsigned long long* d_cstats = nullptr;  // [PR_MAX_CHUNKS][2] per-chunk flight-/panel-steps
@@ -208,6 +211,8 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_fc, sizeof(FlightConst) * max_flights);
     for (int k = 0; k < 2; k++) ALLOC(c->d_lists[k], sizeof(int) * (size_t)NB_MAX * max_flights);
     ALLOC(c->d_counts, sizeof(int) * PR_CNT * NB_MAX);
+    c->max_tiles = (int)((max_flights + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE);
+    ALLOC(c->d_tcount, sizeof(int) * NB_MAX * (size_t)c->max_tiles);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
     ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
     ALLOC(c->d_cstats, sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
@@ -230,7 +235,7 @@ void pr_destroy(pr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
                     c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp,
-                    c->d_cstats, c->d_chunk_n, c->d_steps_own};
+                    c->d_cstats, c->d_chunk_n, c->d_steps_own, c->d_tcount};
     for (void* b : bufs) if (b) (void)hipFree(b);
     for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -330,8 +335,20 @@ static void fill_fly_args(pr_ctx* c, const pr_sim_params* p, float* d_fitness, i
     a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
     a.state = c->d_state; a.fitness = d_fitness; a.steps = d_steps;
     a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
-    a.list_in = nullptr; a.count_in = nullptr; a.list_out = nullptr; a.count_out = nullptr; a.head = nullptr;
+    a.list_in = nullptr; a.count_in = nullptr; a.marks = nullptr; a.head = nullptr;
     a.run_lo = 0; a.run_hi = 0x7fffffff;
+}
+
+static CompactArgs compact_args(pr_ctx* c, int NB) {
+    CompactArgs ca = {};
+    ca.dst = c->d_lists[0];
+    ca.dst_stride = c->cap_flights;
+    ca.counts = c->d_counts;
+    ca.tcount = c->d_tcount;
+    ca.max_tiles = c->max_tiles;
+    ca.nb = NB;
+    ca.chunk_n = nullptr;
+    return ca;
 }
 
 static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_params* p, float* d_fitness,
@@ -352,9 +369,13 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     QueueArgs q;
     q.n_flights = (int)nf; q.layout = layout; q.nseeds = n_scen; q.nt = c->d_nt;
     q.mass = c->d_mass; q.scen = c->d_scen; q.seeds = c->d_seeds; q.spread = p->spread; q.fc = c->d_fc;
-    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.count = c->d_counts; q.count_stride = PR_CNT; q.list = c->d_lists[0];
-    q.list_stride = c->cap_flights; q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
+    q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.bucket = c->d_lists[1];
+    q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
     HCK(launch_queue(q, c->stream));
+    // chunk 0's lists: each bucket's flights in flight order (count slot 0)
+    CompactArgs ca = compact_args(c, NB);
+    ca.src = c->d_lists[1]; ca.src_stride = 0; ca.by_bucket = 1; ca.n_fixed = (int)nf; ca.c_dst = 0;
+    HCK(launch_compact(ca, nf * NB, c->stream));
     *nf_out = nf;
     return 0;
 }
@@ -369,10 +390,10 @@ int pr_init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_s
 }
 
 // Issue the flight sequence after k_init on the context stream, one 1-s chunk at a time:
-// the fused kernel flies every bucket compiled into it (one launch per chunk, which also
-// clears the next chunk's counters); a bucket outside it gets its own k_fly launches.
-// Chunk ck: list_in = lists[ck & 1], counts: in = ck % 3, out = (ck + 1) % 3, cleared =
-// (ck + 2) % 3; queue head ck & 1, cleared (ck + 1) & 1 (k_init fills list 0 / count 0).
+// the fused kernel flies every bucket compiled into it (one launch per chunk); a bucket
+// outside it gets its own k_fly launches; then one compaction builds every bucket's list
+// for the next chunk from the marks (list order kept).  Chunk ck reads count slot ck & 1
+// and its compaction writes slot (ck + 1) & 1.
 static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long nf, bool timed, int* n_events,
                         bool capturing) {
     const int NB = (int)c->buckets.size();
@@ -398,8 +419,8 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
     }
     FlyAllArgs F;
     F.base = a;
-    F.lists0 = c->d_lists[0];
-    F.lists1 = c->d_lists[1];
+    F.lists = c->d_lists[0];
+    F.marks = c->d_lists[1];
     F.list_stride = c->cap_flights;
     F.counts = c->d_counts;
     F.nb = NB;
@@ -409,31 +430,28 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
         F.vid_w[b] = b < NB ? c->vid_w[b] : -1;
         F.thr[b] = grid;  // one flight per wave once they fit one per wave
     }
+    CompactArgs ca = compact_args(c, NB);
+    ca.src = c->d_lists[1];
+    ca.src_stride = c->cap_flights;
+    ca.by_bucket = 0;
     int ne = 0;
     for (int ck = 0; ck < n_chunks; ck++) {
         const int cs = ck < PR_MAX_CHUNKS ? ck : PR_MAX_CHUNKS - 1;  // later chunks share the last slot
-        F.in_list = ck & 1;
-        F.c_in = ck % 3; F.c_out = (ck + 1) % 3; F.c_zero = (ck + 2) % 3;
-        F.h_run = ck & 1; F.h_zero = (ck + 1) & 1;
+        F.c_in = ck & 1;
         F.base.cstats = a.cstats = c->d_cstats + 2 * cs;
-        F.chunk_n = c->d_chunk_n + NB_MAX * cs;
         if (any_fused) {
             const bool tev = timed && ne + 2 <= n_ev;
             if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
             HCK(launch_fly_all(F, grid, c->stream));
             if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; c->ev_chunk[ne / 2] = cs; ne += 2; }
-        } else {
-            HCK(launch_zero_chunk(c->d_counts, NB, F.c_in, F.c_zero, F.h_zero, F.chunk_n, c->stream));
         }
-        if (!any_own) continue;
-        for (int b = 0; b < NB; b++) {
+        for (int b = 0; any_own && b < NB; b++) {
             if (!own_used[b]) continue;
             int* cb = c->d_counts + PR_CNT * b;
-            a.list_in = c->d_lists[ck & 1] + (size_t)b * c->cap_flights;
-            a.list_out = c->d_lists[(ck & 1) ^ 1] + (size_t)b * c->cap_flights;
+            a.list_in = c->d_lists[0] + (size_t)b * c->cap_flights;
+            a.marks = c->d_lists[1] + (size_t)b * c->cap_flights;
             a.count_in = cb + F.c_in;
-            a.count_out = cb + F.c_out;
-            a.head = cb + 3 + F.h_run;
+            a.head = cb + 2;
             const Bucket& bk = c->buckets[b];
             for (int w = 0; w < (bk.Lw ? 2 : 1); w++) {
                 // w = 0: many flights per wave while more than grid_w flights remain;
@@ -453,6 +471,10 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
                 }
             }
         }
+        ca.c_src = ck & 1;
+        ca.c_dst = (ck + 1) & 1;
+        ca.chunk_n = c->d_chunk_n + NB_MAX * cs;
+        HCK(launch_compact(ca, nf, c->stream));
     }
     *n_events = ne;
     return 0;

@@ -5,12 +5,17 @@
 //                check, mass properties in the reference's float summation order,
 //                packed SoA panel records at exact prefix offsets (no MAX_T padding)
 //   k_scenarios  scenario descriptors per seed (reference or config preset)
-//   k_init       (genome, scenario) flights -> constants, launch state, per-bucket work
-//                lists; unprintable -> -100
+//   k_init       (genome, scenario) flights -> constants, launch state, bucket of each
+//                flight; unprintable -> -100
 //   k_fly<L,P>   persistent flight kernel: each group of L lanes flies one flight with
 //                P panels per lane held in VGPRs; groups refill from the work list
+//   k_fly_all    one chunk of every bucket in one launch (the variants of PR_FUSED)
+//   k_compact_*  stable compaction: per-bucket work lists from k_init's buckets and from
+//                each chunk's marks, in a data-determined order
 //   k_scores     tail-aware robust score per genome
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "pr_kernels.h"
 #include "pr_physics.h"
@@ -235,11 +240,10 @@ __global__ void k_scenarios(const int* __restrict__ seeds, int n, ScenarioParams
 // ------------------------------------------------------------------ k_init
 // Flight f -> (genome, scenario) by layout.  Writes the flight's constants (printed mass,
 // scenario terms) and launch state (sim.cu:63-109), or fitness -100 for an unprintable
-// genome (sim.cu:35-36,109), and appends the flight to the work list of its genome's
-// triangle-count bucket with wave-aggregated atomics (lane order kept within a wave).
+// genome (sim.cu:35-36,109), and the triangle-count bucket of its genome (-1 when the
+// flight ends at launch), from which k_compact_* builds the first chunk's lists.
 __global__ void k_init(QueueArgs a) {
-    int f = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
     int b = -1;
     if (f < a.n_flights) {
         const int gi = a.layout == 0 ? f / a.nseeds : f;
@@ -275,18 +279,7 @@ __global__ void k_init(QueueArgs a) {
         }
         a.fc[f] = fc;
         a.state[f] = st;
-    }
-    for (int k = 0; k < a.n_buckets; k++) {
-        unsigned long long m = __ballot(b == k);
-        if (m == 0) continue;
-        int leader = __ffsll((long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(a.count + k * a.count_stride, __popcll(m));
-        base = __shfl(base, leader);
-        if (b == k) {
-            int rank = __popcll(m & ((1ull << lane) - 1ull));
-            a.list[(size_t)k * a.list_stride + base + rank] = f;
-        }
+        a.bucket[f] = b;
     }
 }
 
@@ -521,13 +514,15 @@ struct Flyer {
     }
 };
 
-// Persistent loop over one work list: groups of L lanes take flights with one atomic each,
-// fly them to the end of the chunk (or of the flight), store the state and append the
-// survivors to the next chunk's list.  A group stops taking flights once the queue head
-// has reached `limit` (<= n_in; the rest is left to a wider mapping); the function returns
-// when every group of the wave has stopped.
+// Persistent loop over one work list: groups of L lanes take flights with one atomic each
+// and fly them to the end of the chunk (or of the flight); the function returns when every
+// group of the wave has stopped.
+// Flies list positions [lo, hi) of one bucket's chunk: each group claims the next position
+// from `head` (positions lo + 0, 1, 2, ...), and at the chunk's end marks it with the flight
+// id if the flight goes on, else -1.  Which code path flies a position depends only on the
+// position, so the results do not depend on the order in which groups claim.
 template <int L, int P>
-__device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in, int limit) {
+__device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head) {
     const int lane = threadIdx.x & 63;
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
@@ -535,21 +530,21 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in, int limit) 
     bool active = false, exhausted = false;
     F.B.nt = 0;
     F.np = 0;
-    int npw = 0;
+    int npw = 0, pos = 0;
 
     for (;;) {
         const bool need = !active && !exhausted;
         if (__ballot(need)) {
             if (need) {
-                int k = 0;
+                int k = hi;
                 if (j == 0) {
-                    // check before claiming: a claim that races past `limit` is still flown here
-                    const int h = __atomic_load_n(a.head, __ATOMIC_RELAXED);
-                    k = h < limit ? atomicAdd(a.head, 1) : n_in;
+                    // look before claiming: once the range is spent, no more atomics
+                    const int h = __atomic_load_n(head, __ATOMIC_RELAXED);
+                    if (lo + h < hi) k = lo + atomicAdd(head, 1);
                 }
                 k = __shfl(k, gbase);
-                if (k >= n_in) exhausted = true;
-                else { F.setup(a, a.list_in[k], j); active = true; }
+                if (k >= hi) exhausted = true;
+                else { pos = k; F.setup(a, a.list_in[k], j); active = true; }
             }
             npw = wave_slots<P>(active ? F.np : 0);
         }
@@ -572,10 +567,8 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int n_in, int limit) 
                 if (fin) {
                     a.fitness[F.B.fid] = st.fitness;
                     a.steps[F.B.fid] = F.B.step;
-                } else {
-                    int pos = atomicAdd(a.count_out, 1);
-                    a.list_out[pos] = F.B.fid;
                 }
+                a.marks[pos] = fin ? -1 : F.B.fid;
             }
             active = false;
         }
@@ -588,7 +581,7 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
     // flight count decides which one works, so the choice needs no host synchronisation
     if (n_in < a.run_lo || n_in > a.run_hi) return;
-    fly_list<L, P>(a, n_in, n_in);
+    fly_list<L, P>(a, 0, n_in, a.head);
 }
 
 // Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
@@ -596,16 +589,11 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     X(0, 8, 12) X(1, 8, 24) X(2, 16, 12) X(3, 16, 24) X(4, 32, 12) X(5, 32, 24) X(6, 64, 8) X(7, 64, 16)
 
 // One chunk of every bucket in one launch: each wave drains the buckets' lists, largest
-// triangle counts first, flying each bucket with its own (L, P) code path: the
-// many-flights-per-wave mapping until no more than thr[b] flights (one per wave) are left
-// in the queue, then the one-flight-per-wave mapping for those -- the last flights of a
-// chunk run at the lowest step latency instead of leaving most lanes of a partial round idle.
+// triangle counts first, flying each bucket with its own (L, P) code path: list positions
+// [0, n - thr[b]) with the many-flights-per-wave mapping, the last thr[b] positions (one
+// flight per wave) with the one-flight-per-wave mapping -- the last flights of a chunk run
+// at the lowest step latency instead of leaving most lanes of a partial round idle.
 __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
-    if (blockIdx.x == 0 && threadIdx.x < A.nb) {
-        A.counts[PR_CNT * threadIdx.x + A.c_zero] = 0;
-        A.counts[PR_CNT * threadIdx.x + 3 + A.h_zero] = 0;
-        if (A.chunk_n) A.chunk_n[threadIdx.x] = A.counts[PR_CNT * threadIdx.x + A.c_in];
-    }
     // largest triangle counts first: the longest chunk jobs start first, small ones fill in
     for (int b = A.nb - 1; b >= 0; b--) {
         if (A.vid_p[b] < 0) continue;
@@ -613,26 +601,24 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         const int n_in = cb[A.c_in];
         if (n_in == 0) continue;
         FlyArgs a = A.base;
-        a.list_in = (A.in_list ? A.lists1 : A.lists0) + (size_t)b * A.list_stride;
-        a.list_out = (A.in_list ? A.lists0 : A.lists1) + (size_t)b * A.list_stride;
+        a.list_in = A.lists + (size_t)b * A.list_stride;
+        a.marks = A.marks + (size_t)b * A.list_stride;
         a.count_in = cb + A.c_in;
-        a.count_out = cb + A.c_out;
-        a.head = cb + 3 + A.h_run;
         const int vw = A.vid_w[b];
-        const int limit = vw >= 0 ? n_in - A.thr[b] : n_in;
+        const int limit = vw >= 0 ? max(n_in - A.thr[b], 0) : n_in;
         if (limit > 0) {
             a.stats = A.stats + 2 * (2 * b);
             switch (A.vid_p[b]) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in, limit); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 2); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
             }
         }
-        if (vw >= 0) {
+        if (vw >= 0 && limit < n_in) {
             a.stats = A.stats + 2 * (2 * b + 1);
             switch (vw) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, n_in, n_in); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, limit, n_in, cb + 3); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
@@ -641,18 +627,129 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     }
 }
 
-// Counter clearing for a chunk that has no fused launch (every bucket on its own k_fly).
-__global__ void k_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n) {
-    const int t = threadIdx.x;
-    if (t < nb) {
-        counts[PR_CNT * t + c_zero] = 0;
-        counts[PR_CNT * t + 3 + h_zero] = 0;
-        if (chunk_n) chunk_n[t] = counts[PR_CNT * t + c_in];
+// ------------------------------------------------------------------ k_compact_* (list upkeep)
+// Tiles of PR_COMPACT_TILE source entries, numbered across the buckets in bucket order;
+// every block works out the numbering from the counters, then strides over the tiles.
+struct TileMap {
+    int n[8], t0[9];
+    __device__ void load(const CompactArgs& A) {
+        t0[0] = 0;
+        for (int b = 0; b < A.nb; b++) {
+            n[b] = A.by_bucket ? A.n_fixed : A.counts[PR_CNT * b + A.c_src];
+            t0[b + 1] = t0[b] + (n[b] + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE;
+        }
+    }
+    __device__ int bucket_of(int g, int nb) const {
+        int b = 0;
+        while (b < nb - 1 && g >= t0[b + 1]) b++;
+        return b;
+    }
+};
+
+constexpr int CT_THREADS = 256, CT_ITEMS = PR_COMPACT_TILE / CT_THREADS;
+
+__device__ __forceinline__ bool ct_keep(const CompactArgs& A, int v, int b) { return A.by_bucket ? v == b : v >= 0; }
+
+// block-wide sum over CT_THREADS threads; every thread receives it
+__device__ __forceinline__ int ct_block_sum(int v, int* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int s = 0;
+    for (int w = 0; w < CT_THREADS / 64; w++) s += red[w];
+    return s;
+}
+
+__global__ __launch_bounds__(CT_THREADS) void k_compact_count(CompactArgs A) {
+    __shared__ TileMap M;
+    __shared__ int red[CT_THREADS / 64];
+    if (threadIdx.x == 0) M.load(A);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < A.nb) {
+        A.counts[PR_CNT * threadIdx.x + 2] = 0;
+        A.counts[PR_CNT * threadIdx.x + 3] = 0;
+        if (A.chunk_n) A.chunk_n[threadIdx.x] = M.n[threadIdx.x];
+    }
+    const int total = M.t0[A.nb];
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int b = M.bucket_of(g, A.nb);
+        const int t = g - M.t0[b];
+        const int* src = A.src + (A.by_bucket ? 0 : (size_t)b * A.src_stride);
+        const int base = t * PR_COMPACT_TILE;
+        int c = 0;
+        for (int i = 0; i < CT_ITEMS; i++) {
+            const int x = base + i * CT_THREADS + (int)threadIdx.x;
+            if (x < M.n[b] && ct_keep(A, src[x], b)) c++;
+        }
+        c = ct_block_sum(c, red);
+        if (threadIdx.x == 0) A.tcount[(size_t)b * A.max_tiles + t] = c;
     }
 }
 
-hipError_t launch_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_chunk, dim3(1), dim3(64), 0, s, counts, nb, c_in, c_zero, h_zero, chunk_n);
+__global__ __launch_bounds__(CT_THREADS) void k_compact_scatter(CompactArgs A) {
+    __shared__ TileMap M;
+    __shared__ int red[CT_THREADS / 64];
+    __shared__ int tile[PR_COMPACT_TILE];
+    if (threadIdx.x == 0) M.load(A);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < A.nb && M.n[threadIdx.x] == 0)
+        A.counts[PR_CNT * threadIdx.x + A.c_dst] = 0;
+    const int total = M.t0[A.nb];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int g = blockIdx.x; g < total; g += gridDim.x) {
+        const int b = M.bucket_of(g, A.nb);
+        const int t = g - M.t0[b];
+        const int* src = A.src + (A.by_bucket ? 0 : (size_t)b * A.src_stride);
+        int* dst = A.dst + (size_t)b * A.dst_stride;
+        const int base = t * PR_COMPACT_TILE;
+        const int n = min(M.n[b] - base, PR_COMPACT_TILE);
+        // survivors of the bucket's earlier tiles
+        int pre = 0;
+        for (int u = threadIdx.x; u < t; u += CT_THREADS) pre += A.tcount[(size_t)b * A.max_tiles + u];
+        pre = ct_block_sum(pre, red);
+        // the tile through LDS (coalesced), then CT_ITEMS consecutive entries per thread
+        for (int i = 0; i < CT_ITEMS; i++) {
+            const int x = i * CT_THREADS + (int)threadIdx.x;
+            tile[x] = x < n ? src[base + x] : (A.by_bucket ? -2 : -1);
+        }
+        __syncthreads();
+        const int i0 = threadIdx.x * CT_ITEMS;
+        int c = 0;
+        for (int i = 0; i < CT_ITEMS; i++) c += ct_keep(A, tile[i0 + i], b) ? 1 : 0;
+        // exclusive scan of c over the block: wave scan, then the earlier waves' totals
+        int inc = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        __syncthreads();
+        if (lane == 63) red[wave] = inc;
+        __syncthreads();
+        int off = pre + inc - c;
+        int tile_total = 0;
+        for (int w = 0; w < CT_THREADS / 64; w++) {
+            if (w < wave) off += red[w];
+            tile_total += red[w];
+        }
+        for (int i = 0; i < CT_ITEMS; i++) {
+            const int v = tile[i0 + i];
+            if (ct_keep(A, v, b)) dst[off++] = A.by_bucket ? base + i0 + i : v;
+        }
+        if (t == M.t0[b + 1] - M.t0[b] - 1 && threadIdx.x == 0)
+            A.counts[PR_CNT * b + A.c_dst] = pre + tile_total;
+        __syncthreads();  // tile[] and red[] are reused by the next tile
+    }
+}
+
+hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s) {
+    // every bucket's tiles together: at most n_total / tile + one partial tile per bucket
+    const long long tiles = (n_total + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE + a.nb;
+    const int grid = (int)std::min<long long>(std::max<long long>(tiles, 1), 1024);
+    hipLaunchKernelGGL(k_compact_count, dim3(grid), dim3(CT_THREADS), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact_scatter, dim3(grid), dim3(CT_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

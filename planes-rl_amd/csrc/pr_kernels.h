@@ -31,10 +31,7 @@ struct QueueArgs {
     FlightConst* fc;
     int n_buckets;
     const int* bucket_nt;  // device [n_buckets] upper triangle count of each bucket
-    int* count;            // bucket b's list count at count[b * count_stride]
-    int count_stride;
-    int* list;             // [n_buckets * list_stride]
-    int list_stride;
+    int* bucket;           // [n_flights] bucket of each flight, -1 when it ends at launch
     FlightState* state;
     float* fitness;
     int* steps;
@@ -48,11 +45,10 @@ struct FlyArgs {
     FlightState* state;
     float* fitness;
     int* steps;
-    const int* list_in;
+    const int* list_in;    // flights of this chunk (one bucket), in list order
     const int* count_in;
-    int* list_out;
-    int* count_out;
-    int* head;
+    int* marks;            // marks[k] = list_in[k] if that flight goes on to the next chunk, else -1
+    int* head;             // claim counter of list_in
     int chunk, max_steps;
     int run_lo, run_hi;    // the launch works only if run_lo <= flights in list_in <= run_hi
     float spread;
@@ -60,34 +56,55 @@ struct FlyArgs {
     unsigned long long* cstats; // this chunk's [0] flight-steps, [1] panel-steps (null: none)
 };
 
-// Per-bucket chunk counters (PR_CNT ints per bucket): [0..2] list counts, rotating with the
-// chunk index mod 3 (in, out, cleared for the next chunk), [3..4] queue heads, rotating mod 2.
-#define PR_CNT 5
+// Per-bucket chunk counters (PR_CNT ints per bucket): [0..1] list counts (chunk ck reads
+// slot ck & 1, its compaction writes slot (ck + 1) & 1), [2] claim counter of the
+// many-flights-per-wave pass (and of a bucket's own k_fly), [3] claim counter of the
+// one-flight-per-wave pass.  The compaction after each chunk clears [2..3].
+#define PR_CNT 4
 
-// All buckets' chunk in one persistent launch (k_fly_all).  Bucket b's lists are
-// lists{0,1} + b * list_stride (list_in = lists0 when in_list == 0); its counters are
-// counts + PR_CNT * b.  vid_p / vid_w: its fused-variant ids (many flights per wave / one
-// per wave, -1 none: the bucket is flown by its own k_fly launch); vid_w takes over once
-// the bucket's count is <= thr.  Block 0 clears the counters of the next chunk for every
-// bucket (c_zero, h_zero), so a chunk is one launch.
+// All buckets' chunk in one persistent launch (k_fly_all).  Bucket b's list is lists + b *
+// list_stride, its marks marks + b * list_stride, its counters counts + PR_CNT * b.
+// vid_p / vid_w: its fused-variant ids (many flights per wave / one per wave, -1 none: the
+// bucket is flown by its own k_fly launch); vid_w flies list positions [n - thr, n).
 struct FlyAllArgs {
     FlyArgs base;
-    int* lists0;
-    int* lists1;
+    const int* lists;
+    int* marks;
     int list_stride;
-    int in_list;
     int* counts;
-    int c_in, c_out, c_zero, h_run, h_zero;
+    int c_in;
     int nb;
     int vid_p[8], vid_w[8], thr[8];
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
-    int* chunk_n;               // [nb] flights entering this chunk per bucket (null: none)
 };
+
+// Stable compaction of every bucket's marks into its next list (k_compact_*): keeps the
+// list order, so which flights a chunk's passes fly is a function of the data alone.
+// by_bucket = 0: bucket b's source is src + b * src_stride, n = counts[PR_CNT b + c_src]
+// entries, kept if >= 0 (the value is written); by_bucket = 1: one source for all buckets
+// (k_init's bucket of each flight), n = n_fixed, entry i kept for bucket b if src[i] == b
+// (i is written).  Writes dst + b * dst_stride and counts[PR_CNT b + c_dst]; clears the
+// claim counters; records the source lengths in chunk_n (null: none).
+struct CompactArgs {
+    const int* src;
+    long long src_stride;
+    int by_bucket;
+    int c_src, n_fixed;
+    int* dst;
+    long long dst_stride;
+    int c_dst;
+    int* counts;
+    int* tcount;           // [nb * max_tiles] survivors per tile
+    int max_tiles;
+    int nb;
+    int* chunk_n;
+};
+constexpr int PR_COMPACT_TILE = 4096;
 
 int fused_variant_id(int L, int P);
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
 hipError_t fly_all_occupancy(int* blocks_per_cu);
-hipError_t launch_zero_chunk(int* counts, int nb, int c_in, int c_zero, int h_zero, int* chunk_n, hipStream_t s);
+hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s);
 
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);

@@ -226,6 +226,36 @@ def test_fused_kernel_matches_per_bucket_launches(PA, buckets):
         assert sum(b["launches"] for b in st1["buckets"]) == 5  # the (64, 24) bucket's own chunks
 
 
+def test_repeat_evaluation_bit_identical(PA):
+    """Evaluations are reproducible bit for bit: with the tail mapping on and buckets longer
+    than the fused grid, each chunk flies list positions [0, n - grid) many flights per wave
+    and the rest one flight per wave, and the lists keep their order from chunk to chunk,
+    so which mapping flies a flight does not depend on the order in which waves claim
+    work.  Same context twice, and a fresh context."""
+    n, k = 3000, 12
+    gs = O.random_genomes(n, 5)
+    seeds = np.arange(1, k + 1, dtype=np.int32)
+    params = PA.SimParams.reference(max_steps=3000)
+    runs = []
+    pl = PA.Planes(n, k, n * k)
+    for _ in range(2):
+        runs.append(pl.evaluate(gs, seeds, params))
+    st = pl.stats()
+    pl.close()
+    pl = PA.Planes(n, k, n * k)
+    runs.append(pl.evaluate(gs, seeds, params))
+    pl.close()
+    f0, s0 = runs[0]
+    for f, s in runs[1:]:
+        assert np.array_equal(f, f0) and np.array_equal(s, s0)
+    assert st["flight_steps"] == int(s0.sum())
+    # both passes ran, and the many-flights-per-wave pass had more than the grid's worth
+    many = sum(b["flight_steps"] for b in st["buckets"] if b["fused"] and not b["tail"])
+    tail = sum(b["flight_steps"] for b in st["buckets"] if b["fused"] and b["tail"])
+    print(f"flight-steps: many-per-wave {many}, one-per-wave {tail}; chunk 0 flights {st['chunks'][0]['flights']}")
+    assert many > 0 and tail > 0
+
+
 def test_graph_replay_identical(PA):
     """pr_fly captured as a hipGraph and replayed gives bit-identical results to direct
     launches, on every replay, with other eager work in between."""
