@@ -252,6 +252,10 @@ int pr_pool_set_collect(pr_pool* p, int on);
 int pr_pool_set_explore(pr_pool* p, float frac);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
+/* The last generation's pool [n_pool x 35], fly set [n_fly x 35] in flight order, its
+ * simulated robust scores [n_fly] and, if it collected training pairs, their targets
+ * [n_fly x 4]; null outputs are skipped; *n_fly = fly-set size. */
+int pr_pool_last(pr_pool* p, float* pool, float* fly, float* scores, float* targets, int* n_fly);
 
 /* Device memory helpers so that hosts without a GPU array library can drive the API. */
 int pr_dev_alloc(void** p, size_t bytes);

@@ -158,6 +158,8 @@ struct pr_pool {
     std::vector<float> acc, samples, champion;
     std::vector<int> idx;
     float champion_score = -1e9f;
+    const float* last_fly = nullptr;  // the last generation's fly set (d_pool or d_fly)
+    int last_nf = 0, last_collected = 0;
 };
 
 #define PCK(x)                                                                                   \
@@ -339,6 +341,9 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         PCK(hipDeviceSynchronize());
         mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
     }
+    p->last_fly = fly;
+    p->last_nf = nf;
+    p->last_collected = collect ? 1 : 0;
     double mean = 0;
     for (int i = 0; i < p->n_elite; i++) mean += p->acc[p->idx[i]];
     out[0] = p->acc[p->idx[0]];
@@ -385,6 +390,21 @@ int pr_pool_set_collect(pr_pool* p, int on) {
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss) {
     if (!p || !model || p->buf_n < 32) { g_pool_err = "pr_pool_train: model and >= 32 buffered pairs needed"; return -1; }
     PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, epochs, pr_mlp_max_batch(model), lr, loss));
+    return 0;
+}
+
+// The last generation's pool [n_pool x 35], fly set [n_fly x 35] (in flight order),
+// simulated robust scores of the fly set [n_fly] and, when it collected training pairs,
+// their targets [n_fly x 4] (mean, CVaR25, worst, oversize); null outputs are skipped.
+int pr_pool_last(pr_pool* p, float* pool, float* fly, float* scores, float* targets, int* n_fly) {
+    if (!p || !p->last_fly) { g_pool_err = "pr_pool_last: no generation has run"; return -1; }
+    if (targets && !p->last_collected) { g_pool_err = "pr_pool_last: the last generation collected no targets"; return -1; }
+    const int nf = p->last_nf;
+    if (pool) PCK(hipMemcpy(pool, p->d_pool, sizeof(float) * pr::NG * (size_t)p->n_pool, hipMemcpyDeviceToHost));
+    if (fly) PCK(hipMemcpy(fly, p->last_fly, sizeof(float) * pr::NG * (size_t)nf, hipMemcpyDeviceToHost));
+    if (scores) std::memcpy(scores, p->acc.data(), sizeof(float) * nf);
+    if (targets) PCK(hipMemcpy(targets, p->d_tgt, sizeof(float) * 4 * (size_t)nf, hipMemcpyDeviceToHost));
+    if (n_fly) *n_fly = nf;
     return 0;
 }
 

@@ -129,6 +129,7 @@ def _load():
     lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
     lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
+    lib.pr_pool_last.argtypes = [VP, VP, VP, VP, VP, C.POINTER(C.c_int)]
     lib.pr_mlp_param_count.restype = C.c_int
     lib.pr_mlp_create.argtypes = [C.POINTER(VP), C.c_int, C.c_uint]
     lib.pr_mlp_destroy.argtypes = [VP]
@@ -764,6 +765,7 @@ class PoolGA:
         if rc:
             raise PlanesError(f"pr_pool_create failed ({rc}): {LIB.pr_pool_last_error().decode()}")
         self.n_elite = n_elite
+        self.n_pool = n_pool
 
     def _ck(self, rc, what):
         if rc:
@@ -796,6 +798,22 @@ class PoolGA:
         loss = C.c_float()
         self._ck(LIB.pr_pool_train(self.h, surrogate.h, epochs, lr, C.byref(loss)), "pr_pool_train")
         return loss.value
+
+    def last(self, targets=False):
+        """The last generation: pool, fly set (flight order), simulated scores of the fly set
+        and, with targets=True, the training targets of the fly set (collected generations)."""
+        pool = np.zeros((self.n_pool, N_GENES), np.float32)
+        nf = C.c_int()
+        self._ck(LIB.pr_pool_last(self.h, None, None, None, None, C.byref(nf)), "pr_pool_last")
+        fly = np.zeros((nf.value, N_GENES), np.float32)
+        sc = np.zeros(nf.value, np.float32)
+        tg = np.zeros((nf.value, 4), np.float32) if targets else None
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._ck(LIB.pr_pool_last(self.h, ptr(pool), ptr(fly), ptr(sc), ptr(tg), C.byref(nf)), "pr_pool_last")
+        out = {"pool": pool, "fly": fly, "scores": sc}
+        if targets:
+            out["targets"] = tg
+        return out
 
     def champion(self):
         g = np.zeros(N_GENES, np.float32)
