@@ -294,7 +294,7 @@ __global__ void k_zero_ints(int* p, int n) {
 struct Body {
     float px, py, pz, qw, qx, qy, qz, vx, vy, vz, wx, wy, wz, t;
     float mass, ks, I0, I1, I2, I3, I4, I5, v0, v1, v2, v4, v5, v8;  // ks = invmass * dt
-    float cf, bx, by, bz;
+    float g, bx, by, bz;  // g = -2 / cf (cf: skin friction)
     int step, end, fid, nt, s0;  // s0: step at which this chunk's flying started
 };
 
@@ -312,19 +312,42 @@ __device__ __forceinline__ void wind_term(int k, int& axis, float& a, float& b, 
     }
 }
 
-template <int L>
-struct WindLane {  // this lane's share of the 7 gust terms
-    static constexpr int NT = (7 + L - 1) / L;
-    float ax[NT], ay[NT], az[NT], bt[NT], cs[NT], wg[NT];
-    int comp[NT];
+// Gust terms by lane: every aligned 8 lanes hold the 7 terms, x on lanes 0-2 (lane 3
+// none), y on 4-5, z on 6-7, so one quad sum gives x and one pair sum gives y and z (the
+// addition order of a butterfly over the group, without its selects and extra stages).
+__device__ __forceinline__ int wind_term_of_lane(int j8) {
+    return j8 < 3 ? j8 : j8 == 3 ? -1 : j8 - 1;
+}
+
+// 8-lane broadcast through the LDS crossbar (ds_swizzle bitmask mode: lane & 0x18 | or):
+// no VALU issue slot, which is the flight kernel's bottleneck.
+template <int OR>
+__device__ __forceinline__ float bcast8(float v) {
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18 | (OR << 5)));
+}
+
+// Rounded terms w * sin, summed without contraction: the same value in every lane of the 8.
+__device__ __forceinline__ void gust_sums(float w, float sn, float& gx, float& gy, float& gz) {
+#pragma clang fp contract(off)
+    const float sv = w * sn;
+    const float s1 = sv + dpp_mov<0xB1>(sv);  // lane ^ 1
+    const float s2 = s1 + dpp_mov<0x4E>(s1);  // lane ^ 2
+    gx = bcast8<0>(s2);
+    gy = bcast8<4>(s1);
+    gz = bcast8<6>(s1);
+}
+
+struct WindLane {  // this lane's gust term (zero weight on lane 3 of each 8)
+    float ax, ay, az, bt, cs, wg;
 };
 
 // One group's flight: setup from the flight's constants/state/panels, then 600 Hz steps.
 template <int L, int P>
 struct Flyer {
+    static_assert(L >= 8, "the gust terms need 8 lanes per group");
     Body B;
-    WindLane<L> W;
-    float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];
+    WindLane W;
+    float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];  // kk = cf rho A / 2
     int np;  // panel slots this group uses per lane: ceil(nt / L) <= P
     bool bad;
 
@@ -347,18 +370,19 @@ struct Flyer {
         B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
         B.I0 = fc.I[0]; B.I1 = fc.I[1]; B.I2 = fc.I[2]; B.I3 = fc.I[3]; B.I4 = fc.I[4]; B.I5 = fc.I[5];
         B.v0 = fc.iv[0]; B.v1 = fc.iv[1]; B.v2 = fc.iv[2]; B.v4 = fc.iv[3]; B.v5 = fc.iv[4]; B.v8 = fc.iv[5];
-        B.cf = fc.cf;
+        // cf >= 0.0135 in both scenario presets; the floor only keeps g finite
+        const float cf = fmaxf(fc.cf, 1e-12f);
+        B.g = -2.0f / cf;
         B.bx = fc.base[0]; B.by = fc.base[1]; B.bz = fc.base[2];
         const float phase = fc.phase, gust = fc.gust, hr = fc.half_rho;
         const vec3 shift = V(fc.shift[0], fc.shift[1], fc.shift[2]);
         const float wp = fc.warp_p, wr = fc.warp_r;
-#pragma unroll
-        for (int m = 0; m < WindLane<L>::NT; m++) {
+        {
             int axis, comp;
             float ca, cb, cc, cw;
-            wind_term(j + m * L, axis, ca, cb, cc, cw, comp);
-            W.ax[m] = axis == 0 ? ca : 0.0f; W.ay[m] = axis == 1 ? ca : 0.0f; W.az[m] = axis == 2 ? ca : 0.0f;
-            W.bt[m] = cb; W.cs[m] = cc * phase; W.wg[m] = cw * gust; W.comp[m] = comp;
+            wind_term(wind_term_of_lane(j & 7), axis, ca, cb, cc, cw, comp);
+            W.ax = axis == 0 ? ca : 0.0f; W.ay = axis == 1 ? ca : 0.0f; W.az = axis == 2 ? ca : 0.0f;
+            W.bt = cb; W.cs = cc * phase; W.wg = cw * gust;
         }
 #pragma unroll
         for (int p = 0; p < P; p++) {
@@ -369,7 +393,7 @@ struct Flyer {
                 vec3 nb = warped_normal(rb, V(a.pnx[o], a.pny[o], a.pnz[o]), wp, wr);
                 rx[p] = rb.x; ry[p] = rb.y; rz[p] = rb.z;
                 nx[p] = nb.x; ny[p] = nb.y; nz[p] = nb.z;
-                kk[p] = hr * a.pA[o];
+                kk[p] = (hr * a.pA[o]) * cf;
             } else {
                 rx[p] = 0; ry[p] = 0; rz[p] = 0; nx[p] = 0; ny[p] = 0; nz[p] = 1; kk[p] = 0;
             }
@@ -379,13 +403,15 @@ struct Flyer {
     // Force and torque of panel p in body axes (sim.cu:144-158 rotated into the body frame):
     // v_r = v_b + w x r, s = v_r.n/|v_r|, q = k |v_r|^2 (k = rho A / 2), and
     // f = -q cn n - q cf (v_r/|v_r| - s n) = c1 n + c2 v_r with cn = 2 s h,
-    // h = 0.95 sqrt(1 - s^2) + |s|, c1 = q s (cf - 2h), c2 = -q cf / |v_r|.
+    // h = 0.95 sqrt(1 - s^2) + |s|, c1 = q s (cf - 2h), c2 = -q cf / |v_r|; with the panel
+    // constant kk = cf k and qc = kk |v_r|^2 = cf q: c1 = qc s (1 + g h), g = -2 / cf, and
+    // c2 = -qc / |v_r| (one multiply fewer per panel than with k alone).
     // The reference skips panels with |v_r|^2 < 1e-6 (its NaN guard); here rsq(|v_r|^2 + 1e-30)
     // keeps v_r = 0 finite with zero force, and 0 < |v_r|^2 < 1e-6 contributes < 1e-9 N.
     // FIRST: the lane's first panel initialises the sums instead of adding to zero.
     template <bool FIRST = false>
     __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
-                                                float cf, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
+                                                float g, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
                                                 float& Tz) const {
         const float vrx = __builtin_fmaf(wy, rz[p], __builtin_fmaf(-wz, ry[p], vbx));
         const float vry = __builtin_fmaf(wz, rx[p], __builtin_fmaf(-wx, rz[p], vby));
@@ -395,11 +421,12 @@ struct Flyer {
         const float V2 = __builtin_fmaf(vrz, vrz, __builtin_fmaf(vry, vry, __builtin_fmaf(vrx, vrx, 1e-30f)));
         const float inv = __builtin_amdgcn_rsqf(V2);
         const float sn = (vrx * nx[p] + vry * ny[p] + vrz * nz[p]) * inv;
-        const float qd = kk[p] * V2;
-        const float sq = __builtin_amdgcn_sqrtf(fmaxf(1.0f - sn * sn, 0.0f));
+        const float qc = kk[p] * V2;
+        // clamp(1 - s^2, 0, 1) = max(1 - s^2, 0) (it is <= 1): folds into the FMA's clamp bit
+        const float sq = __builtin_amdgcn_sqrtf(__builtin_amdgcn_fmed3f(__builtin_fmaf(-sn, sn, 1.0f), 0.0f, 1.0f));
         const float h = 0.95f * sq + fabsf(sn);
-        const float c1 = (qd * sn) * (cf - 2.0f * h);
-        const float c2 = -(qd * inv) * cf;
+        const float c1 = (qc * sn) * __builtin_fmaf(g, h, 1.0f);
+        const float c2 = -(qc * inv);
         const float fx = c1 * nx[p] + c2 * vrx;
         const float fy = c1 * ny[p] + c2 * vry;
         const float fz = c1 * nz[p] + c2 * vrz;
@@ -425,34 +452,28 @@ struct Flyer {
         const float r00 = 1 - 2 * (qy * qy + qz * qz), r01 = 2 * (qx * qy - qw * qz), r02 = 2 * (qx * qz + qw * qy);
         const float r10 = 2 * (qx * qy + qw * qz), r11 = 1 - 2 * (qx * qx + qz * qz), r12 = 2 * (qy * qz - qw * qx);
         const float r20 = 2 * (qx * qz - qw * qy), r21 = 2 * (qy * qz + qw * qx), r22 = 1 - 2 * (qx * qx + qy * qy);
-        // gust field at the CG: this lane's terms, then a group sum per component
-        float gx = 0, gy = 0, gz = 0;
-#pragma unroll
-        for (int m = 0; m < WindLane<L>::NT; m++) {
-            float arg = W.ax[m] * B.px + W.ay[m] * B.py + W.az[m] * B.pz + W.bt[m] * B.t + W.cs[m];
-            float sv = W.wg[m] * sin_gust(arg);
-            gx += W.comp[m] == 0 ? sv : 0.0f;
-            gy += W.comp[m] == 1 ? sv : 0.0f;
-            gz += W.comp[m] == 2 ? sv : 0.0f;
-        }
-        gx = group_sum<L>(gx); gy = group_sum<L>(gy); gz = group_sum<L>(gz);
+        // gust field at the CG: this lane's term, summed per component within its 8 lanes
+        // ((t0 + t1) + (t2 + 0), t3 + t4, t5 + t6) and broadcast to them
+        const float arg = W.ax * B.px + W.ay * B.py + W.az * B.pz + W.bt * B.t + W.cs;
+        float gx, gy, gz;
+        gust_sums(W.wg, sin_gust(arg), gx, gy, gz);
         // air velocity relative to the CG, in body axes
         const float ux = B.vx - (B.bx + gx), uy = B.vy - (B.by + gy), uz = B.vz - (B.bz + gz);
         const float vbx = r00 * ux + r10 * uy + r20 * uz;
         const float vby = r01 * ux + r11 * uy + r21 * uz;
         const float vbz = r02 * ux + r12 * uy + r22 * uz;
-        const float wx = B.wx, wy = B.wy, wz = B.wz, cf = B.cf;
+        const float wx = B.wx, wy = B.wy, wz = B.wz, g = B.g;
         float Fx, Fy, Fz, Tx, Ty, Tz;
         // slots in pairs: two independent panel evaluations per branch give the scheduler
         // instruction-level parallelism to interleave (at most one padded slot evaluated);
         // the first pair always runs (a slot past the lane's triangles adds exactly zero)
-        panel_force<true>(0, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
-        if (P > 1) panel_force(1, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+        panel_force<true>(0, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+        if (P > 1) panel_force(1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
 #pragma unroll
         for (int p = 2; p < P; p += 2) {
             if (p < npw) {
-                panel_force(p, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
-                if (p + 1 < P) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, cf, Fx, Fy, Fz, Tx, Ty, Tz);
+                panel_force(p, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+                if (p + 1 < P) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
             }
         }
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
