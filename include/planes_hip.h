@@ -131,9 +131,11 @@ int pr_set_graph(pr_ctx* ctx, int on);
  * (and above the previous bucket's) are flown by groups of L[i] lanes holding P[i]
  * panels per lane (L*P >= nt_max).  Available (L, P) pairs: pr_fly_variants. */
 int pr_set_buckets(pr_ctx* ctx, int n, const int* nt_max, const int* L, const int* P);
-/* 1 (default): each bucket also queues a one-flight-per-wave mapping (64 lanes, P = nt/64)
+/* 1: each bucket also queues a one-flight-per-wave mapping (64 lanes, P = nt/64)
  * that takes over once the bucket's remaining flights fit one per wave -- the long-flight
- * tail is then latency- rather than throughput-bound.  0: one mapping throughout. */
+ * tail is then latency- rather than throughput-bound.  2 (default): the same, except that
+ * in the fused kernel a bucket switches only when the buckets it drains after it hold
+ * fewer flights than there are waves.  0: one mapping throughout. */
 int pr_set_tail_mapping(pr_ctx* ctx, int on);
 /* 1 (default): every bucket whose mappings are compiled into the fused flight kernel is
  * flown by it -- one persistent launch per 1-s chunk drains all those buckets' work lists,

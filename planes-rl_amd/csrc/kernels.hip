@@ -613,19 +613,27 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
 // triangle counts first, flying each bucket with its own (L, P) code path: list positions
 // [0, n - thr[b]) with the many-flights-per-wave mapping, the last thr[b] positions (one
 // flight per wave) with the one-flight-per-wave mapping -- the last flights of a chunk run
-// at the lowest step latency instead of leaving most lanes of a partial round idle.
+// at the lowest step latency instead of leaving most lanes of a partial round idle.  With
+// tail_when_short, a bucket does so only when the buckets drained after it hold fewer
+// flights than thr[b]: otherwise its last flights run beside those buckets' flights
+// anyway, and the one-per-wave mapping costs more instructions per flight-step.
 __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
+    // flights of the buckets drained after the current one (in this launch)
+    int below = 0;
+    for (int b = 0; b < A.nb; b++)
+        if (A.vid_p[b] >= 0) below += A.counts[PR_CNT * b + A.c_in];
     // largest triangle counts first: the longest chunk jobs start first, small ones fill in
     for (int b = A.nb - 1; b >= 0; b--) {
         if (A.vid_p[b] < 0) continue;
         int* cb = A.counts + PR_CNT * b;
         const int n_in = cb[A.c_in];
+        below -= n_in;
         if (n_in == 0) continue;
         FlyArgs a = A.base;
         a.list_in = A.lists + (size_t)b * A.list_stride;
         a.marks = A.marks + (size_t)b * A.list_stride;
         a.count_in = cb + A.c_in;
-        const int vw = A.vid_w[b];
+        const int vw = A.tail_when_short && below >= A.thr[b] ? -1 : A.vid_w[b];
         const int limit = vw >= 0 ? max(n_in - A.thr[b], 0) : n_in;
         if (limit > 0) {
             a.stats = A.stats + 2 * (2 * b);

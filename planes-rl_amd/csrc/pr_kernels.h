@@ -75,6 +75,7 @@ struct FlyAllArgs {
     int c_in;
     int nb;
     int vid_p[8], vid_w[8], thr[8];
+    int tail_when_short;        // 1: vid_w only where the later-drained buckets hold < thr flights
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
 };
 

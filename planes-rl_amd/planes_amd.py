@@ -358,6 +358,9 @@ class Planes:
         _ck(LIB.pr_set_prepare_precision(self.h, bits), "pr_set_prepare_precision")
 
     def set_tail_mapping(self, on=True):
+        """False/0: one mapping per bucket; True/1: every bucket's last flights of a chunk fly
+        one per wave; 2 (default): in the fused kernel only a bucket whose later-drained
+        buckets hold fewer flights than there are waves does."""
         _ck(LIB.pr_set_tail_mapping(self.h, int(on)), "pr_set_tail_mapping")
 
     def set_fused(self, on=True):

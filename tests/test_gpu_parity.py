@@ -179,21 +179,36 @@ def test_mapping_invariance(PA):
         assert np.mean(rel < 1e-3) >= 0.95, (L, P)
 
 
-def test_tail_mapping_equivalent(PA):
+@pytest.mark.parametrize("n", [64, 1200])
+def test_tail_mapping_equivalent(PA, n):
     """The one-flight-per-wave tail mapping flies the same flights to the same results as
-    the many-per-wave mapping, up to float reassociation of the group sums."""
-    gs = O.random_genomes(64, 17)
+    the many-per-wave mapping, up to float reassociation of the group sums: modes 1 (every
+    bucket's last flights) and 2 (only where the buckets drained later hold fewer flights
+    than there are waves) against mode 0.  64 genomes x 8: every flight fits one per wave,
+    so modes 1 and 2 fly all of them with the tail mapping; 1200 x 8: the monoplane bucket
+    (drained last) holds more flights than there are waves, so in mode 2 the biplane
+    bucket's chunk-0 flights stay many per wave."""
+    gs = O.random_genomes(n, 17)
     seeds = np.arange(1, 9, dtype=np.int32)
     params = PA.SimParams.reference(max_steps=3000)
     res = []
-    for tail in (False, True):
-        pl = PA.Planes(64, 8, 512)
+    for tail in (0, 1, 2):
+        pl = PA.Planes(n, 8, n * 8)
         pl.set_tail_mapping(tail)
         fit, steps = pl.evaluate(gs, seeds, params)
-        res.append((fit.reshape(-1), steps.reshape(-1)))
-    rel = np.abs(res[0][0] - res[1][0]) / np.maximum(np.abs(res[0][0]), 1e-3)
-    print(f"tail mapping: {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3, max {rel.max():.2e}")
-    assert np.mean(rel < 1e-2) >= 0.97
+        st = pl.stats()
+        tail_steps = sum(b["flight_steps"] for b in st["buckets"] if b["tail"])
+        res.append((fit.reshape(-1), tail_steps, st["flight_steps"]))
+    for mode in (1, 2):
+        rel = np.abs(res[0][0] - res[mode][0]) / np.maximum(np.abs(res[0][0]), 1e-3)
+        print(f"{n * 8} flights, tail mapping mode {mode}: {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3, "
+              f"max {rel.max():.2e}, tail mapping flew {res[mode][1]} of {res[mode][2]} flight-steps")
+        assert np.mean(rel < 1e-2) >= 0.97
+    assert res[0][1] == 0                                        # mode 0: no tail mapping
+    if n == 64:
+        assert res[1][1] == res[1][2] and res[2][1] == res[2][2]  # everything one per wave
+    else:
+        assert 0 < res[2][1] < res[1][1] < res[1][2]
 
 
 @pytest.mark.parametrize("buckets", [None, ([192, 1536], [8, 64], [24, 24])])
@@ -393,20 +408,34 @@ def test_autotuner_picks_valid_mapping_and_results_hold(PA, tmp_path):
 
 def test_batched_audit_matches_single_design_flights_and_fp64(PA):
     """pr_audit (evaluate_design for many designs in one flight batch): each genome's
-    distances equal those of flying it alone on the seeds base + s*7919 (bit-identical),
-    the statistics equal pr_design_stats of those distances, and the champion's distances
-    agree with the FP64 oracle within 1 % for >= 95 % of scenarios (C5 preset: gusts <= 5
-    m/s RMS, warp +-1 deg, 20 s cap)."""
+    distances equal those of flying it alone on the seeds base + s*7919 -- bit-identical
+    with the tail mapping off (each flight's lane mapping then depends on its triangle
+    count alone), and within float reassociation with the default tail mapping (which
+    mapping flies a chunk's last flights depends on the batch); the statistics equal
+    pr_design_stats of those distances, and the champion's distances agree with the FP64
+    oracle within 1 % for >= 95 % of scenarios (C5 preset: gusts <= 5 m/s RMS, warp +-1 deg,
+    20 s cap)."""
     gs = O.random_genomes(12, 77)
     n_scen, base = 96, 9001
     params = PA.SimParams.config(gust_rms_max=5.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=12000)
-    pl = PA.Planes(12, n_scen, 12 * n_scen)
-    st, dist = pl.audit(gs, n_scen, base, params)
     seeds = (base + 7919 * np.arange(n_scen)).astype(np.int32)
+    pl = PA.Planes(12, n_scen, 12 * n_scen)
     one = PA.Planes(1, n_scen, n_scen)
-    for g in (0, 5, 11):
-        fit, _ = one.evaluate(gs[g:g + 1], seeds, params)
-        assert np.array_equal(fit.reshape(-1), dist[g]), g
+    for tail in (0, 2):
+        pl.set_tail_mapping(tail)
+        one.set_tail_mapping(tail)
+        st, dist = pl.audit(gs, n_scen, base, params)
+        for g in (0, 5, 11):
+            fit, _ = one.evaluate(gs[g:g + 1], seeds, params)
+            if tail == 0:
+                assert np.array_equal(fit.reshape(-1), dist[g]), g
+            else:  # 20-s gusty flights: a few diverge chaotically from ulp-level differences
+                f = fit.reshape(-1)
+                rel = np.abs(f - dist[g]) / np.maximum(np.abs(dist[g]), 1e-3)
+                m0, m1 = np.median(f), np.median(dist[g])
+                print(f"design {g}, default tail mapping: {np.mean(rel < 1e-3) * 100:.1f}% of scenarios within 1e-3 "
+                      f"of the batch, median {m0:.4f} vs {m1:.4f}")
+                assert np.mean(rel < 1e-3) >= 0.90 and abs(m0 - m1) <= 0.01 * abs(m1)
     for g in range(12):
         ref = PA.design_stats(dist[g])
         assert list(st[g]) == [np.float32(ref[k]) for k in PA.AUDIT_FIELDS], g
