@@ -82,3 +82,26 @@ def test_select_best(tmp_path):
     scores = [float(r.split(",")[1]) for r in rows[1:]]
     win = (tmp_path / "o" / "best-final.bin").read_bytes()
     assert win == gs[int(np.argmax(scores))].astype(np.float32).tobytes()
+
+
+def test_presets_seed_population_and_descriptions(tmp_path):
+    """presets (designs.cu:58-73): seeds.bin holds the five preset genomes bit for bit in the
+    pop.bin layout (a warm start for evolve --init), and each description line (designs.cu:45-54)
+    equals the one formatted from the oracle's mesh and mass properties."""
+    d = np.load(os.path.join(ROOT, "tests", "golden", "ref_presets.npz"), allow_pickle=False)
+    gs, names = d["genomes"], d["names"]
+    s = run("presets", "--out", str(tmp_path / "p"), cwd=tmp_path)
+    assert np.array_equal(read_pop(tmp_path / "p" / "seeds.bin"), gs)
+    lines = s.splitlines()
+    for i, (g, name) in enumerate(zip(gs, names)):
+        m, printable, _, _ = O.mass(g)
+        V, _ = O.mesh(g)
+        e = V.max(0) - V.min(0)
+        want = (f"{name:<18} {m[0] * 1000:6.1f}g  {e[0] * 1000:3.0f}x{e[1] * 1000:3.0f}x{e[2] * 1000:3.0f} mm  "
+                f"CG=({m[2] * 1000:5.1f},{m[3] * 1000:5.1f},{m[4] * 1000:5.1f}) mm  "
+                f"{'printable' if printable else 'TOO LARGE'}")
+        assert lines[i] == want, (lines[i], want)
+    assert "preset seed population ->" in s
+    s = run("evolve", "--pop", "16", "--gens", "1", "--seeds", "2", "--out", str(tmp_path / "e"), "--init",
+            str(tmp_path / "p" / "seeds.bin"), cwd=tmp_path)
+    assert "warm start: 5 genomes" in s
