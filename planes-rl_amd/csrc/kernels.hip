@@ -293,7 +293,7 @@ __global__ void k_zero_ints(int* p, int n) {
 // values (they compute the same expressions on the same inputs).
 struct Body {
     float px, py, pz, qw, qx, qy, qz, vx, vy, vz, wx, wy, wz, t;
-    float mass, ks, I0, I1, I2, I3, I4, I5, v0, v1, v2, v4, v5, v8;  // ks = invmass * dt
+    float mass, ks, I0, I1, I2, I3, I4, I5, v0, v1, v2, v4, v5, v8;  // ks = invmass * dt, v* = invI * dt
     float g, bx, by, bz;  // g = -2 / cf (cf: skin friction)
     int step, end, fid, nt, s0;  // s0: step at which this chunk's flying started
 };
@@ -369,7 +369,8 @@ struct Flyer {
         B.end = min(B.step + a.chunk, a.max_steps);
         B.mass = fc.mass; B.ks = fc.invmass * PR_DT;
         B.I0 = fc.I[0]; B.I1 = fc.I[1]; B.I2 = fc.I[2]; B.I3 = fc.I[3]; B.I4 = fc.I[4]; B.I5 = fc.I[5];
-        B.v0 = fc.iv[0]; B.v1 = fc.iv[1]; B.v2 = fc.iv[2]; B.v4 = fc.iv[3]; B.v5 = fc.iv[4]; B.v8 = fc.iv[5];
+        B.v0 = fc.iv[0] * PR_DT; B.v1 = fc.iv[1] * PR_DT; B.v2 = fc.iv[2] * PR_DT;
+        B.v4 = fc.iv[3] * PR_DT; B.v5 = fc.iv[4] * PR_DT; B.v8 = fc.iv[5] * PR_DT;
         // cf >= 0.0135 in both scenario presets; the floor only keeps g finite
         const float cf = fmaxf(fc.cf, 1e-12f);
         B.g = -2.0f / cf;
@@ -486,30 +487,36 @@ struct Flyer {
         const float Iwx = B.I0 * wx + B.I3 * wy + B.I4 * wz;
         const float Iwy = B.I3 * wx + B.I1 * wy + B.I5 * wz;
         const float Iwz = B.I4 * wx + B.I5 * wy + B.I2 * wz;
-        const float tqx = Tx - (wy * Iwz - wz * Iwy);
-        const float tqy = Ty - (wz * Iwx - wx * Iwz);
-        const float tqz = Tz - (wx * Iwy - wy * Iwx);
-        const float dwx = B.v0 * tqx + B.v1 * tqy + B.v2 * tqz;
-        const float dwy = B.v1 * tqx + B.v4 * tqy + B.v5 * tqz;
-        const float dwz = B.v2 * tqx + B.v5 * tqy + B.v8 * tqz;
-        float nwx = wx + dwx * PR_DT, nwy = wy + dwy * PR_DT, nwz = wz + dwz * PR_DT;
-        // hardware v_sqrt / v_rsq (<= 1 ulp) for the clamps and the renormalisation: the
-        // IEEE-exact forms cost ~13 instructions each, ~50 per step (the reference's GPU
-        // build uses the approximate forms too, via -use_fast_math)
-        const float wl = __builtin_amdgcn_sqrtf(nwx * nwx + nwy * nwy + nwz * nwz);
-        if (wl > 150.f) { const float s = 150.f / wl; nwx *= s; nwy *= s; nwz *= s; }
+        // tq = T - w x (I w), then w += dt invI tq (dt folded into the inverse inertia)
+        const float tqx = __builtin_fmaf(-wy, Iwz, __builtin_fmaf(wz, Iwy, Tx));
+        const float tqy = __builtin_fmaf(-wz, Iwx, __builtin_fmaf(wx, Iwz, Ty));
+        const float tqz = __builtin_fmaf(-wx, Iwy, __builtin_fmaf(wy, Iwx, Tz));
+        float nwx = wx + (B.v0 * tqx + B.v1 * tqy + B.v2 * tqz);
+        float nwy = wy + (B.v1 * tqx + B.v4 * tqy + B.v5 * tqz);
+        float nwz = wz + (B.v2 * tqx + B.v5 * tqy + B.v8 * tqz);
+        // magnitude clamps compared on squares (the square roots only where a clamp applies:
+        // v_sqrt / v_rsq issue at half the FMA rate); hardware v_sqrt / v_rsq (<= 1 ulp) for
+        // the clamps and the renormalisation, as the reference's GPU build (-use_fast_math)
+        const float wl2 = nwx * nwx + nwy * nwy + nwz * nwz;
+        if (wl2 > 150.f * 150.f) {
+            const float s = 150.f / __builtin_amdgcn_sqrtf(wl2);
+            nwx *= s; nwy *= s; nwz *= s;
+        }
         const float ks = B.ks;
         float nvx = B.vx + Fwx * ks, nvy = B.vy + Fwy * ks, nvz = B.vz + Fwz * ks;
-        const float vl = __builtin_amdgcn_sqrtf(nvx * nvx + nvy * nvy + nvz * nvz);
-        if (vl > 60.f) { const float s = 60.f / vl; nvx *= s; nvy *= s; nvz *= s; }
+        const float vl2 = nvx * nvx + nvy * nvy + nvz * nvz;
+        if (vl2 > 60.f * 60.f) {
+            const float s = 60.f / __builtin_amdgcn_sqrtf(vl2);
+            nvx *= s; nvy *= s; nvz *= s;
+        }
         const float npx = B.px + nvx * PR_DT, npy = B.py + nvy * PR_DT, npz = B.pz + nvz * PR_DT;
-        // q += 0.5 dt q (x) (0, w)
-        const float dqw = -qx * nwx - qy * nwy - qz * nwz;
-        const float dqx = qw * nwx + qy * nwz - qz * nwy;
-        const float dqy = qw * nwy - qx * nwz + qz * nwx;
-        const float dqz = qw * nwz + qx * nwy - qy * nwx;
+        // q += q (x) (0, w dt / 2)
         const float hd = 0.5f * PR_DT;
-        float nqw = qw + dqw * hd, nqx = qx + dqx * hd, nqy = qy + dqy * hd, nqz = qz + dqz * hd;
+        const float hx = nwx * hd, hy = nwy * hd, hz = nwz * hd;
+        float nqw = qw - qx * hx - qy * hy - qz * hz;
+        float nqx = qx + qw * hx + qy * hz - qz * hy;
+        float nqy = qy + qw * hy - qx * hz + qz * hx;
+        float nqz = qz + qw * hz + qx * hy - qy * hx;
         const float il = __builtin_amdgcn_rsqf(nqw * nqw + nqx * nqx + nqy * nqy + nqz * nqz);
         B.qw = nqw * il; B.qx = nqx * il; B.qy = nqy * il; B.qz = nqz * il;
         B.vx = nvx; B.vy = nvy; B.vz = nvz;

@@ -1,5 +1,6 @@
 // valu_ceiling -- measures the FP32 vector-ALU issue ceiling of the device the flight kernel
-// runs on: wave-instructions per cycle per SIMD for chains of v_fma_f32 (and v_mul/v_add),
+// runs on: wave-instructions per cycle per SIMD for chains of v_fma_f32 (and v_mul/v_add,
+// the v_rsq_f32 / v_sqrt_f32 the flight step issues twice per panel, and a 3:1 FMA:rsq mix),
 // dependent and independent, at 1..8 waves per SIMD, with the shader clock read from
 // s_memtime / s_memrealtime (100 MHz).  The flight kernel is built without packed FP32, so
 // its ceiling is (this rate) x 64 lanes x 2 FLOP x SIMDs x clock, not the packed-FP32 peak.
@@ -11,7 +12,7 @@ template <int C, int OP>
 __global__ __launch_bounds__(64) void k_chain(float* out, long long* clk, float a, float b, int iters) {
     float x[C];
 #pragma unroll
-    for (int c = 0; c < C; c++) x[c] = threadIdx.x * 1e-3f + c;
+    for (int c = 0; c < C; c++) x[c] = 1.0f + threadIdx.x * 1e-3f + c;
     const long long t0 = __builtin_amdgcn_s_memtime();
     const long long r0 = __builtin_amdgcn_s_memrealtime();
     for (int i = 0; i < iters; i++) {
@@ -21,7 +22,10 @@ __global__ __launch_bounds__(64) void k_chain(float* out, long long* clk, float 
             for (int c = 0; c < C; c++) {
                 if (OP == 0) x[c] = __builtin_fmaf(x[c], a, b);
                 else if (OP == 1) x[c] = x[c] * a;
-                else x[c] = x[c] + b;
+                else if (OP == 2) x[c] = x[c] + b;
+                else if (OP == 3) x[c] = __builtin_amdgcn_rsqf(x[c]);  // positive, tends to 1
+                else if (OP == 4) x[c] = __builtin_amdgcn_sqrtf(x[c]);
+                else x[c] = c % 4 == 0 ? __builtin_amdgcn_rsqf(x[c]) : __builtin_fmaf(x[c], a, b);  // 1 in 4
             }
         }
     }
@@ -54,7 +58,7 @@ static bool run(const char* name, int wps, float* d, long long* dc, int ncu) {
     const double ghz = (double)c[0] / ((double)c[1] / 100e6) / 1e9;
     const double instr = (double)blocks * iters * 16 * C;
     const double per_cyc = instr / (ncu * 4.0) / (ms * 1e-3 * ghz * 1e9);
-    const double tflops = per_cyc * 64 * (OP == 0 ? 2 : 1) * ncu * 4.0 * ghz * 1e9 / 1e12;
+    const double tflops = per_cyc * 64 * (OP == 0 ? 2 : OP == 5 ? 1.75 : 1) * ncu * 4.0 * ghz * 1e9 / 1e12;
     printf("%-4s chains=%d waves/SIMD=%d: %7.2f ms, clock %.2f GHz, %.3f wave-instr/cycle/SIMD, %.1f TFLOP/s\n", name,
            C, wps, ms, ghz, per_cyc, tflops);
     (void)hipEventDestroy(e0);
@@ -73,7 +77,9 @@ int main() {
     for (int w : {1, 2, 4, 8}) {
         if (!good) break;
         good = run<1, 0>("fma", w, d, dc, ncu) && run<8, 0>("fma", w, d, dc, ncu) &&
-               run<8, 1>("mul", w, d, dc, ncu) && run<8, 2>("add", w, d, dc, ncu);
+               run<8, 1>("mul", w, d, dc, ncu) && run<8, 2>("add", w, d, dc, ncu) &&
+               run<8, 3>("rsq", w, d, dc, ncu) && run<8, 4>("sqrt", w, d, dc, ncu) &&
+               run<8, 5>("mix", w, d, dc, ncu);
     }
     (void)hipFree(d);
     (void)hipFree(dc);
