@@ -132,15 +132,18 @@ def pool_ab(PA, args):
             gens.append(r)
             log(f"[{mode}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout {r['holdout']:.3f} "
                 f"(best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) "
-                f"flights {r['flights']:.0f} rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m")
+                f"flights {r['flights']:.0f} rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m "
+                f"oversize acc {r['oversize_acc']:.3f}")
         wall = sum(r["wall_ms"] for r in gens) / 1000.0
         rc = [r["rank_corr"] for r in gens if r["rank_corr"] == r["rank_corr"]]
         mae = [r["mae"] for r in gens if r["mae"] == r["mae"]]
+        oacc = [r["oversize_acc"] for r in gens if r["oversize_acc"] == r["oversize_acc"]]
         out[mode] = {"gens": len(gens), "wall_s": round(wall, 3), "gens_per_hour": round(3600.0 * len(gens) / wall, 1),
                      "champion_holdout": round(gens[-1]["best_holdout"], 4),
                      "final_elite_mean": round(gens[-1]["elite_mean"], 4), "flights_per_gen": gens[-1]["flights"],
                      "mean_rank_corr": round(float(np.mean(rc)), 3) if rc else None,
-                     "mean_mae_m": round(float(np.mean(mae)), 3) if mae else None}
+                     "mean_mae_m": round(float(np.mean(mae)), 3) if mae else None,
+                     "mean_oversize_acc": round(float(np.mean(oacc)), 3) if oacc else None}
     ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
     champ = out["surrogate"]["champion_holdout"] / out["plain"]["champion_holdout"]
     line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",

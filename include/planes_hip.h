@@ -233,10 +233,11 @@ int pr_mlp_train(pr_mlp* m, const float* d_genomes, const float* d_targets, int 
  * best surrogate-predicted score (n_fly in total); new elite = best simulated scores;
  * top-8 holdout champion; with a model, the flown pairs join a replay buffer of buf_cap
  * and the network trains on it.  The simulation alone ranks the elite and the champion.
- * pr_pool_generation out[13]: best score, elite mean, holdout champion score, wall ms,
+ * pr_pool_generation out[14]: best score, elite mean, holdout champion score, wall ms,
  * flight ms, surrogate ms, flights, plane-flight-s, training loss, fly-set size, best
- * holdout score so far, Spearman rho and mean absolute error (m) of predicted vs
- * simulated scores on the fly set's exploration sample (uniform draws from the pool). */
+ * holdout score so far; on the fly set's exploration sample (uniform draws from the pool):
+ * Spearman rho of predicted vs simulated scores, mean absolute error (m) over the members
+ * both call flyable, accuracy of the predicted oversize flag. */
 typedef struct pr_pool pr_pool;
 int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds, int buf_cap, float sigma_frac,
                    unsigned seed);

@@ -256,12 +256,14 @@ static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
 
 // One generation.  model = null: plain (fly the whole pool); else pre-screen with the
 // surrogate and train it on the flown pairs for train_epochs epochs over the buffer.
-// out[0..12]: best simulated score, elite mean score, holdout score of the champion,
+// out[0..13]: best simulated score, elite mean score, holdout score of the champion,
 // wall ms, flight ms, surrogate ms (predict + train), flights flown, plane-flight-s,
 // last training loss, fly-set size, best holdout score so far (the champion record),
 // Spearman correlation of predicted and simulated scores over the fly set's exploration
-// sample (NaN when plain; 0 when it has fewer than 3 members), and the mean absolute
-// error of the predicted robust score on that sample in metres (NaN when plain or empty).
+// sample (NaN when plain; 0 when it has fewer than 3 members), the mean absolute error
+// of the predicted robust score in metres over the sample's members that both the network
+// and the simulation call flyable, and the accuracy of the network's oversize flag over
+// the sample (NaN when plain or empty).
 int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* model, int train_epochs, float lr,
                        double* out) {
     if (!p || !prm || !out) { g_pool_err = "pr_pool_generation: null argument"; return -1; }
@@ -316,7 +318,7 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     // screening quality: rank correlation of predicted and simulated scores over the
     // exploration sample of the fly set (uniform draws from the pool, so unbiased by the
     // selection; the surrogate-selected part over-represents over-predicted candidates)
-    double rho = std::nan(""), mae = std::nan("");
+    double rho = std::nan(""), mae = std::nan(""), acc_over = std::nan("");
     if (model) {
         hipLaunchKernelGGL(k_gather_col, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->d_vals2, nf,
                            p->d_keys);
@@ -328,10 +330,18 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         for (int i = 0; i < nf; i++)
             if (key[i] == -2.0e38f) { a.push_back(pf[i]); b.push_back(p->acc[i]); }
         rho = spearman(a, b);
+        // the predicted score is -100 where the oversize head says so: regression error over
+        // the members both call flyable, and the oversize flag's accuracy over all of them
         if (!a.empty()) {
-            mae = 0;
-            for (size_t i = 0; i < a.size(); i++) mae += std::fabs((double)a[i] - (double)b[i]);
-            mae /= (double)a.size();
+            double e = 0;
+            int n_fly = 0, agree = 0;
+            for (size_t i = 0; i < a.size(); i++) {
+                const bool pa = a[i] == -100.0f, pb = b[i] == -100.0f;
+                agree += pa == pb;
+                if (!pa && !pb) { e += std::fabs((double)a[i] - (double)b[i]); n_fly++; }
+            }
+            if (n_fly) mae = e / n_fly;
+            acc_over = (double)agree / (double)a.size();
         }
     }
     float loss = 0;
@@ -365,6 +375,7 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     out[10] = p->best_holdout;
     out[11] = rho;
     out[12] = mae;
+    out[13] = acc_over;
     return 0;
 }
 

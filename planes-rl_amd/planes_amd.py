@@ -760,7 +760,7 @@ class PoolGA:
     pre-screened (fly the elite + the best-predicted candidates)."""
 
     FIELDS = ("best", "elite_mean", "holdout", "wall_ms", "fly_ms", "mlp_ms", "flights", "pfs", "loss", "n_fly",
-              "best_holdout", "rank_corr", "mae")
+              "best_holdout", "rank_corr", "mae", "oversize_acc")
 
     def __init__(self, n_elite, n_pool, n_fly, nseeds, buf_cap=2_000_000, sigma_frac=0.05, seed=1):
         self.h = VP()

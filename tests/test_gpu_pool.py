@@ -10,8 +10,8 @@ operators are pinned in test_gpu_parity.py and test_host.py).  Pinned here:
     fly set's best n_elite by that score;
   - surrogate pre-screening: the fly set is the elite, then the exploration sample (hashed
     per candidate), then the best-predicted candidates, in the order of a stable sort of
-    those keys; the reported rank correlation and mean absolute error are those over the
-    exploration sample;
+    those keys; the reported rank correlation, mean absolute error (members both call
+    flyable) and oversize-flag accuracy are those over the exploration sample;
   - the replay-buffer targets equal the CPU model's fitness statistics of the flights;
   - reproducibility: the same seed gives the same generations bit for bit.
 """
@@ -178,7 +178,10 @@ def test_surrogate_prescreening_selection_and_targets(PA):
     rb = np.argsort(np.argsort(b, kind="stable"), kind="stable").astype(np.float64)
     rho = np.corrcoef(ra, rb)[0, 1]
     assert r["rank_corr"] == pytest.approx(rho, abs=1e-9)
-    assert r["mae"] == pytest.approx(float(np.mean(np.abs(a.astype(np.float64) - b))), rel=1e-9)
+    pa, pb = a == -100.0, b == -100.0
+    both = ~pa & ~pb
+    assert r["mae"] == pytest.approx(float(np.mean(np.abs(a[both].astype(np.float64) - b[both]))), rel=1e-9)
+    assert r["oversize_acc"] == pytest.approx(float(np.mean(pa == pb)), abs=1e-12)
     # the best-predicted part really is the best predicted outside the elite and the sample
     rest = ~is_exp & (i >= n_elite)
     chosen = order[n_elite + n_exp:n_fly]
