@@ -35,7 +35,7 @@ WORKLOADS = {
                desc="64 genomes x 8 seeded launch scenarios, config preset, 8 s cap"),
     "c2": dict(pop=10000, nseeds=8, max_steps=9000, gust=2.0, warp_deg=1.0, wind_sd=2.0,
                desc="one GA generation: pop 10000 x 8 seeds, config preset, 15 s cap, top-8 x 128 holdout"),
-    "c3": dict(pop=100000, nseeds=12, max_steps=7200, gust=4.0, warp_deg=1.0, wind_sd=2.0, families=True,
+    "c3": dict(pop=100000, nseeds=12, max_steps=7200, gust=4.0, warp_deg=1.0, wind_sd=2.0, families=True, autotune=True,
                desc="one GA generation: pop 100000 x 12 environments, mixed families (delta/elliptical/plank/"
                     "biplane, centre/twin/triple/outboard fins), triangles log-uniform 200..900, gusts <= 4 m/s "
                     "RMS, 12 s cap, top-8 x 128 holdout"),
@@ -247,6 +247,9 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
+                    help="c1-c3: flight-kernel mapping per bucket from the autotuner (setup, untimed; "
+                         "default on for c3, whose configuration asks for it)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="c1-c3: finish each generation's holdout before the next generation starts")
@@ -285,6 +288,14 @@ def main():
     E.upload(genomes, res)
     if args.unfused:
         E.set_fused(False)
+    tuned = None
+    if args.autotune if args.autotune is not None else w.get("autotune", False):
+        # one-time setup outside the timed region: the mapping per triangle-count bucket from
+        # timing every compiled (L, P) on a sample of this population's flights
+        t0 = time.perf_counter()
+        tuned = PA.autotune(genomes, res, seeds_for(20000, w["nseeds"]), params)
+        E.set_buckets(*tuned)
+        log(f"autotuned buckets {list(zip(*tuned))} in {time.perf_counter() - t0:.1f} s")
 
     for k in range(args.warmup):
         E.evaluate(w["pop"], seeds_for(10000 + k, w["nseeds"]), params)
@@ -372,7 +383,8 @@ def main():
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"{args.workload.upper()}: {w['desc']}", "pop": w["pop"], "seeds": w["nseeds"],
-                      "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU"},
+                      "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU",
+                      "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table"},
            "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -505,6 +505,13 @@ class Generation:
         for which in (0, 1, 2):
             _ck(LIB.pr_set_graph(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_graph")
 
+    def set_buckets(self, nt_max, L, P):
+        """Flight-kernel mapping per triangle-count bucket (e.g. from autotune) for every
+        flight context of the generation."""
+        for which in (0, 1, 2):
+            _ck(LIB.pr_set_buckets(LIB.pr_gen_context(self.h, which), len(nt_max), np.asarray(nt_max, np.int32),
+                                   np.asarray(L, np.int32), np.asarray(P, np.int32)), "pr_set_buckets")
+
     def evaluate_async(self, pop, seeds, params, holdout_k=8, want_samples=False):
         """Population part now (acc, idx[, samples]); the holdout batch keeps flying on its
         own stream until collect()."""
@@ -594,13 +601,15 @@ def family_population(n, seed=0, nt_min=200, nt_max=900, biplane_frac=0.25):
 
 
 # ------------------------------------------------------------------ autotuner
-def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536, 2048), sample_flights=24576,
-             steps=600, out_json=None, verbose=False):
+def autotune(genomes, res, seeds, params, ranges=(192, 288, 384, 576, 768, 1024, 1536, 2048), sample_flights=98304,
+             steps=600, out_json=None, verbose=False, merge=False, repeats=3):
     """Pick the flight-kernel mapping per triangle-count range: for each range (lo, hi],
     every compiled (L lanes per flight, P panels per lane) variant with L*P >= hi flies
     the same sample of real flights (genomes whose count falls in the range) for one
-    chunk; the variant with the least kernel time per panel-step wins.  Adjacent ranges
-    with the same winner are merged.  Returns (nt_max, L, P) lists; writes JSON if asked."""
+    chunk; the variant with the least kernel time per panel-step wins.  The ranges stay
+    separate buckets by default (a wave runs as many panel slots as its fullest flight
+    needs, so narrower ranges waste fewer); merge=True joins adjacent ranges with the same
+    winner.  Returns (nt_max, L, P) lists; writes JSON if asked."""
     import json
 
     g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
@@ -636,13 +645,17 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
             else:
                 pl.set_buckets([hi, big[0] * big[1]], [L, big[0]], [P, big[1]])
             pl.fly_device(0, ng, len(seeds), tune_params)  # warm
-            pl.fly_device(0, ng, len(seeds), tune_params)
-            st = pl.stats()
-            b0 = st["buckets"][0]
-            # inside the fused kernel (at its occupancy) when the variant is compiled into
-            # it -- the catch-all bucket holds no flights, so the launch time is bucket 0's
-            ms = st["fused_ms"] if b0["fused"] else b0["fly_ms"]
-            cost = ms / max(1, b0["panel_steps"])
+            cost, ms = None, 0.0
+            for _ in range(repeats):  # the fastest of a few runs: timing noise only adds
+                pl.fly_device(0, ng, len(seeds), tune_params)
+                st = pl.stats()
+                b0 = st["buckets"][0]
+                # inside the fused kernel (at its occupancy) when the variant is compiled into
+                # it -- the catch-all bucket holds no flights, so the launch time is bucket 0's
+                t = st["fused_ms"] if b0["fused"] else b0["fly_ms"]
+                c = t / max(1, b0["panel_steps"])
+                if cost is None or c < cost:
+                    cost, ms = c, t
             if verbose:
                 print(f"  range ({lo},{hi}] L={L} P={P}: {ms:.3f} ms, {cost * 1e9:.3f} ps/panel-step")
             if best is None or cost < best[0]:
@@ -652,7 +665,7 @@ def autotune(genomes, res, seeds, params, ranges=(192, 352, 512, 704, 928, 1536,
         lo = hi
     merged = []
     for hi, L, P in choice:
-        if merged and merged[-1][1:] == (L, P):
+        if merge and merged and merged[-1][1:] == (L, P):
             merged[-1] = (hi, L, P)
         else:
             merged.append((hi, L, P))
