@@ -99,6 +99,35 @@ def cpu_baseline(genomes, res, w, budget_s):
                       f"plane-flight-s; 1 thread: {n1} flights, {wall1:.1f} s"}
 
 
+def fp64_check(PA, genomes, res, w, params, n_planes=256):
+    """The oracle as the checker (this leg only, beside the CPU baseline): the first n_planes
+    genomes of the timed population under the first timed generation's seeds, flown on the
+    GPU (after the timed region) and by the FP64 oracle on the host; per-flight distance
+    error against FP64 and the rank agreement of the flights."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    n = min(n_planes, len(genomes))
+    g = np.ascontiguousarray(genomes[:n], np.float32)
+    rs = np.ascontiguousarray(res[:n] if res is not None else np.tile([[5, 8]], (n, 1)), np.int32)
+    seeds = seeds_for(0, w["nseeds"])
+    pl = PA.Planes(n, len(seeds), n * len(seeds), int(rs[:, 0].max()), int(rs[:, 1].max()))
+    fit, _ = pl.evaluate(g, seeds, params, res=rs if res is not None else None)
+    pl.close()
+    gpu = np.asarray(fit, np.float64).reshape(-1)
+    gi = np.repeat(np.arange(n, dtype=np.int32), len(seeds))
+    sd = np.tile(seeds, n).astype(np.int32)
+    f64, _, wall = O.fly_res(g, rs, gi, sd, nthreads=os.cpu_count() or 1, prec=64, preset=1, gust=w["gust"],
+                             warp=float(np.deg2rad(w["warp_deg"])), wsd=w["wind_sd"], max_steps=w["max_steps"])
+    f64 = np.asarray(f64, np.float64)
+    rel = np.abs(gpu - f64) / np.maximum(np.abs(f64), 1e-3)
+    ra = np.argsort(np.argsort(gpu, kind="stable"), kind="stable")
+    rb = np.argsort(np.argsort(f64, kind="stable"), kind="stable")
+    return {"planes": n, "flights": int(len(gpu)), "within_1pct": round(float(np.mean(rel < 0.01)), 4),
+            "median_rel_err": float(f"{np.median(rel):.3g}"), "p99_rel_err": float(f"{np.quantile(rel, 0.99):.3g}"),
+            "spearman": round(float(np.corrcoef(ra, rb)[0, 1]), 6), "fp64_wall_s": round(wall, 1)}
+
+
 def pool_ab(PA, args):
     """C4: GA generations/hour with surrogate pre-screening vs the plain batched sim.
     plain: every generation flies the whole pool of mutants; surrogate: the network scores
@@ -374,10 +403,13 @@ def main():
             f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
 
     cpu = None
+    f64 = None
     if args.cpu_seconds > 0:
         log("cpu baseline ...")
         cpu = cpu_baseline(genomes, res, w, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.1f} pfs/s on {cpu['cores']} threads")
+        f64 = fp64_check(PA, genomes, res, w, params)
+        log(f"FP64 check: {f64}")
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -386,7 +418,7 @@ def main():
                       "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU",
                       "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table"},
            "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu, "fp64_check": f64}
     print(json.dumps(out), flush=True)
 
 
