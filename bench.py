@@ -221,7 +221,7 @@ def audit_bench(PA, args):
     fsteps, stats = 0, []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st, _ = pl.audit(None, n_scen, 9001, params, upload=False)
+        st, dist = pl.audit(None, n_scen, 9001, params, upload=False)
         s = pl.stats()
         stats.append(s)
         fsteps += s["flight_steps"]
@@ -231,6 +231,14 @@ def audit_bench(PA, args):
     log(f"{args.steps} audits of {n_gen} x {n_scen} in {wall:.3f} s; best mean {st[best, mean_col]:.2f} m "
         f"(design {best}), failures {int(st[best, PA.AUDIT_FIELDS.index('failures')])}")
     value = fsteps / 600.0 / wall
+    rank = surrogate_ranking(PA, genomes, res, dist, params)
+    log(f"surrogate ranking of the audited designs: {rank}")
+    cpu, f64 = None, None
+    if args.cpu_seconds > 0:
+        wc5 = dict(gust=5.0, warp_deg=1.0, wind_sd=2.0, max_steps=12000, nseeds=n_scen)
+        cpu = cpu_baseline(genomes, res, wc5, args.cpu_seconds)
+        f64 = audit_fp64_check(genomes, res, best, dist)
+        log(f"cpu baseline {cpu['value']:.1f} pfs/s; FP64 check {f64}")
     out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * wall / args.steps, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -238,8 +246,64 @@ def audit_bench(PA, args):
                                   "s*7919), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap",
                       "designs": n_gen, "scenarios": n_scen, "max_steps": 12000, "parallelism": "single GPU"},
            "audits_per_hour": round(3600.0 * args.steps / wall, 1), "roofline": roofline_of(stats),
-           "cpu_baseline": None}
+           "cpu_baseline": cpu, "fp64_check": f64, "surrogate_ranking": rank}
     print(json.dumps(out), flush=True)
+
+
+def robust_stats(samples):
+    """Per design, from its scenario distances: [mean, CVaR of the worst quarter, worst,
+    all-oversize flag] (the surrogate's targets) and the robust score 0.65 mean + 0.25 CVaR
+    + 0.10 worst (sim.cu:340-357)."""
+    x = np.asarray(samples, np.float64)
+    srt = np.sort(x, axis=1)
+    tail = (x.shape[1] + 3) // 4
+    mean, cvar, worst = x.mean(1), srt[:, :tail].mean(1), srt[:, 0]
+    over = np.all(x == -100.0, axis=1).astype(np.float64)
+    return np.stack([mean, cvar, worst, over], 1).astype(np.float32), 0.65 * mean + 0.25 * cvar + 0.10 * worst
+
+
+def surrogate_ranking(PA, genomes, res, dist, params, n_train=400000, epochs=10):
+    """The audited designs ranked by the surrogate against their sim-judged ranks (robust
+    score over every audit scenario).  The network is trained on sim flights of a separate
+    population from the same family generator (n_train designs x 8 seeds)."""
+    g_tr, r_tr, _ = PA.family_population(n_train, seed=11)
+    seeds = seeds_for(500, 8)
+    pl = PA.Planes(len(g_tr), len(seeds), len(g_tr) * len(seeds), int(r_tr[:, 0].max()), int(r_tr[:, 1].max()))
+    fit, _ = pl.evaluate(g_tr, seeds, params, res=r_tr)
+    pl.close()
+    tg, _ = robust_stats(np.asarray(fit).reshape(len(g_tr), len(seeds)))
+    S = PA.Surrogate(max_batch=4096, seed=3)
+    t0 = time.perf_counter()
+    loss = S.train(g_tr, tg, epochs=epochs, batch=1024, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    pred = S.predict(genomes)[:, 4].astype(np.float64)
+    _, sim = robust_stats(dist)
+    ra = np.argsort(np.argsort(pred, kind="stable"), kind="stable")
+    rb = np.argsort(np.argsort(sim, kind="stable"), kind="stable")
+    top = lambda v, k: set(np.argsort(-v, kind="stable")[:k].tolist())
+    return {"train_pairs": int(len(g_tr)), "train_seeds": int(len(seeds)), "epochs": epochs, "train_s": round(train_s, 2),
+            "final_loss": round(float(loss), 4), "spearman": round(float(np.corrcoef(ra, rb)[0, 1]), 4),
+            "mean_abs_rank_err": round(float(np.mean(np.abs(ra - rb))), 1),
+            "top10_overlap": len(top(pred, 10) & top(sim, 10)), "top100_overlap": len(top(pred, 100) & top(sim, 100)),
+            "sim_best_design": int(np.argmax(sim)), "sim_best_rank_by_surrogate": int(len(sim) - 1 - ra[np.argmax(sim)])}
+
+
+def audit_fp64_check(genomes, res, best, dist, n_scen=256):
+    """The oracle as the checker (beside the CPU baseline): the audit champion's first n_scen
+    scenarios in FP64 against the batched GPU audit's distances, per scenario."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    g = np.ascontiguousarray(genomes[best:best + 1], np.float32)
+    rs = np.ascontiguousarray(res[best:best + 1], np.int32)
+    seeds = (9001 + 7919 * np.arange(n_scen, dtype=np.int64)).astype(np.int32)
+    f64, _, wall = O.fly_res(g, rs, np.zeros(n_scen, np.int32), seeds, nthreads=os.cpu_count() or 1, prec=64,
+                             preset=1, gust=5.0, warp=float(np.deg2rad(1.0)), wsd=2.0, max_steps=12000)
+    gpu = np.asarray(dist[best, :n_scen], np.float64)
+    rel = np.abs(gpu - f64) / np.maximum(np.abs(np.asarray(f64, np.float64)), 1e-3)
+    return {"design": int(best), "scenarios": n_scen, "within_1pct": round(float(np.mean(rel < 0.01)), 4),
+            "median_rel_err": float(f"{np.median(rel):.3g}"), "p99_rel_err": float(f"{np.quantile(rel, 0.99):.3g}"),
+            "fp64_wall_s": round(wall, 1)}
 
 
 def copy_probe(mib):
