@@ -128,24 +128,54 @@ def fp64_check(PA, genomes, res, w, params, n_planes=256):
             "spearman": round(float(np.corrcoef(ra, rb)[0, 1]), 6), "fp64_wall_s": round(wall, 1)}
 
 
+def pretrain_surrogate(PA, S, n_pairs, params, batch=250000, seeds_per=8, epochs=4, lr=2e-3):
+    """Setup (untimed, reported): sim pairs of n_pairs designs from the family generator,
+    each flown under 8 seeds with the workload's scenario preset, in batches; the network
+    trains on their [mean, CVaR25, worst, oversize] targets."""
+    t0 = time.perf_counter()
+    gs, tg = [], []
+    seeds = seeds_for(900, seeds_per)
+    for b0 in range(0, n_pairs, batch):
+        n = min(batch, n_pairs - b0)
+        g, r, _ = PA.family_population(n, seed=31 + b0 // batch)
+        pl = PA.Planes(n, seeds_per, n * seeds_per, int(r[:, 0].max()), int(r[:, 1].max()))
+        fit, _ = pl.evaluate(g, seeds, params, res=r)
+        pl.close()
+        gs.append(g)
+        tg.append(robust_stats(np.asarray(fit).reshape(n, seeds_per))[0])
+    sim_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = S.train(np.concatenate(gs), np.concatenate(tg), epochs=epochs, batch=4096, lr=lr)
+    return {"pairs": n_pairs, "seeds_per_design": seeds_per, "source": "family generator, config preset",
+            "sim_s": round(sim_s, 2), "train_s": round(time.perf_counter() - t0, 2), "epochs": epochs,
+            "final_loss": round(float(loss), 4)}
+
+
 def pool_ab(PA, args):
     """C4: GA generations/hour with surrogate pre-screening vs the plain batched sim.
     plain: every generation flies the whole pool of mutants; surrogate: the network scores
     the pool and only the elite + best-predicted candidates (plus an exploration sample)
     are flown (n_fly); both keep the best simulated genomes as the next elite and report
-    the holdout champion. The surrogate arm's first generation is a plain one whose flights
-    bootstrap the network (its time counts). Champion fitness = best holdout score so far."""
+    the holdout champion. The network is pretrained on args.pretrain sim pairs (setup,
+    untimed, cost reported); with --pretrain 0 the surrogate arm's first generation is a
+    plain one whose flights bootstrap the network (its time counts).  Champion fitness =
+    best holdout score so far."""
     n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
     params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
     g0, _, _ = PA.family_population(n_elite, seed=0)
     out = {}
+    pre = None
     for mode in ("plain", "surrogate"):
         ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
                        sigma_frac=0.05, seed=7)
         ga.set_elite(g0)
         S = PA.Surrogate(max_batch=args.mlp_batch, seed=3) if mode == "surrogate" else None
         gens = []
-        if S is not None:
+        if S is not None and args.pretrain > 0:
+            pre = pretrain_surrogate(PA, S, args.pretrain, params)
+            log(f"[surrogate] pretraining: {pre}")
+            ga.set_explore(args.explore)
+        elif S is not None:
             ga.set_explore(args.explore)
             ga.set_collect(True)
             r = ga.generation(0, params, None)
@@ -182,7 +212,8 @@ def pool_ab(PA, args):
                                    f"surrogate flies the top {n_fly} ({args.explore:.0%} exploration); 12 environments, "
                                    f"gusts <= 4 m/s RMS, 12 s cap",
                        "parallelism": "single GPU"},
-            "plain": out["plain"], "surrogate": out["surrogate"], "speedup_gens_per_hour": round(ratio, 2),
+            "plain": out["plain"], "surrogate": out["surrogate"], "pretraining": pre,
+            "speedup_gens_per_hour": round(ratio, 2),
             "champion_ratio": round(champ, 4)}
     print(json.dumps(line), flush=True)
 
@@ -339,6 +370,8 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=4, help="c4: surrogate epochs per generation")
     ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
+    ap.add_argument("--pretrain", type=int, default=2_000_000,
+                    help="c4: sim pairs the surrogate is pretrained on (setup; 0: bootstrap generation instead)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
                     help="c1-c3: flight-kernel mapping per bucket from the autotuner (setup, untimed; "

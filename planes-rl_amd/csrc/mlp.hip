@@ -9,8 +9,8 @@
 // Every GEMM is one kernel, gemm_bf16<TA, TB, EPI>: a wave computes one 16x16 output tile,
 // stepping K by 32 with one MFMA per step; the epilogue either applies bias + ReLU and
 // stores BF16 (forward), multiplies by the ReLU mask of the layer input and stores BF16
-// (input gradients), stores FP32, or adds FP32 partial sums atomically (weight gradients,
-// split over the batch).
+// (input gradients), stores FP32, or stores FP32 partial products per batch slice (weight
+// gradients, then summed in slice order: training is deterministic).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -83,9 +83,8 @@ __global__ __launch_bounds__(64) void gemm_bf16(const __bf16* __restrict__ A, in
 
 // genomes [n x 35] f32 -> X [n_pad x 64] bf16, genes scaled to [0, 1] by the bounds
 __global__ void k_embed(const float* __restrict__ g, int n, int n_pad, const float* __restrict__ lo,
-                        const float* __restrict__ inv_range, __bf16* __restrict__ X, float* __restrict__ loss) {
+                        const float* __restrict__ inv_range, __bf16* __restrict__ X) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (loss && i == 0) *loss = 0.0f;  // the batch's loss accumulator (k_loss adds to it)
     if (i >= n_pad * K0) return;
     int r = i / K0, k = i - r * K0;
     float v = 0.0f;
@@ -95,7 +94,7 @@ __global__ void k_embed(const float* __restrict__ g, int n, int n_pad, const flo
 
 // loss and output gradient.  out [n_pad x 16] f32, targets [n x 4] raw (m, m, m, 0/1)
 __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, int n, int n_pad, float inv_n,
-                       __bf16* __restrict__ dout, float* __restrict__ dout_f, float* __restrict__ loss) {
+                       __bf16* __restrict__ dout, float* __restrict__ dout_f, float* __restrict__ loss_part) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;  // every lane stays for the wave reduction
     float l = 0.0f;
     float d[NOUT] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -123,7 +122,15 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
         }
     }
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l * inv_n);
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = l;  // one 64-row block per partial
+}
+
+// mean loss = the partials summed in a fixed order (lane-strided, then a butterfly), times 1/n
+__global__ void k_loss_fin(const float* __restrict__ loss_part, int nblk, float inv_n, float* __restrict__ loss) {
+    float l = 0.0f;
+    for (int b = threadIdx.x; b < nblk; b += 64) l += loss_part[b];
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if (threadIdx.x == 0) *loss = l * inv_n;
 }
 
 // bias gradient = column sums of a bf16 [rows x cols] gradient, in two deterministic passes:
@@ -228,6 +235,7 @@ struct pr_mlp {
     __bf16* dout = nullptr;
     float* dout_f = nullptr;
     float* loss = nullptr;
+    float* loss_part = nullptr;  // [max_batch / 64] per-block loss sums
     float* lo = nullptr;
     float* inv_range = nullptr;
     float* colsum_part = nullptr;  // [COLSUM_SLICES x 256] bias-gradient partial sums
@@ -278,13 +286,15 @@ static int forward(pr_mlp* q, int B, bool keep_out) {
 static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, float* loss_out) {
     const int B = pad32(n);
     hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes, n, B, q->lo, q->inv_range,
-                       q->X, q->loss);
+                       q->X);
     MCK(hipGetLastError());
     int rc = forward(q, B, true);
     if (rc) return rc;
     // every gradient entry is written below (weights by k_sum_splits, biases by k_colsum_fin)
     hipLaunchKernelGGL(k_loss, dim3((B + 63) / 64), dim3(64), 0, q->s, q->out, d_targets, n, B, 1.0f / n, q->dout,
-                       q->dout_f, q->loss);
+                       q->dout_f, q->loss_part);
+    MCK(hipGetLastError());
+    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, q->s, q->loss_part, (B + 63) / 64, 1.0f / n, q->loss);
     MCK(hipGetLastError());
     // backward, layer 4 .. 1
     const __bf16* dY = q->dout;
@@ -369,6 +379,7 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     MALLOC(q->dout, 2 * (size_t)B * NOP);
     MALLOC(q->dout_f, 4 * (size_t)B * NOP);
     MALLOC(q->loss, 4);
+    MALLOC(q->loss_part, 4 * (size_t)(B / 64 + 1));
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
     MALLOC(q->colsum_part, 4 * (size_t)COLSUM_SLICES * HID);
@@ -402,7 +413,7 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
 void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
-    void* bufs[] = {q->w, q->g, q->m, q->v, q->wb, q->X, q->out, q->dout, q->dout_f, q->loss, q->lo, q->inv_range,
+    void* bufs[] = {q->w, q->g, q->m, q->v, q->wb, q->X, q->out, q->dout, q->dout_f, q->loss, q->loss_part, q->lo, q->inv_range,
                     q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2], q->colsum_part,
                     q->wgrad_part};
     for (void* b : bufs) if (b) (void)hipFree(b);
@@ -436,7 +447,7 @@ int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
         const int nb = std::min(q->max_batch, n - s0);
         const int B = pad32(nb);
         hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes + (size_t)s0 * NIN, nb,
-                           B, q->lo, q->inv_range, q->X, nullptr);
+                           B, q->lo, q->inv_range, q->X);
         MCK(hipGetLastError());
         int rc = forward(q, B, false);
         if (rc) return rc;

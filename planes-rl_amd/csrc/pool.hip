@@ -251,6 +251,8 @@ static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
         p->buf_n = std::min(p->buf_cap, p->buf_n + k);
         s += k;
     }
+    // the network trains on its own (non-blocking) stream: the buffer must be complete first
+    PCK(hipStreamSynchronize(nullptr));
     return 0;
 }
 
@@ -407,6 +409,7 @@ int pr_pool_set_collect(pr_pool* p, int on) {
 
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss) {
     if (!p || !model || p->buf_n < 32) { g_pool_err = "pr_pool_train: model and >= 32 buffered pairs needed"; return -1; }
+    PCK(hipStreamSynchronize(nullptr));  // buffer writes on the null stream, training on the network's stream
     PRC(pr_mlp_train(model, p->d_buf_g, p->d_buf_t, p->buf_n, epochs, pr_mlp_max_batch(model), lr, loss));
     return 0;
 }
