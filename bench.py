@@ -156,61 +156,79 @@ def pool_ab(PA, args):
     plain: every generation flies the whole pool of mutants; surrogate: the network scores
     the pool and only the elite + best-predicted candidates (plus an exploration sample)
     are flown (n_fly); both keep the best simulated genomes as the next elite and report
-    the holdout champion. The network is pretrained on args.pretrain sim pairs (setup,
-    untimed, cost reported); with --pretrain 0 the surrogate arm's first generation is a
-    plain one whose flights bootstrap the network (its time counts).  Champion fitness =
-    best holdout score so far."""
+    the holdout champion (best holdout score so far).  The network is pretrained once on
+    args.pretrain sim pairs (setup, untimed, cost reported); each surrogate run starts from
+    those weights with a fresh optimizer and fine-tunes on its own flights.  With
+    --pretrain 0 the surrogate arm's first generation is instead a plain one whose flights
+    bootstrap the network (its time counts).  Both arms run args.reps GA seeds; the line
+    reports their totals and means (single runs differ by ~0.5 m of champion score)."""
     n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
     params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
     g0, _, _ = PA.family_population(n_elite, seed=0)
+    pretrained = args.pretrain > 0
+    lr = args.lr if args.lr is not None else (1e-4 if pretrained else 2e-3)
+    tepochs = args.train_epochs if args.train_epochs is not None else (2 if pretrained else 4)
+    pre, p0 = None, None
+    if pretrained:
+        S = PA.Surrogate(max_batch=args.mlp_batch, seed=3)
+        pre = pretrain_surrogate(PA, S, args.pretrain, params)
+        p0 = S.params()
+        log(f"[surrogate] pretraining: {pre}")
+    runs = {"plain": [], "surrogate": []}
+    for rep in range(args.reps):
+        for mode in ("plain", "surrogate"):
+            ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
+                           sigma_frac=0.05, seed=7 + rep)
+            ga.set_elite(g0)
+            S = None
+            gens = []
+            if mode == "surrogate":
+                S = PA.Surrogate(max_batch=args.mlp_batch, seed=3)
+                ga.set_explore(args.explore)
+                if pretrained:
+                    S.set_params(p0)
+                else:
+                    ga.set_collect(True)
+                    r = ga.generation(0, params, None)
+                    ga.set_collect(False)
+                    t0 = time.perf_counter()
+                    r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=lr)
+                    r["wall_ms"] += 1000 * (time.perf_counter() - t0)
+                    gens.append(r)
+                    log(f"[{mode} {rep}] gen 0 (bootstrap): best {r['best']:.3f} holdout {r['holdout']:.3f} "
+                        f"{r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
+            for gen in range(len(gens), args.gens):
+                r = ga.generation(gen, params, S, train_epochs=tepochs, lr=lr)
+                gens.append(r)
+                log(f"[{mode} {rep}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
+                    f"{r['holdout']:.3f} (best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, "
+                    f"mlp {r['mlp_ms']:.0f}) rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m "
+                    f"oversize acc {r['oversize_acc']:.3f}")
+            runs[mode].append(gens)
     out = {}
-    pre = None
-    for mode in ("plain", "surrogate"):
-        ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
-                       sigma_frac=0.05, seed=7)
-        ga.set_elite(g0)
-        S = PA.Surrogate(max_batch=args.mlp_batch, seed=3) if mode == "surrogate" else None
-        gens = []
-        if S is not None and args.pretrain > 0:
-            pre = pretrain_surrogate(PA, S, args.pretrain, params)
-            log(f"[surrogate] pretraining: {pre}")
-            ga.set_explore(args.explore)
-        elif S is not None:
-            ga.set_explore(args.explore)
-            ga.set_collect(True)
-            r = ga.generation(0, params, None)
-            ga.set_collect(False)
-            t0 = time.perf_counter()
-            r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=args.lr)
-            r["wall_ms"] += 1000 * (time.perf_counter() - t0)
-            gens.append(r)
-            log(f"[{mode}] gen 0 (bootstrap): best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
-                f"{r['holdout']:.3f} {r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
-        for gen in range(len(gens), args.gens):
-            r = ga.generation(gen, params, S, train_epochs=args.train_epochs, lr=args.lr)
-            gens.append(r)
-            log(f"[{mode}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout {r['holdout']:.3f} "
-                f"(best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, mlp {r['mlp_ms']:.0f}) "
-                f"flights {r['flights']:.0f} rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m "
-                f"oversize acc {r['oversize_acc']:.3f}")
-        wall = sum(r["wall_ms"] for r in gens) / 1000.0
-        rc = [r["rank_corr"] for r in gens if r["rank_corr"] == r["rank_corr"]]
-        mae = [r["mae"] for r in gens if r["mae"] == r["mae"]]
-        oacc = [r["oversize_acc"] for r in gens if r["oversize_acc"] == r["oversize_acc"]]
-        out[mode] = {"gens": len(gens), "wall_s": round(wall, 3), "gens_per_hour": round(3600.0 * len(gens) / wall, 1),
-                     "champion_holdout": round(gens[-1]["best_holdout"], 4),
-                     "final_elite_mean": round(gens[-1]["elite_mean"], 4), "flights_per_gen": gens[-1]["flights"],
-                     "mean_rank_corr": round(float(np.mean(rc)), 3) if rc else None,
-                     "mean_mae_m": round(float(np.mean(mae)), 3) if mae else None,
-                     "mean_oversize_acc": round(float(np.mean(oacc)), 3) if oacc else None}
+    for mode, reps in runs.items():
+        allg = [r for gens in reps for r in gens]
+        wall = sum(r["wall_ms"] for r in allg) / 1000.0
+        fin = lambda key: [r[key] for r in allg if r[key] == r[key]]
+        champs = [gens[-1]["best_holdout"] for gens in reps]
+        out[mode] = {"gens": len(allg), "reps": len(reps), "wall_s": round(wall, 3),
+                     "gens_per_hour": round(3600.0 * len(allg) / wall, 1),
+                     "champion_holdout_mean": round(float(np.mean(champs)), 4),
+                     "champion_holdout": [round(c, 4) for c in champs], "flights_per_gen": allg[-1]["flights"],
+                     "mean_rank_corr": round(float(np.mean(fin("rank_corr"))), 3) if fin("rank_corr") else None,
+                     "mean_mae_m": round(float(np.mean(fin("mae"))), 3) if fin("mae") else None,
+                     "mean_oversize_acc": round(float(np.mean(fin("oversize_acc"))), 3) if fin("oversize_acc") else None,
+                     "mlp_ms_per_gen": round(float(np.mean([r["mlp_ms"] for r in allg])), 1)}
     ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
-    champ = out["surrogate"]["champion_holdout"] / out["plain"]["champion_holdout"]
+    champ = out["surrogate"]["champion_holdout_mean"] / out["plain"]["champion_holdout_mean"]
     line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",
-            "n_gpus": 1, "steps": args.gens, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+            "n_gpus": 1, "steps": args.gens * args.reps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 sim, bf16 surrogate", "data": "synthetic",
             "config": {"workload": f"C4: pool {n_pool} Gaussian mutants (sigma 5% of range) of a {n_elite} elite, "
                                    f"surrogate flies the top {n_fly} ({args.explore:.0%} exploration); 12 environments, "
-                                   f"gusts <= 4 m/s RMS, 12 s cap",
+                                   f"gusts <= 4 m/s RMS, 12 s cap; {args.reps} GA seeds x {args.gens} generations "
+                                   f"per arm",
+                       "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": args.buffer},
                        "parallelism": "single GPU"},
             "plain": out["plain"], "surrogate": out["surrogate"], "pretraining": pre,
             "speedup_gens_per_hour": round(ratio, 2),
@@ -367,8 +385,11 @@ def main():
     ap.add_argument("--buffer", type=int, default=300_000, help="c4: surrogate replay buffer")
     ap.add_argument("--explore", type=float, default=0.3, help="c4: fraction of the fly set drawn at random")
     ap.add_argument("--boot-epochs", type=int, default=16, help="c4: surrogate epochs on the bootstrap generation")
-    ap.add_argument("--train-epochs", type=int, default=4, help="c4: surrogate epochs per generation")
-    ap.add_argument("--lr", type=float, default=2e-3, help="c4: surrogate learning rate (Adam)")
+    ap.add_argument("--train-epochs", type=int, default=None,
+                    help="c4: surrogate epochs per generation (default 2 after pretraining, else 4)")
+    ap.add_argument("--lr", type=float, default=None,
+                    help="c4: surrogate learning rate (Adam; default 1e-4 fine-tuning after pretraining, else 2e-3)")
+    ap.add_argument("--reps", type=int, default=3, help="c4: GA seeds per arm")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
     ap.add_argument("--pretrain", type=int, default=2_000_000,
                     help="c4: sim pairs the surrogate is pretrained on (setup; 0: bootstrap generation instead)")
