@@ -399,6 +399,8 @@ def main():
                          "default on for c3, whose configuration asks for it)")
     ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=False,
                     help="c1-c3: each evaluation's flight sequence (k_init, chunks, compactions) as a replayed hipGraph")
+    ap.add_argument("--tail-mode", type=int, default=None, choices=[0, 1, 2],
+                    help="c1-c3: tail-mapping mode of every flight context (pr_set_tail_mapping)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="c1-c3: finish each generation's holdout before the next generation starts")
@@ -439,6 +441,8 @@ def main():
         E.set_fused(False)
     if args.graph:
         E.set_graph(True)
+    if args.tail_mode is not None:
+        E.set_tail_mapping(args.tail_mode)
     tuned = None
     if args.autotune if args.autotune is not None else w.get("autotune", False):
         # one-time setup outside the timed region: the mapping per triangle-count bucket from

@@ -505,6 +505,11 @@ class Generation:
         for which in (0, 1, 2):
             _ck(LIB.pr_set_graph(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_graph")
 
+    def set_tail_mapping(self, mode):
+        """Tail-mapping mode (pr_set_tail_mapping) for every flight context of the generation."""
+        for which in (0, 1, 2):
+            _ck(LIB.pr_set_tail_mapping(LIB.pr_gen_context(self.h, which), int(mode)), "pr_set_tail_mapping")
+
     def set_buckets(self, nt_max, L, P):
         """Flight-kernel mapping per triangle-count bucket (e.g. from autotune) for every
         flight context of the generation."""
