@@ -33,7 +33,7 @@ WORKLOADS = {
     # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
     "c1": dict(pop=64, nseeds=8, max_steps=4800, gust=2.0, warp_deg=1.0, wind_sd=2.0,
                desc="64 genomes x 8 seeded launch scenarios, config preset, 8 s cap"),
-    "c2": dict(pop=10000, nseeds=8, max_steps=9000, gust=2.0, warp_deg=1.0, wind_sd=2.0,
+    "c2": dict(pop=10000, nseeds=8, max_steps=9000, gust=2.0, warp_deg=1.0, wind_sd=2.0, graph=True,
                desc="one GA generation: pop 10000 x 8 seeds, config preset, 15 s cap, top-8 x 128 holdout"),
     "c3": dict(pop=100000, nseeds=12, max_steps=7200, gust=4.0, warp_deg=1.0, wind_sd=2.0, families=True, autotune=True,
                desc="one GA generation: pop 100000 x 12 environments, mixed families (delta/elliptical/plank/"
@@ -397,8 +397,9 @@ def main():
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
                     help="c1-c3: flight-kernel mapping per bucket from the autotuner (setup, untimed; "
                          "default on for c3, whose configuration asks for it)")
-    ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=False,
-                    help="c1-c3: each evaluation's flight sequence (k_init, chunks, compactions) as a replayed hipGraph")
+    ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=None,
+                    help="c1-c3: each evaluation's flight sequence (k_init, chunks, compactions) as a replayed hipGraph "
+                         "(default on for c2, whose configuration asks for it; the roofline then times whole graphs)")
     ap.add_argument("--tail-mode", type=int, default=None, choices=[0, 1, 2],
                     help="c1-c3: tail-mapping mode of every flight context (pr_set_tail_mapping)")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
@@ -439,6 +440,8 @@ def main():
     E.upload(genomes, res)
     if args.unfused:
         E.set_fused(False)
+    if args.graph is None:
+        args.graph = w.get("graph", False)
     if args.graph:
         E.set_graph(True)
     if args.tail_mode is not None:
@@ -552,7 +555,8 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"{args.workload.upper()}: {w['desc']}", "pop": w["pop"], "seeds": w["nseeds"],
                       "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU",
-                      "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table"},
+                      "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table",
+                      "launch": "hipGraph replay per evaluation" if args.graph else "direct launches"},
            "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
            "roofline": roof, "cpu_baseline": cpu, "fp64_check": f64}
     print(json.dumps(out), flush=True)
