@@ -397,6 +397,9 @@ def main():
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
                     help="c1-c3: flight-kernel mapping per bucket from the autotuner (setup, untimed; "
                          "default on for c3, whose configuration asks for it)")
+    ap.add_argument("--buckets", default=None,
+                    help="c1-c3: flight-kernel mapping per bucket, as the autotuner's JSON "
+                         "({\"nt_max\": [...], \"L\": [...], \"P\": [...]}), inline or a file path; replaces the autotuner")
     ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=None,
                     help="c1-c3: each evaluation's flight sequence (k_init, chunks, compactions) as a replayed hipGraph "
                          "(default on for c2, whose configuration asks for it; the roofline then times whole graphs)")
@@ -447,7 +450,12 @@ def main():
     if args.tail_mode is not None:
         E.set_tail_mapping(args.tail_mode)
     tuned = None
-    if args.autotune if args.autotune is not None else w.get("autotune", False):
+    if args.buckets:
+        spec = args.buckets if args.buckets.lstrip().startswith("{") else open(args.buckets).read()
+        b = json.loads(spec)
+        tuned = (list(b["nt_max"]), list(b["L"]), list(b["P"]))
+        E.set_buckets(*tuned)
+    elif args.autotune if args.autotune is not None else w.get("autotune", False):
         # one-time setup outside the timed region: the mapping per triangle-count bucket from
         # timing every compiled (L, P) on a sample of this population's flights
         t0 = time.perf_counter()

@@ -111,23 +111,34 @@ static int wide_for(int nt_max, int* Lw, int* Pw) {
 }
 
 static int default_buckets(pr_ctx* c) {
-    // smallest group whose lanes hold the flight's panels in <= 24 slots each.  A wave
-    // runs as many slots as its fullest group needs, so the L = 16 and 32 ranges are split
-    // in two: flights that share a wave then differ less in triangle count (measured +2.3 %
-    // on the 100k-genome 200..900-triangle mix)
-    static const int table[][3] = {{192, 8, 24},  {288, 16, 24},  {384, 16, 24},  {576, 32, 24},
-                                   {768, 32, 24}, {1024, 64, 16}, {1536, 64, 24}, {2048, 64, 32}};
+    // smallest group whose lanes hold the flight's panels: up to 24 slots per lane in VGPRs,
+    // 33 with the LDS slots (P = 33: 22 in VGPRs + 11 in LDS) -- narrower groups put more
+    // flights in a wave and share its per-step rigid-body work among them (measured +5.8 %
+    // on the 100k-genome 200..900-triangle mix).  A wave runs as many slots as its fullest
+    // group needs, so the VGPR-only L = 16 and 32 ranges are split in two (split = 1): the
+    // flights sharing a wave then differ less in triangle count.  Split entries are dropped
+    // first when the resolution needs more than NB_MAX buckets.
+    static const int table[][4] = {{192, 8, 24, 0},  {264, 8, 33, 0},  {320, 16, 24, 1},  {384, 16, 24, 0},
+                                   {528, 16, 33, 0}, {640, 32, 24, 1}, {768, 32, 24, 0},  {1024, 32, 33, 0},
+                                   {1536, 64, 24, 0}, {2048, 64, 32, 0}};
     int nv, nt;
     mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
-    c->buckets.clear();
+    std::vector<const int*> rows;
     for (const auto& t : table) {
+        rows.push_back(t);
+        if (t[0] >= nt) break;
+    }
+    if (rows.back()[0] < nt) return fail(-1, "mesh resolution exceeds the largest flight-kernel bucket (2048 triangles)");
+    for (size_t i = 0; i < rows.size() && rows.size() > (size_t)NB_MAX;)
+        if (rows[i][3]) rows.erase(rows.begin() + i);
+        else i++;
+    c->buckets.clear();
+    for (const int* t : rows) {
         Bucket b = {t[0], t[1], t[2], 0, 0};
         wide_for(b.nt_max, &b.Lw, &b.Pw);
         if (b.Lw == b.L && b.Pw == b.P) b.Lw = 0;
         c->buckets.push_back(b);
-        if (b.nt_max >= nt) break;
     }
-    if (c->buckets.back().nt_max < nt) return fail(-1, "mesh resolution exceeds the largest flight-kernel bucket (2048 triangles)");
     return 0;
 }
 

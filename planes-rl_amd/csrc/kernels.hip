@@ -341,17 +341,33 @@ struct WindLane {  // this lane's gust term (zero weight on lane 3 of each 8)
     float ax, ay, az, bt, cs, wg;
 };
 
+// Panel slots per lane beyond which the extra slots live in LDS: P <= 24 slots sit in VGPRs
+// (7 per slot; 24 is what 256 VGPRs hold beside the flight state at two waves per SIMD);
+// a variant with P > 24 keeps 22 slots in VGPRs (room for the LDS loads in flight) and the
+// other P - 22 in the wave's LDS, one column per lane: float [P - 22][7][64].
+constexpr int PR_LDS_PV = 22;
+__host__ __device__ constexpr int lds_slots(int P) { return P > 24 ? P - PR_LDS_PV : 0; }
+// LDS bytes of one wave's panel slots; 11 slots (19.7 KB) keep eight waves per CU within
+// the 160 KB of LDS, the same occupancy the VGPRs allow
+__host__ __device__ constexpr int lds_bytes(int P) { return lds_slots(P) * 7 * 64 * 4; }
+
 // One group's flight: setup from the flight's constants/state/panels, then 600 Hz steps.
 template <int L, int P>
 struct Flyer {
     static_assert(L >= 8, "the gust terms need 8 lanes per group");
+    static constexpr int PQ = lds_slots(P);  // slots in LDS
+    static constexpr int PV = P - PQ;        // slots in VGPRs
+    static_assert(PQ == 0 || PV % 2 == 0, "the LDS slots continue the VGPR slots' pairs");
     Body B;
     WindLane W;
-    float rx[P], ry[P], rz[P], nx[P], ny[P], nz[P], kk[P];  // kk = cf rho A / 2
+    float rx[PV], ry[PV], rz[PV], nx[PV], ny[PV], nz[PV], kk[PV];  // kk = cf rho A / 2
+    float* lds;  // this lane's LDS column (PQ x 7 rows, 64 floats apart); only when PQ > 0
     int np;  // panel slots this group uses per lane: ceil(nt / L) <= P
     bool bad;
 
-    __device__ __forceinline__ void setup(const FlyArgs& a, int fid, int j) {
+    // smem: the wave's LDS panel block (lds_bytes(P)); unused when PQ == 0
+    __device__ __forceinline__ void setup(const FlyArgs& a, int fid, int j, float* smem) {
+        lds = smem + (threadIdx.x & 63);
         const FlightConst& fc = a.fc[fid];
         const FlightState& st = a.state[fid];
         const int gi = fc.genome;
@@ -388,15 +404,20 @@ struct Flyer {
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const int t = j + p * L;
+            float v[7] = {0, 0, 0, 0, 0, 1, 0};  // padding: zero area, so zero force
             if (t < B.nt) {
                 const size_t o = (size_t)off + t;
                 vec3 rb = vsub(V(a.pcx[o], a.pcy[o], a.pcz[o]), shift);
                 vec3 nb = warped_normal(rb, V(a.pnx[o], a.pny[o], a.pnz[o]), wp, wr);
-                rx[p] = rb.x; ry[p] = rb.y; rz[p] = rb.z;
-                nx[p] = nb.x; ny[p] = nb.y; nz[p] = nb.z;
-                kk[p] = (hr * a.pA[o]) * cf;
+                v[0] = rb.x; v[1] = rb.y; v[2] = rb.z;
+                v[3] = nb.x; v[4] = nb.y; v[5] = nb.z;
+                v[6] = (hr * a.pA[o]) * cf;
+            }
+            if (p < PV) {
+                rx[p] = v[0]; ry[p] = v[1]; rz[p] = v[2]; nx[p] = v[3]; ny[p] = v[4]; nz[p] = v[5]; kk[p] = v[6];
             } else {
-                rx[p] = 0; ry[p] = 0; rz[p] = 0; nx[p] = 0; ny[p] = 0; nz[p] = 1; kk[p] = 0;
+#pragma unroll
+                for (int k = 0; k < 7; k++) lds[((p - PV) * 7 + k) * 64] = v[k];
             }
         }
     }
@@ -417,33 +438,52 @@ struct Flyer {
     // contributes < 1e-9 N.
     // FIRST: the lane's first panel initialises the sums instead of adding to zero.
     template <bool FIRST = false>
-    __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
-                                                float g, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
-                                                float& Tz) const {
-        const float vrx = __builtin_fmaf(wy, rz[p], __builtin_fmaf(-wz, ry[p], vbx));
-        const float vry = __builtin_fmaf(wz, rx[p], __builtin_fmaf(-wx, rz[p], vby));
-        const float vrz = __builtin_fmaf(wx, ry[p], __builtin_fmaf(-wy, rx[p], vbz));
+    __device__ __forceinline__ static void panel_force_at(float prx, float pry, float prz, float pnx, float pny,
+                                                          float pnz, float pk, float vbx, float vby, float vbz,
+                                                          float wx, float wy, float wz, float g, float& Fx,
+                                                          float& Fy, float& Fz, float& Tx, float& Ty, float& Tz) {
+        const float vrx = __builtin_fmaf(wy, prz, __builtin_fmaf(-wz, pry, vbx));
+        const float vry = __builtin_fmaf(wz, prx, __builtin_fmaf(-wx, prz, vby));
+        const float vrz = __builtin_fmaf(wx, pry, __builtin_fmaf(-wy, prx, vbz));
         const float V2 = __builtin_fmaf(vrz, vrz, __builtin_fmaf(vry, vry, vrx * vrx));
-        const float d = __builtin_fmaf(vrz, nz[p], __builtin_fmaf(vry, ny[p], vrx * nx[p]));
+        const float d = __builtin_fmaf(vrz, pnz, __builtin_fmaf(vry, pny, vrx * pnx));
         const float vm = __builtin_amdgcn_sqrtf(V2);
         const float sq = __builtin_amdgcn_sqrtf(fabsf(__builtin_fmaf(-d, d, V2)));
         const float e = __builtin_fmaf(0.95f, sq, fabsf(d));
-        const float c1 = (kk[p] * d) * __builtin_fmaf(g, e, vm);
-        const float c2 = -(kk[p] * vm);
-        const float fx = c1 * nx[p] + c2 * vrx;
-        const float fy = c1 * ny[p] + c2 * vry;
-        const float fz = c1 * nz[p] + c2 * vrz;
+        const float c1 = (pk * d) * __builtin_fmaf(g, e, vm);
+        const float c2 = -(pk * vm);
+        const float fx = c1 * pnx + c2 * vrx;
+        const float fy = c1 * pny + c2 * vry;
+        const float fz = c1 * pnz + c2 * vrz;
         if (FIRST) {
             Fx = fx; Fy = fy; Fz = fz;
-            Tx = __builtin_fmaf(ry[p], fz, -rz[p] * fy);
-            Ty = __builtin_fmaf(rz[p], fx, -rx[p] * fz);
-            Tz = __builtin_fmaf(rx[p], fy, -ry[p] * fx);
+            Tx = __builtin_fmaf(pry, fz, -prz * fy);
+            Ty = __builtin_fmaf(prz, fx, -prx * fz);
+            Tz = __builtin_fmaf(prx, fy, -pry * fx);
         } else {
             Fx += fx; Fy += fy; Fz += fz;
-            Tx = __builtin_fmaf(ry[p], fz, __builtin_fmaf(-rz[p], fy, Tx));
-            Ty = __builtin_fmaf(rz[p], fx, __builtin_fmaf(-rx[p], fz, Ty));
-            Tz = __builtin_fmaf(rx[p], fy, __builtin_fmaf(-ry[p], fx, Tz));
+            Tx = __builtin_fmaf(pry, fz, __builtin_fmaf(-prz, fy, Tx));
+            Ty = __builtin_fmaf(prz, fx, __builtin_fmaf(-prx, fz, Ty));
+            Tz = __builtin_fmaf(prx, fy, __builtin_fmaf(-pry, fx, Tz));
         }
+    }
+
+    // slot p < PV: the panel held in VGPRs
+    template <bool FIRST = false>
+    __device__ __forceinline__ void panel_force(int p, float vbx, float vby, float vbz, float wx, float wy, float wz,
+                                                float g, float& Fx, float& Fy, float& Fz, float& Tx, float& Ty,
+                                                float& Tz) const {
+        panel_force_at<FIRST>(rx[p], ry[p], rz[p], nx[p], ny[p], nz[p], kk[p], vbx, vby, vbz, wx, wy, wz, g, Fx, Fy,
+                              Fz, Tx, Ty, Tz);
+    }
+
+    // slot PV + q: the panel read from this lane's LDS column
+    __device__ __forceinline__ void panel_force_lds(int q, float vbx, float vby, float vbz, float wx, float wy,
+                                                    float wz, float g, float& Fx, float& Fy, float& Fz, float& Tx,
+                                                    float& Ty, float& Tz) const {
+        const float* c = lds + q * 7 * 64;
+        panel_force_at<false>(c[0], c[64], c[128], c[192], c[256], c[320], c[384], vbx, vby, vbz, wx, wy, wz, g, Fx,
+                              Fy, Fz, Tx, Ty, Tz);
     }
 
     // One step (sim.cu:129-205) for the group; returns true when the flight ended (ground,
@@ -471,12 +511,20 @@ struct Flyer {
         // instruction-level parallelism to interleave (at most one padded slot evaluated);
         // the first pair always runs (a slot past the lane's triangles adds exactly zero)
         panel_force<true>(0, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
-        if (P > 1) panel_force(1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+        if (PV > 1) panel_force(1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
 #pragma unroll
-        for (int p = 2; p < P; p += 2) {
+        for (int p = 2; p < PV; p += 2) {
             if (p < npw) {
                 panel_force(p, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
-                if (p + 1 < P) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+                if (p + 1 < PV) panel_force(p + 1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+            }
+        }
+        // LDS slots (PV is even, so the pairs continue the VGPR slots' pairing)
+#pragma unroll
+        for (int q = 0; q < PQ; q += 2) {
+            if (PV + q < npw) {
+                panel_force_lds(q, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
+                if (q + 1 < PQ) panel_force_lds(q + 1, vbx, vby, vbz, wx, wy, wz, g, Fx, Fy, Fz, Tx, Ty, Tz);
             }
         }
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
@@ -552,7 +600,7 @@ struct Flyer {
 // id if the flight goes on, else -1.  Which code path flies a position depends only on the
 // position, so the results do not depend on the order in which groups claim.
 template <int L, int P>
-__device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head) {
+__device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head, float* smem) {
     const int lane = threadIdx.x & 63;
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
@@ -574,7 +622,7 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* 
                 }
                 k = __shfl(k, gbase);
                 if (k >= hi) exhausted = true;
-                else { pos = k; F.setup(a, a.list_in[k], j); active = true; }
+                else { pos = k; F.setup(a, a.list_in[k], j, smem); active = true; }
             }
             npw = wave_slots<P>(active ? F.np : 0);
         }
@@ -611,12 +659,23 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
     // flight count decides which one works, so the choice needs no host synchronisation
     if (n_in < a.run_lo || n_in > a.run_hi) return;
-    fly_list<L, P>(a, 0, n_in, a.head);
+    __shared__ float smem[lds_bytes(P) > 0 ? lds_bytes(P) / 4 : 1];
+    fly_list<L, P>(a, 0, n_in, a.head, smem);
 }
 
 // Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
-#define PR_FUSED(X) \
-    X(0, 8, 12) X(1, 8, 24) X(2, 16, 12) X(3, 16, 24) X(4, 32, 12) X(5, 32, 24) X(6, 64, 8) X(7, 64, 16)
+#define PR_FUSED(X)                                                                               \
+    X(0, 8, 12) X(1, 8, 24) X(2, 16, 12) X(3, 16, 24) X(4, 32, 12) X(5, 32, 24) X(6, 64, 8) X(7, 64, 16) \
+    X(8, 8, 33) X(9, 16, 33) X(10, 32, 33)
+
+// LDS of the fused kernel: the largest panel block among its variants
+constexpr int fused_lds_bytes() {
+    int m = 0;
+#define LDSMAX(ID, LL, PP) m = lds_bytes(PP) > m ? lds_bytes(PP) : m;
+    PR_FUSED(LDSMAX)
+#undef LDSMAX
+    return m;
+}
 
 // One chunk of every bucket in one launch: each wave drains the buckets' lists, largest
 // triangle counts first, flying each bucket with its own (L, P) code path: list positions
@@ -627,6 +686,7 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
 // flights than thr[b]: otherwise its last flights run beside those buckets' flights
 // anyway, and the one-per-wave mapping costs more instructions per flight-step.
 __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
+    __shared__ float smem[fused_lds_bytes() > 0 ? fused_lds_bytes() / 4 : 1];
     // flights of the buckets drained after the current one (in this launch)
     int below = 0;
     for (int b = 0; b < A.nb; b++)
@@ -647,7 +707,7 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         if (limit > 0) {
             a.stats = A.stats + 2 * (2 * b);
             switch (A.vid_p[b]) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 2); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 2, smem); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
@@ -656,7 +716,7 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         if (vw >= 0 && limit < n_in) {
             a.stats = A.stats + 2 * (2 * b + 1);
             switch (vw) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, limit, n_in, cb + 3); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, limit, n_in, cb + 3, smem); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
@@ -817,8 +877,9 @@ __global__ __launch_bounds__(64) void k_fly_rec(FlyArgs a, int n_flights, float*
     const int j = lane & (L - 1);
     const int fid = blockIdx.x * (64 / L) + lane / L;
     if (fid >= n_flights) return;  // whole groups leave together
+    __shared__ float smem[lds_bytes(P) > 0 ? lds_bytes(P) / 4 : 1];
     Flyer<L, P> F;
-    F.setup(a, fid, j);
+    F.setup(a, fid, j, smem);
     bool done = a.state[fid].done != 0;
     int rec_i = 0;
     for (int st = 0; st < a.max_steps; st++) {
@@ -912,7 +973,8 @@ static hipError_t fly_one(const FlyArgs& a, int grid, hipStream_t s) {
 
 // compiled (L, P) variants; capacity L*P triangles per flight
 #define PR_VARIANTS(X) \
-    X(8, 12) X(8, 24) X(16, 12) X(16, 24) X(32, 12) X(32, 24) X(64, 8) X(64, 16) X(64, 24) X(64, 32)
+    X(8, 12) X(8, 24) X(16, 12) X(16, 24) X(32, 12) X(32, 24) X(64, 8) X(64, 16) X(64, 24) X(64, 32) \
+    X(8, 33) X(16, 33) X(32, 33)
 
 int fly_variant_count() {
     int n = 0;

@@ -606,7 +606,9 @@ def family_population(n, seed=0, nt_min=200, nt_max=900, biplane_frac=0.25):
 
 
 # ------------------------------------------------------------------ autotuner
-def autotune(genomes, res, seeds, params, ranges=(192, 288, 384, 576, 768, 1024, 1536, 2048), sample_flights=98304,
+NB_MAX = 8  # triangle-count buckets a context holds
+def autotune(genomes, res, seeds, params, ranges=(192, 264, 320, 384, 528, 640, 768, 1024, 1536, 2048),
+             sample_flights=98304,
              steps=600, out_json=None, verbose=False, merge=False, repeats=3):
     """Pick the flight-kernel mapping per triangle-count range: for each range (lo, hi],
     every compiled (L lanes per flight, P panels per lane) variant with L*P >= hi flies
@@ -614,7 +616,12 @@ def autotune(genomes, res, seeds, params, ranges=(192, 288, 384, 576, 768, 1024,
     chunk; the variant with the least kernel time per panel-step wins.  The ranges stay
     separate buckets by default (a wave runs as many panel slots as its fullest flight
     needs, so narrower ranges waste fewer); merge=True joins adjacent ranges with the same
-    winner.  Returns (nt_max, L, P) lists; writes JSON if asked."""
+    winner.  The default ranges end where the variants' capacities L*P do (8/16/32 x 33 =
+    264/528/1056 with the LDS slots, x 24 = 192/384/768 in VGPRs), plus splits of the
+    VGPR-only ranges.  More than 8 buckets (the context's limit) are merged down: adjacent
+    ranges with the same winner first, then a range into the next one (whose mapping
+    covers it) where the sample holds the fewest flights.  Returns (nt_max, L, P) lists;
+    writes JSON if asked."""
     import json
 
     g = np.ascontiguousarray(genomes, np.float32).reshape(-1, N_GENES)
@@ -666,25 +673,31 @@ def autotune(genomes, res, seeds, params, ranges=(192, 288, 384, 576, 768, 1024,
             if best is None or cost < best[0]:
                 best = (cost, L, P)
         pl.close()
-        choice.append((hi, best[1], best[2]))
+        choice.append((hi, best[1], best[2], len(sel)))
         lo = hi
     merged = []
-    for hi, L, P in choice:
-        if merge and merged and merged[-1][1:] == (L, P):
-            merged[-1] = (hi, L, P)
+    for hi, L, P, n in choice:
+        if merge and merged and merged[-1][1:3] == (L, P):
+            merged[-1] = (hi, L, P, merged[-1][3] + n)
         else:
-            merged.append((hi, L, P))
-    merged[-1] = (max(merged[-1][0], nt_cap), merged[-1][1], merged[-1][2])
+            merged.append((hi, L, P, n))
+    merged[-1] = (max(merged[-1][0], nt_cap),) + merged[-1][1:]
     # a context sized for (max_ch, max_sp) needs its last bucket to cover that resolution's
     # triangle bound, not just the sample's largest count
     bound = 8 * max_ch * max_sp + 24 if max_ch and max_sp else 8 * 5 * 8 + 24
     if merged[-1][0] < bound:
-        hi, L, P = merged[-1]
+        hi, L, P, n = merged[-1]
         if L * P >= bound:
-            merged[-1] = (bound, L, P)
+            merged[-1] = (bound, L, P, n)
         else:
             cover = min((v for v in variants if v[0] * v[1] >= bound), key=lambda v: v[0] * v[1])
-            merged.append((bound, cover[0], cover[1]))
+            merged.append((bound, cover[0], cover[1], 0))
+    # a merged range takes the upper range's mapping, which covers both (L*P >= its nt_max)
+    while len(merged) > NB_MAX:
+        same = [i for i in range(len(merged) - 1) if merged[i][1:3] == merged[i + 1][1:3]]
+        i = same[0] if same else min(range(len(merged) - 1), key=lambda k: merged[k][3])
+        hi, L, P, n = merged[i + 1]
+        merged[i:i + 2] = [(hi, L, P, n + merged[i][3])]
     nt_max = [m[0] for m in merged]
     Ls = [m[1] for m in merged]
     Ps = [m[2] for m in merged]

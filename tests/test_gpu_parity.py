@@ -502,6 +502,49 @@ def test_fine_meshes_largest_buckets_vs_fp64(PA, res):
     assert np.mean(rel < 0.01) >= 0.95 and own
 
 
+@pytest.mark.parametrize("res", [(5, 12), (8, 12)])
+def test_lds_panel_slots_vs_vgpr_mappings_and_fp64(PA, res):
+    """Panel slots beyond 22 per lane live in the wave's LDS (variants with P = 33): meshes
+    of 254-258 / 504 triangles at (5, 12) and 402 / 792 at (8, 12) fill the (8, 33),
+    (16, 33) and (32, 33) buckets past their VGPR slots.  Against VGPR-only mappings of
+    the same flights: equal up to float reassociation of the group sums; against the FP64
+    oracle: within 1 %; fused kernel and own launches of the LDS variants bit-identical."""
+    ch, sp = res
+    gs = O.random_genomes(32, 53 + ch)
+    r = np.tile(np.array([[ch, sp]], np.int32), (len(gs), 1))
+    seeds = np.arange(1, 5, dtype=np.int32)
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=2400)
+    nt = np.array([PA.mesh_triangles(g, ch, sp) for g in gs])
+    lds_table = ([264, 528, 1024], [8, 16, 32], [33, 33, 33])
+    vgpr_table = ([384, 768, 1024], [16, 32, 64], [24, 24, 16])
+    out = {}
+    for name, table, fused in (("lds", lds_table, True), ("lds_own", lds_table, False), ("vgpr", vgpr_table, True)):
+        pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=ch, max_sp=sp)
+        pl.set_tail_mapping(False)
+        pl.set_buckets(*table)
+        pl.set_fused(fused)
+        fit, steps = pl.evaluate(gs, seeds, params, res=r)
+        out[name] = (fit.reshape(-1).copy(), steps.reshape(-1).copy(), pl.stats())
+        pl.close()
+    used = [(b["L"], b["P"]) for b in out["lds"][2]["buckets"] if b["flight_steps"] > 0]
+    assert used and all(P == 33 for _, P in used)
+    # every mesh needs more than the 22 VGPR slots per lane in its bucket's group width
+    for n in set(nt):
+        L = next(L for hi, L in zip(lds_table[0], lds_table[1]) if n <= hi)
+        assert (n + L - 1) // L > 22, (n, L)
+    assert np.array_equal(out["lds"][0], out["lds_own"][0]) and np.array_equal(out["lds"][1], out["lds_own"][1])
+    a, b = out["lds"][0], out["vgpr"][0]
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+    gi = np.repeat(np.arange(len(gs)), 4).astype(np.int32)
+    f64, _, _ = O.fly(gs, gi, np.tile(seeds, len(gs)), prec=64, preset=1, gust=2.0, warp=float(params.warp_max),
+                      wsd=2.0, max_steps=2400, ch=ch, sp=sp)
+    rel64 = np.abs(a - f64) / np.maximum(np.abs(f64), 1e-3)
+    print(f"res {res}: nt {sorted(set(nt))}, mappings {used}: {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3 of the "
+          f"VGPR-only mappings, {np.mean(rel64 < 0.01) * 100:.1f}% within 1% of FP64")
+    assert np.mean(rel < 1e-3) >= 0.95 and np.mean(rel < 1e-2) >= 0.97
+    assert np.mean(rel64 < 0.01) >= 0.95
+
+
 def test_pr_sim_is_gpu_sim(PA):
     """pr_sim, the C entry point behind gpu_sim (sim.h:12): vary = 0 flies every plane under
     the one seed, vary = 1 plane p under seed + p * 7919 (32-bit wrap), identical to the
