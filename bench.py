@@ -27,7 +27,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 FLOP_PANEL, FLOP_PLANE = 68, 222  # algorithmic FLOPs per panel-step and per plane-step
 # HBM bytes per k_fly_all launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled
 # by a 1 GiB copy's readings) on this workload; the counters cannot be read inside a timed run
-TRAFFIC = {"c3": (8.11e8, "profiles/k_fly_all_c3_hbm_traffic.md")}  # per population launch
+TRAFFIC = {"c3": (7.94e8, "profiles/k_fly_all_c3_hbm_traffic.md")}  # per population launch
 
 WORKLOADS = {
     # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
@@ -161,7 +161,12 @@ def pool_ab(PA, args):
     those weights with a fresh optimizer and fine-tunes on its own flights.  With
     --pretrain 0 the surrogate arm's first generation is instead a plain one whose flights
     bootstrap the network (its time counts).  Both arms run args.reps GA seeds; the line
-    reports their totals and means (single runs differ by ~0.5 m of champion score)."""
+    reports their totals and means (single runs differ by ~0.5 m of champion score).
+    The plain arm runs args.gens generations, the surrogate arm 3x as many (--sur-gens):
+    champion_ratio compares the champions after args.gens generations each;
+    time_to_plain_champion gives, per GA seed, the first surrogate generation whose
+    champion reaches 99 % of the plain arm's and the wall-time ratio of the two arms to
+    that point (the generations/hour gain at unchanged champion fitness)."""
     n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
     params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
     g0, _, _ = PA.family_population(n_elite, seed=0)
@@ -197,7 +202,8 @@ def pool_ab(PA, args):
                     gens.append(r)
                     log(f"[{mode} {rep}] gen 0 (bootstrap): best {r['best']:.3f} holdout {r['holdout']:.3f} "
                         f"{r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
-            for gen in range(len(gens), args.gens):
+            n_gens = args.gens if mode == "plain" else max(args.gens, args.sur_gens or 3 * args.gens)
+            for gen in range(len(gens), n_gens):
                 r = ga.generation(gen, params, S, train_epochs=tepochs, lr=lr)
                 gens.append(r)
                 log(f"[{mode} {rep}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
@@ -205,6 +211,22 @@ def pool_ab(PA, args):
                     f"mlp {r['mlp_ms']:.0f}) rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m "
                     f"oversize acc {r['oversize_acc']:.3f}")
             runs[mode].append(gens)
+    # time to the plain arm's champion: the first surrogate generation whose champion (best
+    # holdout so far) reaches 99 % of the same GA seed's plain champion after args.gens
+    # generations, and the wall time the two arms took to get there
+    match = []
+    for rep in range(args.reps):
+        pg, sg = runs["plain"][rep], runs["surrogate"][rep]
+        target, wp = pg[-1]["best_holdout"], sum(r["wall_ms"] for r in pg) / 1000.0
+        ws, hit = 0.0, None
+        for i, r in enumerate(sg):
+            ws += r["wall_ms"] / 1000.0
+            if r["best_holdout"] >= 0.99 * target:
+                hit = {"gen": i + 1, "wall_s": round(ws, 3)}
+                break
+        match.append({"plain_champion": round(target, 4), "plain_gens": len(pg), "plain_wall_s": round(wp, 3),
+                      "surrogate_reached_at": hit, "speedup": round(wp / hit["wall_s"], 2) if hit else None})
+    sur_equal = [gens[min(args.gens, len(gens)) - 1]["best_holdout"] for gens in runs["surrogate"]]
     out = {}
     for mode, reps in runs.items():
         allg = [r for gens in reps for r in gens]
@@ -220,19 +242,24 @@ def pool_ab(PA, args):
                      "mean_oversize_acc": round(float(np.mean(fin("oversize_acc"))), 3) if fin("oversize_acc") else None,
                      "mlp_ms_per_gen": round(float(np.mean([r["mlp_ms"] for r in allg])), 1)}
     ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
-    champ = out["surrogate"]["champion_holdout_mean"] / out["plain"]["champion_holdout_mean"]
+    champ = float(np.mean(sur_equal)) / out["plain"]["champion_holdout_mean"]
+    out["surrogate"]["champion_holdout_at_plain_gens"] = [round(c, 4) for c in sur_equal]
+    hits = [m["speedup"] for m in match if m["speedup"] is not None]
     line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",
             "n_gpus": 1, "steps": args.gens * args.reps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 sim, bf16 surrogate", "data": "synthetic",
             "config": {"workload": f"C4: pool {n_pool} Gaussian mutants (sigma 5% of range) of a {n_elite} elite, "
                                    f"surrogate flies the top {n_fly} ({args.explore:.0%} exploration); 12 environments, "
-                                   f"gusts <= 4 m/s RMS, 12 s cap; {args.reps} GA seeds x {args.gens} generations "
-                                   f"per arm",
+                                   f"gusts <= 4 m/s RMS, 12 s cap; {args.reps} GA seeds, {args.gens} generations "
+                                   f"plain, {out['surrogate']['gens'] // args.reps} surrogate",
                        "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": args.buffer},
                        "parallelism": "single GPU"},
             "plain": out["plain"], "surrogate": out["surrogate"], "pretraining": pre,
             "speedup_gens_per_hour": round(ratio, 2),
-            "champion_ratio": round(champ, 4)}
+            "champion_ratio": round(champ, 4),
+            "time_to_plain_champion": {"per_seed": match, "reached": f"{len(hits)}/{len(match)}",
+                                       "speedup_min": min(hits) if hits else None,
+                                       "speedup_median": float(np.median(hits)) if hits else None}}
     print(json.dumps(line), flush=True)
 
 
@@ -381,7 +408,11 @@ def main():
     ap.add_argument("--pool", type=int, default=1_000_000, help="c4: candidate pool per generation")
     ap.add_argument("--fly", type=int, default=100_000, help="c4: candidates flown after pre-screening")
     ap.add_argument("--elite", type=int, default=10_000, help="c4: elite size")
-    ap.add_argument("--gens", type=int, default=6, help="c4: generations per mode")
+    ap.add_argument("--gens", type=int, default=6, help="c4: generations of the plain arm (and of the equal-"
+                                                          "generation champion comparison)")
+    ap.add_argument("--sur-gens", type=int, default=None,
+                    help="c4: generations of the surrogate arm (default 3 x --gens), for the time it takes to "
+                         "reach the plain arm's champion")
     ap.add_argument("--buffer", type=int, default=300_000, help="c4: surrogate replay buffer")
     ap.add_argument("--explore", type=float, default=0.3, help="c4: fraction of the fly set drawn at random")
     ap.add_argument("--boot-epochs", type=int, default=16, help="c4: surrogate epochs on the bootstrap generation")
@@ -405,6 +436,8 @@ def main():
                          "(default on for c2, whose configuration asks for it; the roofline then times whole graphs)")
     ap.add_argument("--tail-mode", type=int, default=None, choices=[0, 1, 2],
                     help="c1-c3: tail-mapping mode of every flight context (pr_set_tail_mapping)")
+    ap.add_argument("--list-order", choices=["size", "flight"], default="size",
+                    help="c1-c3: chunk 0's work lists with genomes in order of mesh size (default) or flight order")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="c1-c3: finish each generation's holdout before the next generation starts")
@@ -443,6 +476,8 @@ def main():
     E.upload(genomes, res)
     if args.unfused:
         E.set_fused(False)
+    if args.list_order != "size":
+        E.set_list_order(False)
     if args.graph is None:
         args.graph = w.get("graph", False)
     if args.graph:

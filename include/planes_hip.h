@@ -142,12 +142,20 @@ int pr_set_tail_mapping(pr_ctx* ctx, int on);
  * largest triangle counts first, each with its own lane mapping.  0: one launch per bucket
  * and mapping per chunk (per-mapping timings in pr_stats, used by autotuning). */
 int pr_set_fused(pr_ctx* ctx, int on);
+/* 1 (default): chunk 0's work lists hold each bucket's flights with their genomes in a
+ * stable order of mesh size (triangle count / 8), so the groups of a wave -- which take
+ * neighbouring list positions -- fly meshes of nearly equal size; 0: flight order.  Either
+ * order is a function of the data alone (evaluations stay bit-reproducible).  No
+ * reference counterpart (the reference flies one plane per block). */
+int pr_set_list_order(pr_ctx* ctx, int sorted);
 int pr_fly_variants(int* n, int* L, int* P, int cap);
 
 /* Readback of intermediate results (tests and tools). */
 int pr_read_prepared(pr_ctx* ctx, int n, float* mass20, int* nt, int* nv, int* off);
 int pr_read_panels(pr_ctx* ctx, long long first, int count, float* out7); /* cx cy cz A nx ny nz rows */
 int pr_read_scenarios(pr_ctx* ctx, int n, float* out26);
+/* genome order of the last flight batch's chunk-0 lists (pr_set_list_order 1) */
+int pr_read_genome_order(pr_ctx* ctx, int n, int* out);
 int pr_read_extent(pr_ctx* ctx, int n, float* ext3);                      /* mesh bounding-box extents */
 int pr_read_states(pr_ctx* ctx, int first, int n, float* out16);       /* FlightState rows, 16 words */
 

@@ -65,6 +65,7 @@ struct pr_ctx {
     int* d_lists[2] = {};
     int* d_counts = nullptr;  // PR_CNT per bucket: 2 alternating list counts, 2 claim counters
     int* d_tcount = nullptr;  // [NB_MAX][max_tiles] survivors per compaction tile
+    int* d_perm = nullptr;    // [max_genomes] genomes in chunk 0's list order (sorted by mesh size)
     int max_tiles = 0;
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
@@ -86,6 +87,7 @@ struct pr_ctx {
     int prep_fp64 = 0;
     int tail_wide = 2;
     int fused = 1;
+    int sort_lists = 1;
     int pending = 0, pend_events = 0, pend_chunks = 0;  // an asynchronous pr_fly in flight
     hipGraphExec_t gexec = nullptr;
     long long gkey[10] = {};
@@ -226,6 +228,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_counts, sizeof(int) * PR_CNT * NB_MAX);
     c->max_tiles = (int)((max_flights + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE);
     ALLOC(c->d_tcount, sizeof(int) * NB_MAX * (size_t)c->max_tiles);
+    ALLOC(c->d_perm, sizeof(int) * max_genomes);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
     ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
     ALLOC(c->d_cstats, sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
@@ -248,7 +251,7 @@ void pr_destroy(pr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
                     c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp,
-                    c->d_cstats, c->d_chunk_n, c->d_steps_own, c->d_tcount};
+                    c->d_cstats, c->d_chunk_n, c->d_steps_own, c->d_tcount, c->d_perm};
     for (void* b : bufs) if (b) (void)hipFree(b);
     for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -388,9 +391,15 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     q.n_buckets = NB; q.bucket_nt = c->d_bucket_nt; q.bucket = c->d_lists[1];
     q.state = c->d_state; q.fitness = d_fitness; q.steps = d_steps;
     HCK(launch_queue(q, c->stream));
-    // chunk 0's lists: each bucket's flights in flight order (count slot 0)
+    // chunk 0's lists (count slot 0): each bucket's flights with their genomes in order of
+    // mesh size (a stable sort, so the order is a function of the data), or in flight order
     CompactArgs ca = compact_args(c, NB);
     ca.src = c->d_lists[1]; ca.src_stride = 0; ca.by_bucket = 1; ca.n_fixed = (int)nf; ca.c_dst = 0;
+    if (c->sort_lists) {
+        HCK(launch_sort_genomes(c->d_nt, n_genomes, c->d_perm, c->stream));
+        ca.perm = c->d_perm;
+        ca.perm_div = layout == 0 ? n_scen : 1;
+    }
     HCK(launch_compact(ca, nf * NB, c->stream));
     *nf_out = nf;
     return 0;
@@ -503,6 +512,16 @@ int pr_set_fused(pr_ctx* c, int on) {
     ARG(c, "pr_set_fused: null context");
     c->fused = on ? 1 : 0;
     return upload_buckets(c);
+}
+
+int pr_set_list_order(pr_ctx* c, int sorted) {
+    ARG(c, "pr_set_list_order: null context");
+    c->sort_lists = sorted ? 1 : 0;
+    if (c->gexec) {  // a captured flight graph holds the old order
+        HCK(hipGraphExecDestroy(c->gexec));
+        c->gexec = nullptr;
+    }
+    return 0;
 }
 
 int pr_set_prepare_precision(pr_ctx* c, int bits) {
@@ -710,6 +729,13 @@ int pr_read_scenarios(pr_ctx* c, int n, float* out26) {
     ARG(c && out26 && n >= 0 && n <= c->n_scen, "pr_read_scenarios: n exceeds the prepared scenarios");
     HCK(hipStreamSynchronize(c->stream));
     HCK(hipMemcpy(out26, c->d_scen, sizeof(float) * NSF * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pr_read_genome_order(pr_ctx* c, int n, int* out) {
+    ARG(c && out && n >= 0 && n <= c->n_prepared, "pr_read_genome_order: n exceeds the prepared genomes");
+    HCK(hipStreamSynchronize(c->stream));
+    HCK(hipMemcpy(out, c->d_perm, sizeof(int) * n, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -748,6 +748,11 @@ constexpr int CT_THREADS = 256, CT_ITEMS = PR_COMPACT_TILE / CT_THREADS;
 
 __device__ __forceinline__ bool ct_keep(const CompactArgs& A, int v, int b) { return A.by_bucket ? v == b : v >= 0; }
 
+// by_bucket: the flight that source slot i stands for
+__device__ __forceinline__ int ct_flight(const CompactArgs& A, int i) {
+    return A.perm ? A.perm[i / A.perm_div] * A.perm_div + i % A.perm_div : i;
+}
+
 // block-wide sum over CT_THREADS threads; every thread receives it
 __device__ __forceinline__ int ct_block_sum(int v, int* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -778,7 +783,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_compact_count(CompactArgs A) {
         int c = 0;
         for (int i = 0; i < CT_ITEMS; i++) {
             const int x = base + i * CT_THREADS + (int)threadIdx.x;
-            if (x < M.n[b] && ct_keep(A, src[x], b)) c++;
+            if (x < M.n[b] && ct_keep(A, src[A.by_bucket ? ct_flight(A, x) : x], b)) c++;
         }
         c = ct_block_sum(c, red);
         if (threadIdx.x == 0) A.tcount[(size_t)b * A.max_tiles + t] = c;
@@ -809,7 +814,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_compact_scatter(CompactArgs A) {
         // the tile through LDS (coalesced), then CT_ITEMS consecutive entries per thread
         for (int i = 0; i < CT_ITEMS; i++) {
             const int x = i * CT_THREADS + (int)threadIdx.x;
-            tile[x] = x < n ? src[base + x] : (A.by_bucket ? -2 : -1);
+            tile[x] = x < n ? src[A.by_bucket ? ct_flight(A, base + x) : base + x] : (A.by_bucket ? -2 : -1);
         }
         __syncthreads();
         const int i0 = threadIdx.x * CT_ITEMS;
@@ -832,7 +837,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_compact_scatter(CompactArgs A) {
         }
         for (int i = 0; i < CT_ITEMS; i++) {
             const int v = tile[i0 + i];
-            if (ct_keep(A, v, b)) dst[off++] = A.by_bucket ? base + i0 + i : v;
+            if (ct_keep(A, v, b)) dst[off++] = A.by_bucket ? ct_flight(A, base + i0 + i) : v;
         }
         if (t == M.t0[b + 1] - M.t0[b] - 1 && threadIdx.x == 0)
             A.counts[PR_CNT * b + A.c_dst] = pre + tile_total;
@@ -848,6 +853,71 @@ hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_compact_scatter, dim3(grid), dim3(CT_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ k_sort_genomes
+// Stable counting sort of genomes by panel-slot key min(nt, 2047) / 8 (256 keys) in one
+// 1024-thread block: a histogram and its scan give each key's start; then 1024 genomes at
+// a time, each wave ranks its lanes among equal keys (a ballot per distinct key, in lane
+// order), the waves' counts are scanned per key, and every genome lands after all earlier
+// genomes of its key.  The result depends on the data alone.  Built into chunk 0's lists
+// (through CompactArgs::perm), it lets the groups of a wave -- which take neighbouring
+// list positions -- fly meshes of nearly equal size, so a wave's panel-slot count (its
+// fullest group's) wastes little on the others.
+constexpr int SORT_THREADS = 1024, SORT_KEYS = 256;
+
+__device__ __forceinline__ int sort_key(int nt) { return min(nt, 2047) >> 3; }
+
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_genomes(const int* __restrict__ nt, int n, int* __restrict__ perm) {
+    __shared__ int start[SORT_KEYS];
+    __shared__ int tot[SORT_KEYS];
+    __shared__ int wcnt[SORT_THREADS / 64][SORT_KEYS];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) start[k] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += SORT_THREADS) atomicAdd(&start[sort_key(nt[i])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int k = 0; k < SORT_KEYS; k++) { const int c = start[k]; start[k] = s; s += c; }
+    }
+    __syncthreads();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int c0 = 0; c0 < n; c0 += SORT_THREADS) {
+        const int i = c0 + tid;
+        const bool valid = i < n;
+        const int key = valid ? sort_key(nt[i]) : -1;
+        for (int k = lane; k < SORT_KEYS; k += 64) wcnt[w][k] = 0;
+        __syncthreads();
+        // rank among this wave's lanes of equal key, in lane order (every lane loops until
+        // the wave has placed all its valid lanes)
+        int rank = 0;
+        bool left = valid;
+        for (unsigned long long m = __ballot(left); m; m = __ballot(left)) {
+            const int first = __builtin_ctzll(m);
+            const int kk = __shfl(key, first);
+            const unsigned long long same = __ballot(left && key == kk);
+            if (left && key == kk) { rank = __popcll(same & lt); left = false; }
+            if (lane == first) wcnt[w][kk] = __popcll(same);
+        }
+        __syncthreads();
+        for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) {  // waves' offsets per key
+            int s = 0;
+            for (int ww = 0; ww < SORT_THREADS / 64; ww++) { const int c = wcnt[ww][k]; wcnt[ww][k] = s; s += c; }
+            tot[k] = s;
+        }
+        __syncthreads();
+        if (valid) perm[start[key] + wcnt[w][key] + rank] = i;
+        __syncthreads();
+        for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) start[k] += tot[k];
+        __syncthreads();
+    }
+}
+
+hipError_t launch_sort_genomes(const int* nt, int n, int* perm, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_genomes, dim3(1), dim3(SORT_THREADS), 0, s, nt, n, perm);
     return hipGetLastError();
 }
 

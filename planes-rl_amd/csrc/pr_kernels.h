@@ -99,6 +99,11 @@ struct CompactArgs {
     int max_tiles;
     int nb;
     int* chunk_n;
+    // by_bucket only: slot i of the source stands for flight perm[i / perm_div] * perm_div +
+    // i % perm_div (genomes in the order of perm, each with its perm_div scenarios); that
+    // flight's bucket is read and its id written.  Null perm: slot i is flight i.
+    const int* perm;
+    int perm_div;
 };
 constexpr int PR_COMPACT_TILE = 4096;
 
@@ -106,6 +111,8 @@ int fused_variant_id(int L, int P);
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
 hipError_t fly_all_occupancy(int* blocks_per_cu);
 hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s);
+// perm[0..n) = genome indices stably sorted by panel-slot key (triangle count / 8)
+hipError_t launch_sort_genomes(const int* nt, int n, int* perm, hipStream_t s);
 
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);

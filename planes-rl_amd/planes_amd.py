@@ -96,6 +96,7 @@ def _load():
     lib.pr_read_prepared.argtypes = [VP, C.c_int, VP, VP, VP, VP]
     lib.pr_read_panels.argtypes = [VP, C.c_longlong, C.c_int, F32P]
     lib.pr_read_scenarios.argtypes = [VP, C.c_int, F32P]
+    lib.pr_read_genome_order.argtypes = [VP, C.c_int, I32P]
     lib.pr_read_states.argtypes = [VP, C.c_int, C.c_int, F32P]
     lib.pr_dev_alloc.argtypes = [C.POINTER(VP), C.c_size_t]
     lib.pr_dev_free.argtypes = [VP]
@@ -107,6 +108,7 @@ def _load():
     lib.pr_set_graph.argtypes = [VP, C.c_int]
     lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
     lib.pr_set_fused.argtypes = [VP, C.c_int]
+    lib.pr_set_list_order.argtypes = [VP, C.c_int]
     lib.pr_sim.argtypes = [VP, C.c_int, C.c_int, VP, VP, C.c_int, C.c_int, C.c_float, C.c_int, C.POINTER(SimParams)]
     lib.pr_design_stats.argtypes = [F32P, C.c_int, F32P]
     lib.pr_audit.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, F32P, C.c_int]
@@ -367,6 +369,10 @@ class Planes:
         """One fused flight launch per chunk for all buckets (default) or one per bucket."""
         _ck(LIB.pr_set_fused(self.h, int(on)), "pr_set_fused")
 
+    def set_list_order(self, sorted_by_size=True):
+        """Chunk 0's work lists in a stable order of mesh size (default) or flight order."""
+        _ck(LIB.pr_set_list_order(self.h, int(sorted_by_size)), "pr_set_list_order")
+
     def set_graph(self, on=True):
         """Capture pr_fly's launch sequence as a hipGraph and replay it."""
         _ck(LIB.pr_set_graph(self.h, int(on)), "pr_set_graph")
@@ -401,6 +407,12 @@ class Planes:
         out = np.zeros(n * N_SF, np.float32)
         _ck(LIB.pr_read_scenarios(self.h, n, out), "pr_read_scenarios")
         return out.reshape(n, N_SF)
+
+    def read_genome_order(self, n):
+        """Genome order of the last flight batch's chunk-0 work lists (sorted by mesh size)."""
+        out = np.zeros(n, np.int32)
+        _ck(LIB.pr_read_genome_order(self.h, n, out), "pr_read_genome_order")
+        return out
 
     def read_states(self, first, n):
         """FlightState rows: pos3 q4 vel3 wb3 (float), step, done (int), fitness."""
@@ -500,6 +512,11 @@ class Generation:
         """Fused flight kernel (default) or one launch per bucket, for every flight context."""
         for which in (0, 1, 2):
             _ck(LIB.pr_set_fused(LIB.pr_gen_context(self.h, which), int(on)), "pr_set_fused")
+
+    def set_list_order(self, sorted_by_size=True):
+        """Work-list order (pr_set_list_order) for every flight context of the generation."""
+        for which in (0, 1, 2):
+            _ck(LIB.pr_set_list_order(LIB.pr_gen_context(self.h, which), int(sorted_by_size)), "pr_set_list_order")
 
     def set_graph(self, on=True):
         for which in (0, 1, 2):

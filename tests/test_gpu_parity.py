@@ -545,6 +545,36 @@ def test_lds_panel_slots_vs_vgpr_mappings_and_fp64(PA, res):
     assert np.mean(rel64 < 0.01) >= 0.95
 
 
+def test_size_sorted_lists_order_and_results(PA):
+    """Chunk 0's lists take the genomes in a stable order of mesh size: the order read back
+    equals numpy's stable argsort of min(nt, 2047) // 8 (nt = 0 for unprintable designs);
+    with one mapping per bucket (tail mapping off) the fitness does not depend on the list
+    order (bit-identical to flight order), and with the default tail rule it agrees up to
+    float reassociation."""
+    gs, res, _ = PA.family_population(3000, seed=5)
+    seeds = np.arange(1, 5, dtype=np.int32)
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=1800)
+    mc, ms = int(res[:, 0].max()), int(res[:, 1].max())
+    out = {}
+    for tail in (0, 2):
+        for order in (False, True):
+            pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=mc, max_sp=ms)
+            pl.set_tail_mapping(tail)
+            pl.set_list_order(order)
+            fit, steps = pl.evaluate(gs, seeds, params, res=res)
+            if order and tail == 0:
+                _, nt, _, _ = pl.read_prepared()
+                key = np.minimum(nt[:len(gs)], 2047) // 8
+                assert np.array_equal(pl.read_genome_order(len(gs)), np.argsort(key, kind="stable"))
+            out[(tail, order)] = (fit.reshape(-1).copy(), steps.reshape(-1).copy())
+            pl.close()
+    assert np.array_equal(out[(0, False)][0], out[(0, True)][0])
+    assert np.array_equal(out[(0, False)][1], out[(0, True)][1])
+    a, b = out[(2, True)][0], out[(0, False)][0]
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+    assert np.mean(rel < 1e-2) >= 0.97, np.mean(rel < 1e-2)
+
+
 def test_pr_sim_is_gpu_sim(PA):
     """pr_sim, the C entry point behind gpu_sim (sim.h:12): vary = 0 flies every plane under
     the one seed, vary = 1 plane p under seed + p * 7919 (32-bit wrap), identical to the
