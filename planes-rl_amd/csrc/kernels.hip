@@ -529,6 +529,10 @@ struct Flyer {
         }
         Fx = group_sum<L>(Fx); Fy = group_sum<L>(Fy); Fz = group_sum<L>(Fz);
         Tx = group_sum<L>(Tx); Ty = group_sum<L>(Ty); Tz = group_sum<L>(Tz);
+        // the sums are complete here: without this (empty) statement the compiler sinks the
+        // force sums' last additions past the spin clamp's branch, where their DPP moves can
+        // no longer fold into them (two extra VALU instructions per sum)
+        asm volatile("" : "+v"(Fx), "+v"(Fy), "+v"(Fz));
         // rigid-body update (sim.cu:177-197); torque is already in body axes
         float Fwx = r00 * Fx + r01 * Fy + r02 * Fz;
         float Fwy = r10 * Fx + r11 * Fy + r12 * Fz;
@@ -998,6 +1002,41 @@ __global__ void k_scores(const float* __restrict__ samples, int pop, int nseeds,
     scores[i] = 0.65f * mean + 0.25f * cv + 0.10f * row[0];
 }
 
+// Wide rows (the 128-scenario holdout, evaluate_design-sized batches): one wave per genome.
+// Each lane places its samples at their stable ascending rank (the position the insertion
+// sort above gives them: count of smaller samples plus earlier equal ones), then one lane
+// sums the mean in seed order and the CVaR over the sorted head -- the same float
+// operations in the same order as k_scores, so the same bits.
+constexpr int SCORES_WAVE_MAX = 256;
+__global__ __launch_bounds__(64) void k_scores_wave(const float* __restrict__ samples, int pop, int nseeds,
+                                                    float* __restrict__ scores) {
+#pragma clang fp contract(off)
+    __shared__ float row[SCORES_WAVE_MAX];
+    __shared__ float sorted[SCORES_WAVE_MAX];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= pop) return;  // whole block
+    for (int s = lane; s < nseeds; s += 64) row[s] = samples[(size_t)i * nseeds + s];
+    __syncthreads();
+    for (int s = lane; s < nseeds; s += 64) {
+        const float x = row[s];
+        int r = 0;
+        for (int j = 0; j < nseeds; j++) {
+            const float y = row[j];
+            r += (y < x || (y == x && j < s)) ? 1 : 0;
+        }
+        sorted[r] = x;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        float mean = 0.0f;
+        for (int s = 0; s < nseeds; s++) mean += row[s] / (float)nseeds;
+        const int tail = (nseeds + 3) / 4;
+        float cv = 0.0f;
+        for (int s = 0; s < tail; s++) cv += sorted[s] / (float)tail;
+        scores[i] = 0.65f * mean + 0.25f * cv + 0.10f * sorted[0];
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -1090,8 +1129,7 @@ hipError_t launch_scores(const float* samples, int pop, int nseeds, float* score
     if (pop <= 0) return hipSuccess;
     dim3 grid((pop + 127) / 128), blk(128);
     if (nseeds <= 16) hipLaunchKernelGGL(k_scores<16>, grid, blk, 0, s, samples, pop, nseeds, scores);
-    else if (nseeds <= 64) hipLaunchKernelGGL(k_scores<64>, grid, blk, 0, s, samples, pop, nseeds, scores);
-    else if (nseeds <= 256) hipLaunchKernelGGL(k_scores<256>, grid, blk, 0, s, samples, pop, nseeds, scores);
+    else if (nseeds <= SCORES_WAVE_MAX) hipLaunchKernelGGL(k_scores_wave, dim3(pop), dim3(64), 0, s, samples, pop, nseeds, scores);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

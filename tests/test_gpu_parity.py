@@ -294,7 +294,7 @@ def test_graph_replay_identical(PA):
 def test_scores_bit_exact(PA):
     rng = np.random.default_rng(5)
     pl = PA.Planes(1, 1, 4096)
-    for nseeds in (1, 8, 12, 128):
+    for nseeds in (1, 8, 12, 40, 128, 256):
         pop = 4096 // nseeds
         s = rng.normal(6, 5, (pop, nseeds)).astype(np.float32)
         s[rng.random(s.shape) < 0.05] = -100.0
