@@ -135,7 +135,8 @@ int pr_set_buckets(pr_ctx* ctx, int n, const int* nt_max, const int* L, const in
  * that takes over once the bucket's remaining flights fit one per wave -- the long-flight
  * tail is then latency- rather than throughput-bound.  2 (default): the same, except that
  * in the fused kernel a bucket switches only when the buckets it drains after it hold
- * fewer flights than there are waves.  0: one mapping throughout. */
+ * fewer flights than there are waves, and a chunk too small to fill the waves at a
+ * bucket's group width flies that bucket 2x or 4x wider.  0: one mapping throughout. */
 int pr_set_tail_mapping(pr_ctx* ctx, int on);
 /* 1 (default): every bucket whose mappings are compiled into the fused flight kernel is
  * flown by it -- one persistent launch per 1-s chunk drains all those buckets' work lists,

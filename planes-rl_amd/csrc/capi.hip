@@ -454,8 +454,12 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
         F.vid_p[b] = b < NB ? c->vid_p[b] : -1;
         F.vid_w[b] = b < NB ? c->vid_w[b] : -1;
         F.thr[b] = grid;  // one flight per wave once they fit one per wave
+        F.lanes[b] = b < NB ? c->buckets[b].L : 64;
+        F.vid_2[b] = b < NB && c->buckets[b].L < 64 ? fused_variant_for(2 * c->buckets[b].L, c->buckets[b].nt_max) : -1;
+        F.vid_4[b] = b < NB && c->buckets[b].L < 32 ? fused_variant_for(4 * c->buckets[b].L, c->buckets[b].nt_max) : -1;
     }
     F.tail_when_short = c->tail_wide == 2;
+    F.widen = c->tail_wide == 2;
     CompactArgs ca = compact_args(c, NB);
     ca.src = c->d_lists[1];
     ca.src_stride = c->cap_flights;

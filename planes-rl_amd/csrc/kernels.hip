@@ -695,6 +695,11 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     int below = 0;
     for (int b = 0; b < A.nb; b++)
         if (A.vid_p[b] >= 0) below += A.counts[PR_CNT * b + A.c_in];
+    // widest group width at which this chunk's flights still fill the grid's waves: a bucket
+    // whose own width is at most half (a quarter) of it flies 2x (4x) wider -- fewer
+    // flights per wave, so the few long flights of a late chunk spread over every SIMD
+    // instead of crowding part of the waves (the choice depends on the counts only)
+    const int lwant = A.widen && below > 0 ? (64 * (int)gridDim.x) / below : 0;
     // largest triangle counts first: the longest chunk jobs start first, small ones fill in
     for (int b = A.nb - 1; b >= 0; b--) {
         if (A.vid_p[b] < 0) continue;
@@ -708,9 +713,12 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         a.count_in = cb + A.c_in;
         const int vw = A.tail_when_short && below >= A.thr[b] ? -1 : A.vid_w[b];
         const int limit = vw >= 0 ? max(n_in - A.thr[b], 0) : n_in;
+        int vp = A.vid_p[b];
+        if (lwant >= 4 * A.lanes[b] && A.vid_4[b] >= 0) vp = A.vid_4[b];
+        else if (lwant >= 2 * A.lanes[b] && A.vid_2[b] >= 0) vp = A.vid_2[b];
         if (limit > 0) {
             a.stats = A.stats + 2 * (2 * b);
-            switch (A.vid_p[b]) {
+            switch (vp) {
 #define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 2, smem); break;
                 PR_FUSED(CASE)
 #undef CASE
@@ -930,6 +938,14 @@ int fused_variant_id(int L, int P) {
     PR_FUSED(ID)
 #undef ID
     return -1;
+}
+
+int fused_variant_for(int L, int nt_max) {
+    int best = -1, bp = 1 << 30;
+#define FOR(I, LL, PP) if (L == LL && LL * PP >= nt_max && PP < bp) { best = I; bp = PP; }
+    PR_FUSED(FOR)
+#undef FOR
+    return best;
 }
 
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s) {

@@ -76,6 +76,10 @@ struct FlyAllArgs {
     int nb;
     int vid_p[8], vid_w[8], thr[8];
     int tail_when_short;        // 1: vid_w only where the later-drained buckets hold < thr flights
+    // widen = 1: a chunk whose flights fill fewer than the grid's waves at a bucket's width
+    // lanes[b] flies that bucket with its 2x / 4x wider variant vid_2 / vid_4 (-1: none)
+    int widen;
+    int lanes[8], vid_2[8], vid_4[8];
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
 };
 
@@ -108,6 +112,8 @@ struct CompactArgs {
 constexpr int PR_COMPACT_TILE = 4096;
 
 int fused_variant_id(int L, int P);
+// fused variant with L lanes and the fewest panel slots that hold nt_max triangles (-1: none)
+int fused_variant_for(int L, int nt_max);
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
 hipError_t fly_all_occupancy(int* blocks_per_cu);
 hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s);

@@ -545,6 +545,38 @@ def test_lds_panel_slots_vs_vgpr_mappings_and_fp64(PA, res):
     assert np.mean(rel64 < 0.01) >= 0.95
 
 
+def test_pose_recording_with_lds_panel_slots(PA):
+    """Pose recording flies every flight with the widest bucket's mapping; with (16, 33) as
+    that mapping, meshes of 500-504 triangles use its LDS slots.  The recorded flights end
+    where the same mapping in the work-list kernel ends them, up to float rounding (the two
+    kernels are compiled separately, and FMA contraction may differ between them), and the
+    last alive frame is the flight's final position."""
+    ch, sp = 5, 12
+    gs = O.random_genomes(16, 71)
+    r = np.tile(np.array([[ch, sp]], np.int32), (len(gs), 1))
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=1800)
+    seeds = np.array([3, 4], np.int32)
+    pl = PA.Planes(len(gs), 2, 2 * len(gs), max_ch=ch, max_sp=sp)
+    pl.set_tail_mapping(False)
+    pl.set_buckets([528], [16], [33])
+    fit, steps = pl.evaluate(gs, seeds, params, res=r)
+    _, nt, _, _ = pl.read_prepared()
+    assert nt.max() > 22 * 16  # LDS slots in use
+    fr, sr, frames = pl.fly_record(0, len(gs), 2, params, rec_every=5, rec_cap=1800 // 5)
+    f0 = fit.reshape(-1)
+    rel = np.abs(fr - f0) / np.maximum(np.abs(f0), 1e-3)
+    assert np.mean(rel < 1e-3) >= 0.9 and np.mean(rel < 1e-2) >= 0.97, rel
+    assert np.all((fr == -100) == (f0 == -100)) and np.mean(sr == steps.reshape(-1)) >= 0.9
+    alive = frames[:, :, 7]
+    assert np.all(np.diff(alive, axis=1) <= 0)  # once ended, a flight stays ended
+    for f in range(len(fr)):
+        dead = np.nonzero(alive[f] == 0)[0]
+        if sr[f] > 0 and fr[f] not in (-10.0, -100.0) and len(dead):
+            # an ended flight keeps writing its final pose: x there is its fitness
+            assert frames[f, dead[0], 0] == fr[f], (f, frames[f, dead[0]], fr[f], sr[f])
+    pl.close()
+
+
 def test_size_sorted_lists_order_and_results(PA):
     """Chunk 0's lists take the genomes in a stable order of mesh size: the order read back
     equals numpy's stable argsort of min(nt, 2047) // 8 (nt = 0 for unprintable designs);
