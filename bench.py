@@ -420,7 +420,7 @@ def main():
                     help="c4: surrogate epochs per generation (default 2 after pretraining, else 4)")
     ap.add_argument("--lr", type=float, default=None,
                     help="c4: surrogate learning rate (Adam; default 1e-4 fine-tuning after pretraining, else 2e-3)")
-    ap.add_argument("--reps", type=int, default=3, help="c4: GA seeds per arm")
+    ap.add_argument("--reps", type=int, default=5, help="c4: GA seeds per arm")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
     ap.add_argument("--pretrain", type=int, default=2_000_000,
                     help="c4: sim pairs the surrogate is pretrained on (setup; 0: bootstrap generation instead)")

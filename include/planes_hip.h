@@ -262,6 +262,11 @@ int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* sample
 int pr_pool_set_collect(pr_pool* p, int on);
 /* Fraction of the surrogate fly set (beyond the elite) drawn at random from the pool. */
 int pr_pool_set_explore(pr_pool* p, float frac);
+/* 1 (default): once a generation has simulated the elite, a mutant's selection key is its
+ * parent's simulated score plus the network's predicted change from parent to mutant
+ * (the error the network makes on both cancels); 0, and before the first generation: the
+ * predicted score itself.  No reference counterpart (the reference has no surrogate). */
+int pr_pool_set_parent_delta(pr_pool* p, int on);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 /* The last generation's pool [n_pool x 35], fly set [n_fly x 35] in flight order, its

@@ -74,13 +74,21 @@ __global__ void k_gather(const float* __restrict__ src, const int* __restrict__ 
 // keys for a descending sort: -score, with the elite (i < n_keep) and an exploration sample
 // (each other candidate with probability p_explore, hashed from (seed, gen, i)) forced to
 // the front
+// escore (non-null): the elite's simulated scores; a mutant's key is then its parent's
+// simulated score plus the predicted change from parent to mutant (the network's error
+// common to both cancels), with the parent drawn as k_mutate draws it
 __global__ void k_sort_keys(const float* __restrict__ pred, int stride, int col, int n, int n_keep,
-                            float p_explore, uint32_t seed, uint32_t gen, float* __restrict__ keys,
-                            int* __restrict__ vals) {
+                            float p_explore, uint32_t seed, uint32_t gen, const float* __restrict__ escore,
+                            float* __restrict__ keys, int* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool explore = i >= n_keep && unit24(hash3(seed ^ 0x2545f491u, gen, (uint32_t)i)) < p_explore;
-    keys[i] = i < n_keep ? -3.0e38f : explore ? -2.0e38f : -pred[(size_t)i * stride + col];
+    float s = pred[(size_t)i * stride + col];
+    if (escore && i >= n_keep) {
+        const int par = (int)(hash3(seed, gen, (uint32_t)i) % (uint32_t)n_keep);
+        s = escore[par] + (s - pred[(size_t)par * stride + col]);
+    }
+    keys[i] = i < n_keep ? -3.0e38f : explore ? -2.0e38f : -s;
     vals[i] = i;
 }
 
@@ -148,6 +156,9 @@ struct pr_pool {
     float *d_elite = nullptr, *d_pool = nullptr, *d_fly = nullptr, *d_lo = nullptr, *d_hi = nullptr;
     float *d_pred = nullptr, *d_keys = nullptr, *d_keys2 = nullptr, *d_samples = nullptr, *d_tgt = nullptr;
     float *d_buf_g = nullptr, *d_buf_t = nullptr;
+    float* d_escore = nullptr;  // [n_elite] simulated scores of the current elite
+    int has_escore = 0;         // set by a generation, cleared by pr_pool_set_elite
+    int parent_delta = 1;       // mutants' keys from the parent's simulated score (when known)
     int *d_vals = nullptr, *d_vals2 = nullptr, *d_idx = nullptr;
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -191,6 +202,7 @@ int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds
     PRC(pr_gen_create(&p->gen_eval, n_pool, nseeds, 0, 0));
     const size_t G = sizeof(float) * pr::NG;
     PCK(hipMalloc(&p->d_elite, G * n_elite));
+    PCK(hipMalloc(&p->d_escore, sizeof(float) * n_elite));
     PCK(hipMalloc(&p->d_pool, G * n_pool));
     PCK(hipMalloc(&p->d_fly, G * n_pool));
     PCK(hipMalloc(&p->d_lo, G));
@@ -221,7 +233,8 @@ void pr_pool_destroy(pr_pool* p) {
     if (!p) return;
     pr_gen_destroy(p->gen_eval);
     void* bufs[] = {p->d_elite, p->d_pool, p->d_fly, p->d_lo, p->d_hi, p->d_pred, p->d_keys, p->d_keys2,
-                    p->d_vals, p->d_vals2, p->d_idx, p->d_samples, p->d_tgt, p->d_buf_g, p->d_buf_t, p->d_sort_tmp};
+                    p->d_vals, p->d_vals2, p->d_idx, p->d_samples, p->d_tgt, p->d_buf_g, p->d_buf_t, p->d_sort_tmp,
+                    p->d_escore};
     for (void* b : bufs) if (b) (void)hipFree(b);
     delete p;
 }
@@ -230,6 +243,7 @@ void pr_pool_destroy(pr_pool* p) {
 int pr_pool_set_elite(pr_pool* p, const float* genomes, int n) {
     if (!p || n < p->n_elite) { g_pool_err = "pr_pool_set_elite: need n >= n_elite"; return -1; }
     PCK(hipMemcpy(p->d_elite, genomes, sizeof(float) * pr::NG * p->n_elite, hipMemcpyHostToDevice));
+    p->has_escore = 0;
     return 0;
 }
 
@@ -282,8 +296,9 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         PRC(pr_mlp_predict(model, p->d_pool, p->n_pool, p->d_pred));
         const float p_explore =
             p->n_pool > p->n_elite ? p->explore * (float)(p->n_fly - p->n_elite) / (float)(p->n_pool - p->n_elite) : 0.f;
+        const float* escore = p->has_escore && p->parent_delta ? p->d_escore : nullptr;
         hipLaunchKernelGGL(k_sort_keys, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->n_pool,
-                           p->n_elite, p_explore, p->seed, (uint32_t)gen, p->d_keys, p->d_vals);
+                           p->n_elite, p_explore, p->seed, (uint32_t)gen, escore, p->d_keys, p->d_vals);
         PCK(hipGetLastError());
         size_t tb = p->sort_tmp_bytes;
         PCK(hipcub::DeviceRadixSort::SortPairs(p->d_sort_tmp, tb, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
@@ -314,6 +329,12 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     hipLaunchKernelGGL(k_gather, dim3((p->n_elite * pr::NG + 255) / 256), dim3(256), 0, 0, fly, p->d_idx, p->n_elite,
                        p->d_elite);
     PCK(hipGetLastError());
+    {
+        std::vector<float> es(p->n_elite);
+        for (int k = 0; k < p->n_elite; k++) es[k] = p->acc[p->idx[k]];
+        PCK(hipMemcpy(p->d_escore, es.data(), sizeof(float) * p->n_elite, hipMemcpyHostToDevice));
+        p->has_escore = 1;
+    }
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
     p->champion_score = holdout;
     p->best_holdout = std::max(p->best_holdout, holdout);
@@ -401,6 +422,12 @@ int pr_pool_set_explore(pr_pool* p, float frac) {
 
 // 1: plain generations (model = null) also add their flown pairs to the replay buffer,
 // e.g. to bootstrap the surrogate before pre-screening starts.
+int pr_pool_set_parent_delta(pr_pool* p, int on) {
+    if (!p) { g_pool_err = "pr_pool_set_parent_delta: null pool"; return -1; }
+    p->parent_delta = on ? 1 : 0;
+    return 0;
+}
+
 int pr_pool_set_collect(pr_pool* p, int on) {
     if (!p) { g_pool_err = "pr_pool_set_collect: null pool"; return -1; }
     p->collect = on ? 1 : 0;

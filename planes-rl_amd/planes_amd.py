@@ -130,6 +130,7 @@ def _load():
     lib.pr_pool_train.argtypes = [VP, VP, C.c_int, C.c_float, C.POINTER(C.c_float)]
     lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
     lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
+    lib.pr_pool_set_parent_delta.argtypes = [VP, C.c_int]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_pool_last.argtypes = [VP, VP, VP, VP, VP, C.POINTER(C.c_int)]
     lib.pr_mlp_param_count.restype = C.c_int
@@ -840,6 +841,11 @@ class PoolGA:
     def set_explore(self, frac):
         """Fraction of the surrogate fly set drawn at random from the pool."""
         self._ck(LIB.pr_pool_set_explore(self.h, float(frac)), "pr_pool_set_explore")
+
+    def set_parent_delta(self, on=True):
+        """Mutants ranked by parent's simulated score + predicted change (default) or by the
+        predicted score itself."""
+        self._ck(LIB.pr_pool_set_parent_delta(self.h, int(on)), "pr_pool_set_parent_delta")
 
     def set_collect(self, on=True):
         """Plain generations also feed the surrogate's replay buffer (bootstrap)."""

@@ -188,6 +188,44 @@ def test_surrogate_prescreening_selection_and_targets(PA):
     assert pred[chosen].min() >= np.sort(pred[rest])[::-1][len(chosen) - 1]
 
 
+def test_surrogate_parent_delta_keys(PA):
+    """After a generation has simulated the elite, a mutant's selection key is its parent's
+    simulated score plus the network's predicted change from parent to mutant: the second
+    generation's fly set is the stable-sort order of exactly those keys (parents from the
+    mutation hash, the elite's scores from the first generation's flights), and with the
+    option off it is the order of the predicted scores themselves."""
+    n_elite, n_pool, n_fly, k, seed, explore = 32, 8192, 1024, 4, 9, 0.25
+    params = PA.SimParams.config(**PARAMS)
+    g0, _, _ = PA.family_population(n_elite, seed=4)
+    S = PA.Surrogate(max_batch=1024, seed=5)
+    for delta in (True, False):
+        ga = PA.PoolGA(n_elite, n_pool, n_fly, k, buf_cap=4 * n_fly, sigma_frac=0.05, seed=seed)
+        ga.set_elite(g0)
+        ga.set_explore(explore)
+        ga.set_parent_delta(delta)
+        gen = 3
+        ga.generation(gen, params, S, train_epochs=0)
+        last = ga.last()
+        fly1, sc1 = last["fly"], last["scores"]
+        E = ga.elite()
+        rows = {fly1[i].tobytes(): i for i in range(len(fly1))}
+        escore = sc1[np.array([rows[e.tobytes()] for e in E])].astype(np.float32)
+        ga.generation(gen + 1, params, S, train_epochs=0)
+        last = ga.last()
+        pool, fly = last["pool"], last["fly"]
+        assert np.array_equal(pool[:n_elite], E)
+        pred = S.predict(pool)[:, 4].astype(np.float32)
+        i = np.arange(n_pool)
+        par = (hash3(seed, gen + 1, i) % n_elite).astype(np.int64)
+        p_explore = np.float32(explore) * np.float32(n_fly - n_elite) / np.float32(n_pool - n_elite)
+        is_exp = (i >= n_elite) & (unit24(hash3(seed ^ 0x2545F491, gen + 1, i)) < p_explore)
+        s = escore[par] + (pred - pred[par]) if delta else pred
+        keys = np.where(i < n_elite, np.float32(-3.0e38), np.where(is_exp, np.float32(-2.0e38), -s)).astype(np.float32)
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(fly, pool[order[:n_fly]]), delta
+        del ga
+
+
 def test_surrogate_generations_reproducible(PA):
     """Surrogate mode end to end -- predict, select, fly, add the flown pairs to the replay
     buffer, train -- gives the same elites, holdout scores, losses and network weights on a
