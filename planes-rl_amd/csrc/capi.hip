@@ -459,7 +459,9 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
         F.vid_4[b] = b < NB && c->buckets[b].L < 32 ? fused_variant_for(4 * c->buckets[b].L, c->buckets[b].nt_max) : -1;
     }
     F.tail_when_short = c->tail_wide == 2;
-    F.widen = c->tail_wide == 2;
+    // widening factor 2 (measured against 1, 4 and 8: C3 within noise of each other up to 4,
+    // C2 +1 % over 1; 8 slows C3)
+    F.widen = c->tail_wide == 2 ? 2 : 0;
     CompactArgs ca = compact_args(c, NB);
     ca.src = c->d_lists[1];
     ca.src_stride = c->cap_flights;

@@ -695,11 +695,11 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     int below = 0;
     for (int b = 0; b < A.nb; b++)
         if (A.vid_p[b] >= 0) below += A.counts[PR_CNT * b + A.c_in];
-    // widest group width at which this chunk's flights still fill the grid's waves: a bucket
-    // whose own width is at most half (a quarter) of it flies 2x (4x) wider -- fewer
-    // flights per wave, so the few long flights of a late chunk spread over every SIMD
-    // instead of crowding part of the waves (the choice depends on the counts only)
-    const int lwant = A.widen && below > 0 ? (64 * (int)gridDim.x) / below : 0;
+    // widest group width at which this chunk's flights still fill widen x the grid's waves:
+    // a bucket whose own width is at most half (a quarter) of it flies 2x (4x) wider --
+    // fewer flights per wave, so the few long flights of a late chunk spread over more
+    // SIMDs instead of crowding part of the waves (the choice depends on the counts only)
+    const int lwant = A.widen && below > 0 ? (A.widen * 64 * (int)gridDim.x) / below : 0;
     // largest triangle counts first: the longest chunk jobs start first, small ones fill in
     for (int b = A.nb - 1; b >= 0; b--) {
         if (A.vid_p[b] < 0) continue;

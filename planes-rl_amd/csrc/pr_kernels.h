@@ -76,8 +76,9 @@ struct FlyAllArgs {
     int nb;
     int vid_p[8], vid_w[8], thr[8];
     int tail_when_short;        // 1: vid_w only where the later-drained buckets hold < thr flights
-    // widen = 1: a chunk whose flights fill fewer than the grid's waves at a bucket's width
-    // lanes[b] flies that bucket with its 2x / 4x wider variant vid_2 / vid_4 (-1: none)
+    // widen = k > 0: a chunk whose flights would fill fewer than k x the grid's waves at a
+    // bucket's width lanes[b] flies that bucket with its 2x / 4x wider variant vid_2 / vid_4
+    // (-1: none); 0: never
     int widen;
     int lanes[8], vid_2[8], vid_4[8];
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
