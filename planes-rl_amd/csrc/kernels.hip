@@ -603,8 +603,35 @@ struct Flyer {
 // from `head` (positions lo + 0, 1, 2, ...), and at the chunk's end marks it with the flight
 // id if the flight goes on, else -1.  Which code path flies a position depends only on the
 // position, so the results do not depend on the order in which groups claim.
+// XCD-local claiming: list positions [lo, hi) in PR_XCDS equal shares, share x claimed
+// through counter xh[x]; a workgroup takes positions from its own XCD's share first
+// (workgroups are dispatched to the XCDs round-robin by index, so blockIdx % 8 is its
+// XCD) and then from the others'.  A genome's flights sit next to each other in the list,
+// so its panels are mostly fetched into one XCD's L2 instead of up to eight.  Every
+// position is taken exactly once (each share's counter hands out its positions once);
+// hi when all shares are spent.  Which code path flies a position still depends only on
+// the position.
+__device__ __forceinline__ int claim_xcd(int* xh, int lo, int hi) {
+    const int n = hi - lo;  // < 2^28 (list capacity), so n * PR_XCDS fits an int
+    const int me = (int)(blockIdx.x % PR_XCDS);
+    // not unrolled: the eight shares' bounds would otherwise be kept in scalar registers at
+    // every call site of the fused kernel's variants (hundreds of scalar spills)
+#pragma unroll 1
+    for (int r = 0; r < PR_XCDS; r++) {
+        const int x = (me + r) % PR_XCDS;
+        const int a = lo + n * x / PR_XCDS, b = lo + n * (x + 1) / PR_XCDS;
+        if (a + __atomic_load_n(xh + x, __ATOMIC_RELAXED) < b) {
+            const int k = a + atomicAdd(xh + x, 1);
+            if (k < b) return k;
+        }
+    }
+    return hi;
+}
+
+// xheads: PR_XCDS claim counters (XCD-local claiming) or null (one counter, head)
 template <int L, int P>
-__device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head, float* smem) {
+__device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head, float* smem,
+                                         int* xheads = nullptr) {
     const int lane = threadIdx.x & 63;
     const int j = lane & (L - 1);
     const int gbase = lane & ~(L - 1);
@@ -620,9 +647,13 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* 
             if (need) {
                 int k = hi;
                 if (j == 0) {
-                    // look before claiming: once the range is spent, no more atomics
-                    const int h = __atomic_load_n(head, __ATOMIC_RELAXED);
-                    if (lo + h < hi) k = lo + atomicAdd(head, 1);
+                    if (xheads) {
+                        k = claim_xcd(xheads, lo, hi);
+                    } else {
+                        // look before claiming: once the range is spent, no more atomics
+                        const int h = __atomic_load_n(head, __ATOMIC_RELAXED);
+                        if (lo + h < hi) k = lo + atomicAdd(head, 1);
+                    }
                 }
                 k = __shfl(k, gbase);
                 if (k >= hi) exhausted = true;
@@ -716,10 +747,14 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         int vp = A.vid_p[b];
         if (lwant >= 4 * A.lanes[b] && A.vid_4[b] >= 0) vp = A.vid_4[b];
         else if (lwant >= 2 * A.lanes[b] && A.vid_2[b] >= 0) vp = A.vid_2[b];
+        // XCD-local claiming where the pass holds many flights per wave of the grid (the
+        // large early chunks, which read most of the panels); a small chunk's flights are
+        // taken from one counter, so they spread over the grid in claim order
+        int* xh = limit >= 8 * (int)gridDim.x ? cb + 4 : nullptr;
         if (limit > 0) {
             a.stats = A.stats + 2 * (2 * b);
             switch (vp) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 2, smem); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, 0, limit, cb + 4 + PR_XCDS, smem, xh); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
@@ -728,7 +763,7 @@ __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
         if (vw >= 0 && limit < n_in) {
             a.stats = A.stats + 2 * (2 * b + 1);
             switch (vw) {
-#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, limit, n_in, cb + 3, smem); break;
+#define CASE(ID, LL, PP) case ID: fly_list<LL, PP>(a, limit, n_in, cb + 4 + PR_XCDS + 1, smem); break;
                 PR_FUSED(CASE)
 #undef CASE
                 default: break;
@@ -782,8 +817,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_compact_count(CompactArgs A) {
     if (threadIdx.x == 0) M.load(A);
     __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x < A.nb) {
-        A.counts[PR_CNT * threadIdx.x + 2] = 0;
-        A.counts[PR_CNT * threadIdx.x + 3] = 0;
+        for (int k = PR_CNT_CLAIM0; k < PR_CNT; k++) A.counts[PR_CNT * threadIdx.x + k] = 0;
         if (A.chunk_n) A.chunk_n[threadIdx.x] = M.n[threadIdx.x];
     }
     const int total = M.t0[A.nb];

@@ -57,10 +57,16 @@ struct FlyArgs {
 };
 
 // Per-bucket chunk counters (PR_CNT ints per bucket): [0..1] list counts (chunk ck reads
-// slot ck & 1, its compaction writes slot (ck + 1) & 1), [2] claim counter of the
-// many-flights-per-wave pass (and of a bucket's own k_fly), [3] claim counter of the
-// one-flight-per-wave pass.  The compaction after each chunk clears [2..3].
-#define PR_CNT 4
+// slot ck & 1, its compaction writes slot (ck + 1) & 1), [2] claim counter of a bucket's
+// own k_fly many-flights-per-wave launch, [3] of its one-flight-per-wave launch,
+// [4..11] the fused kernel's many-flights-per-wave claim counters, one per XCD (its
+// share of the list positions), [12] that pass's single counter (small chunks), [13] the
+// fused one-flight-per-wave pass's counter.  The compaction after each chunk clears
+// [2..19].
+#define PR_CNT 20
+#define PR_CNT_CLAIM0 2
+#define PR_XCDS 8
+static_assert(4 + PR_XCDS + 2 <= PR_CNT, "counter layout");
 
 // All buckets' chunk in one persistent launch (k_fly_all).  Bucket b's list is lists + b *
 // list_stride, its marks marks + b * list_stride, its counters counts + PR_CNT * b.
