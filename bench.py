@@ -100,7 +100,7 @@ def cpu_baseline(genomes, res, w, budget_s):
 
 
 def fp64_check(PA, genomes, res, w, params, n_planes=256):
-    """The oracle as the checker (this leg only, beside the CPU baseline): the first n_planes
+    """The oracle checking the GPU results (this leg only, beside the CPU baseline): the first n_planes
     genomes of the timed population under the first timed generation's seeds, flown on the
     GPU (after the timed region) and by the FP64 oracle on the host; per-flight distance
     error against FP64 and the rank agreement of the flights."""
@@ -339,7 +339,7 @@ def robust_stats(samples):
 
 
 def surrogate_ranking(PA, genomes, res, dist, params, n_train=400000, epochs=10):
-    """The audited designs ranked by the surrogate against their sim-judged ranks (robust
+    """The audited designs ranked by the surrogate against their simulated ranks (robust
     score over every audit scenario).  The network is trained on sim flights of a separate
     population from the same family generator (n_train designs x 8 seeds)."""
     g_tr, r_tr, _ = PA.family_population(n_train, seed=11)
@@ -365,7 +365,7 @@ def surrogate_ranking(PA, genomes, res, dist, params, n_train=400000, epochs=10)
 
 
 def audit_fp64_check(genomes, res, best, dist, n_scen=256):
-    """The oracle as the checker (beside the CPU baseline): the audit champion's first n_scen
+    """The oracle checking the GPU results (beside the CPU baseline): the audit champion's first n_scen
     scenarios in FP64 against the batched GPU audit's distances, per scenario."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O

@@ -216,7 +216,7 @@ int pr_gen_collect(pr_gen* e, float* holdout_out, int* champion_out);
 /* ---------------------------------------------------------------- surrogate MLP
  * BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU) on the matrix cores, FP32 accumulation,
  * FP32 master weights + Adam (no reference counterpart: a pre-screen for candidate pools;
- * the flight simulation stays the judge).  Targets per genome: mean distance, CVaR of the
+ * the flight simulation alone decides).  Targets per genome: mean distance, CVaR of the
  * worst quarter, worst distance [m], oversize flag (0/1).  Parameter layout (FP32):
  * W1[64x256] b1[256] W2[256x256] b2[256] W3[256x256] b3[256] W4[256x32] b4[32]. */
 typedef struct pr_mlp pr_mlp;

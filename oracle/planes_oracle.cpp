@@ -1,8 +1,8 @@
 // planes_oracle.cpp -- CPU restatement of the planes-rl fitness-evaluation path.
 //
 // THIS IS TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
-// cpu_baseline leg of bench.py load it, and only as the checker or as the timed
-// CPU baseline.  The product (planes-rl_amd/, libplanes_hip.so) never links it.
+// cpu_baseline leg of bench.py load it, and only to check the GPU path's results
+// or as the timed CPU baseline.  The product (planes-rl_amd/, libplanes_hip.so) never links it.
 //
 // Every function is templated on the arithmetic type R (float = the FP32
 // transcription, double = the FP64 oracle).  Model constants are the reference's

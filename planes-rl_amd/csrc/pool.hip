@@ -12,7 +12,7 @@
 //      128-scenario holdout suite -> champion
 //   5. surrogate mode: the flown (genome, fitness statistics) pairs join a replay buffer
 //      and the network trains on it (pr_mlp_train)
-// The simulation is the only judge of the elite and the champion.
+// The simulation alone decides the elite and the champion.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 

@@ -1,8 +1,8 @@
 """ctypes bindings for the CPU oracle (oracle/_build/liboracle.so).
 
 Test infrastructure: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg only -- as the checker or the timed CPU baseline, never as the
-product path.
+cpu_baseline leg only -- to check the GPU path's results or as the timed CPU
+baseline, never as the product path.
 """
 import ctypes as C
 import os
