@@ -2,7 +2,7 @@
 // (main.cu:36-105) for the modes on the fitness-evaluation path: evolve, evaluate,
 // select-best, inspect, presets (seed population and descriptions; its STL export is the
 // reference's exporter).  The video and STL modes belong to the reference's
-// renderer/exporter and are not part of this build.
+// renderer/exporter and are not part of this library.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
