@@ -436,6 +436,10 @@ def main():
                          "(default on for c2, whose configuration asks for it; the roofline then times whole graphs)")
     ap.add_argument("--tail-mode", type=int, default=None, choices=[0, 1, 2],
                     help="c1-c3: tail-mapping mode of every flight context (pr_set_tail_mapping)")
+    ap.add_argument("--no-split", action="store_true",
+                    help="c1-c3: fly small chunks like the others instead of one flight per 2- or 4-wave workgroup")
+    ap.add_argument("--split-holdout", action="store_true",
+                    help="c1-c3: the small-chunk mapping for the holdout batches too (default: population only)")
     ap.add_argument("--list-order", choices=["size", "flight"], default="size",
                     help="c1-c3: chunk 0's work lists with genomes in order of mesh size (default) or flight order")
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
@@ -478,6 +482,10 @@ def main():
         E.set_fused(False)
     if args.list_order != "size":
         E.set_list_order(False)
+    if args.no_split:
+        E.set_split(False)
+    elif args.split_holdout:
+        E.set_split(True, holdout=True)
     if args.graph is None:
         args.graph = w.get("graph", False)
     if args.graph:
@@ -542,6 +550,10 @@ def main():
                 if b["fused"]:
                     fused["flight_steps"] += b["flight_steps"]
                     fused["panel_steps"] += b["panel_steps"]
+            # small chunks: k_fly_split, timed inside the fused kernel's event bracket
+            fused["flight_steps"] += st["split_flight_steps"]
+            fused["panel_steps"] += st["split_panel_steps"]
+            fused["split_steps"] = fused.get("split_steps", 0) + st["split_flight_steps"]
     PA.LIB.pr_device_sync()
     wall = time.perf_counter() - t0
     pfs = fsteps / 600.0
@@ -560,6 +572,8 @@ def main():
         name, fl, ms, nl = "k_fly_all (graph replays)", fused_graph, fused_graph["ms"], fused_graph["launches"]
     elif fused["launches"]:
         name, fl, ms, nl = "k_fly_all", fused, fused["ms"], fused["launches"]
+        if fused.get("split_steps"):
+            name += " + k_fly_split (small chunks)"
     else:
         key, fl = max(bucket.items(), key=lambda kv: kv[1]["fly_ms"])
         name, ms, nl = f"k_fly<{key[0]},{key[1]}>" + (" (tail mapping)" if key[2] else ""), fl["fly_ms"], fl["launches"]
@@ -577,8 +591,9 @@ def main():
     last = E.stats()[0]
     for k, ch in enumerate(last["chunks"]):
         fl = FLOP_PANEL * ch["panel_steps"] + FLOP_PLANE * ch["flight_steps"]
+        rate = f"{fl / ch['ms'] / 1e9:.1f} TFLOP/s" if ch["ms"] > 0 else "untimed (graph replay)"
         log(f"  last generation chunk {k}: {ch['flights']} flights, {ch['flight_steps']} flight-steps, "
-            f"{ch['ms']:.2f} ms, {fl / max(ch['ms'], 1e-9) / 1e9:.1f} TFLOP/s")
+            f"{ch['ms']:.2f} ms, {rate}")
     for kk, bb in sorted(bucket.items()):
         log(f"  {'(fused) ' if bb['fused'] else ''}k_fly<{kk[0]},{kk[1]}>{' tail' if kk[2] else ''}: "
             f"{bb['launches']} own launches, {bb['fly_ms']:.2f} ms, "

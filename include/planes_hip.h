@@ -54,6 +54,8 @@ typedef struct pr_stats {
     int c_flights[PR_MAX_CHUNKS];
     unsigned long long c_flight_steps[PR_MAX_CHUNKS], c_panel_steps[PR_MAX_CHUNKS];
     double c_ms[PR_MAX_CHUNKS];
+    /* of the totals, the steps flown by the small-chunk kernel (pr_set_split) */
+    unsigned long long split_flight_steps, split_panel_steps;
 } pr_stats;
 
 /* Replaces panelbuf_alloc/panelbuf_free (sim.h:9-10, sim.cu:220-235).
@@ -149,6 +151,15 @@ int pr_set_fused(pr_ctx* ctx, int on);
  * order is a function of the data alone (evaluations stay bit-reproducible).  No
  * reference counterpart (the reference flies one plane per block). */
 int pr_set_list_order(pr_ctx* ctx, int sorted);
+/* 1 (default): a chunk with at most one flight per two SIMDs (the long-flight tail of a
+ * generation, small batches) is flown one flight per workgroup of 4 waves (at most one
+ * flight per 4 SIMDs) or 2 waves, the flight's panels spread over all their lanes and the
+ * partial force sums added in LDS each step: a step's panel work is 4 (2) times shorter
+ * than with one wave per flight.  0: such chunks fly like the others.  The choice depends
+ * on the chunk's flight count only (evaluations stay bit-reproducible); the two settings
+ * sum the panels in different orders.  No reference counterpart (the reference flies one
+ * plane per block). */
+int pr_set_split(pr_ctx* ctx, int on);
 int pr_fly_variants(int* n, int* L, int* P, int cap);
 
 /* Readback of intermediate results (tests and tools). */

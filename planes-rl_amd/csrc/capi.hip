@@ -88,6 +88,7 @@ struct pr_ctx {
     int tail_wide = 2;
     int fused = 1;
     int sort_lists = 1;
+    int split = 1;  // small chunks by k_fly_split (several waves per flight)
     int pending = 0, pend_events = 0, pend_chunks = 0;  // an asynchronous pr_fly in flight
     hipGraphExec_t gexec = nullptr;
     long long gkey[10] = {};
@@ -96,6 +97,9 @@ struct pr_ctx {
 };
 
 static const int NB_MAX = 8;
+// work counters: [bucket][many per wave, one per wave][flight-steps, panel-steps], then the
+// small-chunk kernel's pair
+static const int PR_STATS_N = 4 * NB_MAX + 2;
 
 static int mesh_bounds(int ch, int sp, int* nv, int* nt) {
     int nw = (2 * sp + 1) * (ch + 1), tw = 4 * sp * ch;
@@ -230,7 +234,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     ALLOC(c->d_tcount, sizeof(int) * NB_MAX * (size_t)c->max_tiles);
     ALLOC(c->d_perm, sizeof(int) * max_genomes);
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
-    ALLOC(c->d_stats, sizeof(unsigned long long) * 4 * NB_MAX);
+    ALLOC(c->d_stats, sizeof(unsigned long long) * PR_STATS_N);
     ALLOC(c->d_cstats, sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
     ALLOC(c->d_chunk_n, sizeof(int) * PR_MAX_CHUNKS * NB_MAX);
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, max_genomes + 1, c->stream) !=
@@ -356,6 +360,7 @@ static void fill_fly_args(pr_ctx* c, const pr_sim_params* p, float* d_fitness, i
     a.chunk = p->chunk; a.max_steps = p->max_steps; a.spread = p->spread; a.stats = c->d_stats;
     a.list_in = nullptr; a.count_in = nullptr; a.marks = nullptr; a.head = nullptr;
     a.run_lo = 0; a.run_hi = 0x7fffffff;
+    a.counts_all = nullptr; a.nb_all = 0; a.c_all = 0; a.split_max = 0;
 }
 
 static CompactArgs compact_args(pr_ctx* c, int NB) {
@@ -382,7 +387,7 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     ARG(nf <= c->cap_flights, "pr_fly: flights exceed max_flights");
     const int NB = (int)c->buckets.size();
     HCK(launch_zero(c->d_counts, PR_CNT * NB_MAX, c->stream));
-    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 8 * NB_MAX, c->stream));
+    HCK(launch_zero(reinterpret_cast<int*>(c->d_stats), 2 * PR_STATS_N, c->stream));
     HCK(launch_zero(reinterpret_cast<int*>(c->d_cstats), 4 * PR_MAX_CHUNKS, c->stream));
     HCK(launch_zero(c->d_chunk_n, PR_MAX_CHUNKS * NB_MAX, c->stream));
     QueueArgs q;
@@ -450,6 +455,7 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
     F.counts = c->d_counts;
     F.nb = NB;
     F.stats = c->d_stats;
+    F.split_stats = c->d_stats + 4 * NB_MAX;
     for (int b = 0; b < 8; b++) {
         F.vid_p[b] = b < NB ? c->vid_p[b] : -1;
         F.vid_w[b] = b < NB ? c->vid_w[b] : -1;
@@ -462,6 +468,18 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
     // widening factor 2 (measured against 1, 4 and 8: C3 within noise of each other up to 4,
     // C2 +1 % over 1; 8 slows C3)
     F.widen = c->tail_wide == 2 ? 2 : 0;
+    // Small chunks: at most one flight per 4 (2) SIMDs are flown with the flight's panels
+    // spread over a workgroup of 4 (2) waves, one wave per SIMD; the other launches of such
+    // a chunk return at once.  Decided on the device from the chunk's flight count, so a
+    // captured graph holds all the launches.
+    const int s4 = c->simds / 4, s2 = c->simds / 2;
+    const bool split = c->split && s4 > 0 && c->buckets.back().nt_max <= PR_SPLIT_NT;
+    a.counts_all = c->d_counts;
+    a.nb_all = NB;
+    a.split_max = split ? s2 : 0;
+    F.base.counts_all = a.counts_all;
+    F.base.nb_all = NB;
+    F.base.split_max = a.split_max;
     CompactArgs ca = compact_args(c, NB);
     ca.src = c->d_lists[1];
     ca.src_stride = c->cap_flights;
@@ -470,10 +488,20 @@ static int issue_chunks(pr_ctx* c, const pr_sim_params* p, FlyArgs a, long long 
     for (int ck = 0; ck < n_chunks; ck++) {
         const int cs = ck < PR_MAX_CHUNKS ? ck : PR_MAX_CHUNKS - 1;  // later chunks share the last slot
         F.c_in = ck & 1;
+        F.base.c_all = a.c_all = F.c_in;
         F.base.cstats = a.cstats = c->d_cstats + 2 * cs;
+        // the fused kernel's timed bracket (when it has one) includes the small-chunk launches
+        const bool tev = any_fused && timed && ne + 2 <= n_ev;
+        if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
+        if (split) {
+            F.split_lo = 1;
+            F.split_hi = s4;
+            HCK(launch_fly_split(F, 4, s4, c->stream));
+            F.split_lo = s4 + 1;
+            F.split_hi = s2;
+            HCK(launch_fly_split(F, 2, s2, c->stream));
+        }
         if (any_fused) {
-            const bool tev = timed && ne + 2 <= n_ev;
-            if (tev) HCK(hipEventRecord(c->ev[ne], c->stream));
             HCK(launch_fly_all(F, grid, c->stream));
             if (tev) { HCK(hipEventRecord(c->ev[ne + 1], c->stream)); c->ev_slot[ne / 2] = -1; c->ev_chunk[ne / 2] = cs; ne += 2; }
         }
@@ -520,6 +548,12 @@ int pr_set_fused(pr_ctx* c, int on) {
     return upload_buckets(c);
 }
 
+int pr_set_split(pr_ctx* c, int on) {
+    ARG(c, "pr_set_split: null context");
+    c->split = on ? 1 : 0;
+    return 0;
+}
+
 int pr_set_list_order(pr_ctx* c, int sorted) {
     ARG(c, "pr_set_list_order: null context");
     c->sort_lists = sorted ? 1 : 0;
@@ -552,7 +586,7 @@ int pr_fly_async(pr_ctx* c, int layout, int n_genomes, int n_scen, const pr_sim_
     long long nf;
     if (c->use_graph) {
         long long key[10] = {layout, n_genomes, n_scen, p->max_steps, p->chunk, (long long)(size_t)d_fitness,
-                             (long long)(size_t)d_steps, p->preset, (long long)c->buckets.size() + 64 * c->fused,
+                             (long long)(size_t)d_steps, p->preset, (long long)c->buckets.size() + 64 * c->fused + 128 * c->split,
                              c->n_prepared};
         if (!c->gexec || memcmp(key, c->gkey, sizeof key) != 0) {
             if (c->gexec) { HCK(hipGraphExecDestroy(c->gexec)); c->gexec = nullptr; }
@@ -629,8 +663,12 @@ int pr_fly_wait(pr_ctx* c) {
         HCK(hipEventElapsedTime(&t, c->g0, c->g1));
         S.fly_ms = t;
     }
-    unsigned long long st[4 * NB_MAX];
+    unsigned long long st[PR_STATS_N];
     HCK(hipMemcpy(st, c->d_stats, sizeof st, hipMemcpyDeviceToHost));
+    S.split_flight_steps = st[4 * NB_MAX];
+    S.split_panel_steps = st[4 * NB_MAX + 1];
+    S.flight_steps += S.split_flight_steps;
+    S.panel_steps += S.split_panel_steps;
     for (int k = 0; k < 2 * NB; k++) {
         S.s_flight_steps[k] = st[2 * k];
         S.s_panel_steps[k] = st[2 * k + 1];

@@ -176,6 +176,11 @@ int pr_gen_create(pr_gen** out, int pop, int nseeds_max, int max_ch, int max_sp)
     e->nseeds_max = nseeds_max;
     int rc = pr_create(&e->main, pop, 0, nseeds_max, pop * nseeds_max, max_ch, max_sp);
     for (int s = 0; s < 2 && !rc; s++) rc = pr_create(&e->hold[s].ctx, 8, 0, e->hold_n, 8 * e->hold_n, max_ch, max_sp);
+    // The holdout's small late chunks keep one wave per flight: a pipelined holdout shares
+    // the GPU with the next generation's population flights, where the small-chunk mapping's
+    // extra instructions (its rigid-body step runs in every wave of a flight) slowed the
+    // population more than they sped the holdout up (100k x 12 generation -3 %; 10k x 8 +1 %).
+    for (int s = 0; s < 2 && !rc; s++) rc = pr_set_split(e->hold[s].ctx, 0);
     if (rc) { pr_gen_destroy(e); return rc; }
     e->seeds.ensure(sizeof(int) * nseeds_max);
     e->fit.ensure(sizeof(float) * pop * nseeds_max);

@@ -352,9 +352,16 @@ __host__ __device__ constexpr int lds_slots(int P) { return P > 24 ? P - PR_LDS_
 __host__ __device__ constexpr int lds_bytes(int P) { return lds_slots(P) * 7 * 64 * 4; }
 
 // One group's flight: setup from the flight's constants/state/panels, then 600 Hz steps.
-template <int L, int P>
+// NW > 1 (L = 64): the flight's panels are spread over the NW waves of a workgroup (lane
+// threadIdx.x holds triangles threadIdx.x + s * 64 NW); each step the waves' partial force
+// and torque sums meet in LDS behind one barrier and every wave adds them in the same fixed
+// order, so all NW waves carry bitwise-identical flight states and take the same number of
+// steps (every barrier is reached by all of them).
+template <int L, int P, int NW = 1>
 struct Flyer {
     static_assert(L >= 8, "the gust terms need 8 lanes per group");
+    static_assert(NW == 1 || (L == 64 && lds_slots(P) == 0 && (NW == 2 || NW == 4)),
+                  "a flight over several waves: 2 or 4 full waves, panels in VGPRs");
     static constexpr int PQ = lds_slots(P);  // slots in LDS
     static constexpr int PV = P - PQ;        // slots in VGPRs
     static_assert(PQ == 0 || PV % 2 == 0, "the LDS slots continue the VGPR slots' pairs");
@@ -362,18 +369,24 @@ struct Flyer {
     WindLane W;
     float rx[PV], ry[PV], rz[PV], nx[PV], ny[PV], nz[PV], kk[PV];  // kk = cf rho A / 2
     float* lds;  // this lane's LDS column (PQ x 7 rows, 64 floats apart); only when PQ > 0
-    int np;  // panel slots this group uses per lane: ceil(nt / L) <= P
+    float* xsum;  // NW > 1: the workgroup's exchange block, float [2][NW][8]
+    int np;  // panel slots this group uses per lane: ceil(nt / (L NW)) <= P
     bool bad;
 
-    // smem: the wave's LDS panel block (lds_bytes(P)); unused when PQ == 0
+    // smem: the wave's LDS panel block (lds_bytes(P)); unused when PQ == 0.  NW > 1: the
+    // workgroup's exchange block instead
     __device__ __forceinline__ void setup(const FlyArgs& a, int fid, int j, float* smem) {
         lds = smem + (threadIdx.x & 63);
+        xsum = smem;
+        // this lane's first triangle and the stride between its slots
+        const int jt = NW > 1 ? (int)threadIdx.x : j;
+        constexpr int TS = L * NW;
         const FlightConst& fc = a.fc[fid];
         const FlightState& st = a.state[fid];
         const int gi = fc.genome;
         B.fid = fid;
         B.nt = a.nt[gi];
-        np = (B.nt + L - 1) / L;
+        np = (B.nt + TS - 1) / TS;
         const int off = a.off[gi];
         B.px = st.pos[0]; B.py = st.pos[1]; B.pz = st.pos[2];
         B.qw = st.q[0]; B.qx = st.q[1]; B.qy = st.q[2]; B.qz = st.q[3];
@@ -403,7 +416,7 @@ struct Flyer {
         }
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            const int t = j + p * L;
+            const int t = jt + p * TS;
             float v[7] = {0, 0, 0, 0, 0, 1, 0};  // padding: zero area, so zero force
             if (t < B.nt) {
                 const size_t o = (size_t)off + t;
@@ -533,6 +546,25 @@ struct Flyer {
         // force sums' last additions past the spin clamp's branch, where their DPP moves can
         // no longer fold into them (two extra VALU instructions per sum)
         asm volatile("" : "+v"(Fx), "+v"(Fy), "+v"(Fz));
+        if constexpr (NW > 1) {
+            // the waves' partial sums, double-buffered by step parity: a wave writes block
+            // (s & 1) at step s after passing step s - 1's barrier, which every wave reaches
+            // only after its reads of that block at step s - 2; this step's barrier orders
+            // the writes before the reads
+            float* xb = xsum + (B.step & 1) * (8 * NW);
+            if ((threadIdx.x & 63) == 0) {
+                float* o = xb + 8 * (threadIdx.x >> 6);
+                o[0] = Fx; o[1] = Fy; o[2] = Fz; o[3] = Tx; o[4] = Ty; o[5] = Tz;
+            }
+            __syncthreads();
+            float sm[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                if constexpr (NW == 2) sm[k] = xb[k] + xb[8 + k];
+                else sm[k] = (xb[k] + xb[8 + k]) + (xb[16 + k] + xb[24 + k]);
+            }
+            Fx = sm[0]; Fy = sm[1]; Fz = sm[2]; Tx = sm[3]; Ty = sm[4]; Tz = sm[5];
+        }
         // rigid-body update (sim.cu:177-197); torque is already in body axes
         float Fwx = r00 * Fx + r01 * Fy + r02 * Fz;
         float Fwy = r10 * Fx + r11 * Fy + r12 * Fz;
@@ -628,6 +660,28 @@ __device__ __forceinline__ int claim_xcd(int* xh, int lo, int hi) {
     return hi;
 }
 
+// End of a flight's chunk (one lane per flight): work counters, state, and the list mark
+// (the flight id if the flight goes on, -1 when it ended, with its fitness and step count)
+template <class FL>
+__device__ __forceinline__ void store_flight(const FlyArgs& a, const FL& F, int pos, bool fin) {
+    // algorithmic work counters: integrated steps and panel-steps of this chunk
+    const unsigned long long fs = (unsigned long long)(F.B.step - F.B.s0);
+    const unsigned long long ps = fs * (unsigned long long)F.B.nt;
+    atomicAdd(a.stats + 0, fs);
+    atomicAdd(a.stats + 1, ps);
+    if (a.cstats) {
+        atomicAdd(a.cstats + 0, fs);
+        atomicAdd(a.cstats + 1, ps);
+    }
+    const FlightState st = F.state(fin);
+    a.state[F.B.fid] = st;
+    if (fin) {
+        a.fitness[F.B.fid] = st.fitness;
+        a.steps[F.B.fid] = F.B.step;
+    }
+    a.marks[pos] = fin ? -1 : F.B.fid;
+}
+
 // xheads: PR_XCDS claim counters (XCD-local claiming) or null (one counter, head)
 template <int L, int P>
 __device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* head, float* smem,
@@ -665,27 +719,22 @@ __device__ __forceinline__ void fly_list(const FlyArgs& a, int lo, int hi, int* 
         if (!active) continue;
         const bool fin = F.step(a.max_steps, npw);
         if (fin || F.B.step >= F.B.end) {
-            if (j == 0) {
-                // algorithmic work counters: integrated steps and panel-steps of this chunk
-                const unsigned long long fs = (unsigned long long)(F.B.step - F.B.s0);
-                const unsigned long long ps = fs * (unsigned long long)F.B.nt;
-                atomicAdd(a.stats + 0, fs);
-                atomicAdd(a.stats + 1, ps);
-                if (a.cstats) {
-                    atomicAdd(a.cstats + 0, fs);
-                    atomicAdd(a.cstats + 1, ps);
-                }
-                const FlightState st = F.state(fin);
-                a.state[F.B.fid] = st;
-                if (fin) {
-                    a.fitness[F.B.fid] = st.fitness;
-                    a.steps[F.B.fid] = F.B.step;
-                }
-                a.marks[pos] = fin ? -1 : F.B.fid;
-            }
+            if (j == 0) store_flight(a, F, pos, fin);
             active = false;
         }
     }
+}
+
+// Flights over all buckets' lists of the chunk (counters slot c)
+__device__ __forceinline__ int chunk_total(const int* counts, int nb, int c) {
+    int n = 0;
+    for (int b = 0; b < nb; b++) n += counts[PR_CNT * b + c];
+    return n;
+}
+
+// true when this chunk belongs to k_fly_split, so the k_fly / k_fly_all launches skip it
+__device__ __forceinline__ bool chunk_is_split(const FlyArgs& a) {
+    return a.split_max > 0 && chunk_total(a.counts_all, a.nb_all, a.c_all) <= a.split_max;
 }
 
 template <int L, int P>
@@ -693,9 +742,44 @@ __global__ __launch_bounds__(64) void k_fly(FlyArgs a) {
     const int n_in = *a.count_in;
     // two mappings are queued per chunk (many-flights-per-wave, one-flight-per-wave); the
     // flight count decides which one works, so the choice needs no host synchronisation
-    if (n_in < a.run_lo || n_in > a.run_hi) return;
+    if (n_in < a.run_lo || n_in > a.run_hi || chunk_is_split(a)) return;
     __shared__ float smem[lds_bytes(P) > 0 ? lds_bytes(P) / 4 : 1];
     fly_list<L, P>(a, 0, n_in, a.head, smem);
+}
+
+// A small chunk (at most one flight per W SIMDs): one flight per workgroup of W waves, its
+// panels spread over all 64 W lanes, so a step's panel work is W times shorter than with one
+// wave per flight.  Positions are numbered across the buckets in bucket order; which
+// workgroup flies a position, and how, depends on the position and the chunk's flight count
+// only.  The whole workgroup takes the same branches: the position loop and the flight's
+// step loop are uniform (identical flight states in every wave).
+template <int W, int P>
+__global__ __launch_bounds__(64 * W) void k_fly_split(FlyAllArgs A) {
+    static_assert(64 * W * P >= PR_SPLIT_NT, "every mesh must fit the workgroup's panel slots");
+    __shared__ float xs[2 * 8 * W];
+    const int n = chunk_total(A.counts, A.nb, A.c_in);
+    if (n < A.split_lo || n > A.split_hi) return;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        int b = 0, k0 = 0;
+        while (b < A.nb - 1 && k >= k0 + A.counts[PR_CNT * b + A.c_in]) {
+            k0 += A.counts[PR_CNT * b + A.c_in];
+            b++;
+        }
+        const int pos = k - k0;
+        FlyArgs a = A.base;
+        a.list_in = A.lists + (size_t)b * A.list_stride;
+        a.marks = A.marks + (size_t)b * A.list_stride;
+        a.stats = A.split_stats;
+        Flyer<64, P, W> F;
+        F.setup(a, a.list_in[pos], threadIdx.x & 63, xs);
+        const int npw = __builtin_amdgcn_readfirstlane(F.np);
+        bool fin;
+        do {
+            fin = F.step(a.max_steps, npw);
+        } while (!fin && F.B.step < F.B.end);
+        if (threadIdx.x == 0) store_flight(a, F, pos, fin);
+        __syncthreads();  // the exchange block is reused by the next flight
+    }
 }
 
 // Variants compiled into the fused flight kernel (all fit 2 waves per SIMD): id, L, P.
@@ -722,6 +806,7 @@ constexpr int fused_lds_bytes() {
 // anyway, and the one-per-wave mapping costs more instructions per flight-step.
 __global__ __launch_bounds__(64, 2) void k_fly_all(FlyAllArgs A) {
     __shared__ float smem[fused_lds_bytes() > 0 ? fused_lds_bytes() / 4 : 1];
+    if (chunk_is_split(A.base)) return;
     // flights of the buckets drained after the current one (in this launch)
     int below = 0;
     for (int b = 0; b < A.nb; b++)
@@ -984,6 +1069,13 @@ int fused_variant_for(int L, int nt_max) {
 
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_fly_all, dim3(grid), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fly_split(const FlyAllArgs& a, int waves, int grid, hipStream_t s) {
+    if (waves == 4) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fly_split<4, 8>), dim3(grid), dim3(256), 0, s, a);
+    else if (waves == 2) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fly_split<2, 16>), dim3(grid), dim3(128), 0, s, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

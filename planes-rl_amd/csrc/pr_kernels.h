@@ -54,6 +54,11 @@ struct FlyArgs {
     float spread;
     unsigned long long* stats;  // [0] flight-steps, [1] panel-steps
     unsigned long long* cstats; // this chunk's [0] flight-steps, [1] panel-steps (null: none)
+    // split_max > 0: a chunk whose flights, summed over the nb_all buckets' counters
+    // counts_all[PR_CNT b + c_all], number at most split_max is flown by k_fly_split
+    // (several waves per flight), and k_fly / k_fly_all launches of that chunk do nothing
+    const int* counts_all;
+    int nb_all, c_all, split_max;
 };
 
 // Per-bucket chunk counters (PR_CNT ints per bucket): [0..1] list counts (chunk ck reads
@@ -88,6 +93,8 @@ struct FlyAllArgs {
     int widen;
     int lanes[8], vid_2[8], vid_4[8];
     unsigned long long* stats;  // slot 2b + (tail ? 1 : 0): flight-steps, panel-steps
+    int split_lo, split_hi;     // k_fly_split: flies the chunk if split_lo <= flights <= split_hi
+    unsigned long long* split_stats;  // k_fly_split's flight-steps, panel-steps
 };
 
 // Stable compaction of every bucket's marks into its next list (k_compact_*): keeps the
@@ -122,6 +129,10 @@ int fused_variant_id(int L, int P);
 // fused variant with L lanes and the fewest panel slots that hold nt_max triangles (-1: none)
 int fused_variant_for(int L, int nt_max);
 hipError_t launch_fly_all(const FlyAllArgs& a, int grid, hipStream_t s);
+// Small chunks: one flight per workgroup of `waves` (2 or 4) waves, every bucket's list
+// (positions numbered across the buckets in bucket order); triangles <= PR_SPLIT_NT.
+hipError_t launch_fly_split(const FlyAllArgs& a, int waves, int grid, hipStream_t s);
+constexpr int PR_SPLIT_NT = 2048;
 hipError_t fly_all_occupancy(int* blocks_per_cu);
 hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s);
 // perm[0..n) = genome indices stably sorted by panel-slot key (triangle count / 8)

@@ -46,11 +46,13 @@ class Stats(C.Structure):
                 ("s_panel_steps", C.c_ulonglong * NS_MAX), ("s_fly_ms", C.c_double * NS_MAX),
                 ("s_fused", C.c_int * NS_MAX), ("fused_ms", C.c_double), ("fused_launches", C.c_int),
                 ("n_chunks", C.c_int), ("c_flights", C.c_int * NC_MAX), ("c_flight_steps", C.c_ulonglong * NC_MAX),
-                ("c_panel_steps", C.c_ulonglong * NC_MAX), ("c_ms", C.c_double * NC_MAX)]
+                ("c_panel_steps", C.c_ulonglong * NC_MAX), ("c_ms", C.c_double * NC_MAX),
+                ("split_flight_steps", C.c_ulonglong), ("split_panel_steps", C.c_ulonglong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k in ("flight_steps", "panel_steps", "fly_ms", "fly_launches", "prep_ms",
-                                           "fused_ms", "fused_launches")}
+                                           "fused_ms", "fused_launches", "split_flight_steps",
+                                           "split_panel_steps")}
         d["buckets"] = [{"nt_max": self.s_nt_max[i], "L": self.s_L[i], "P": self.s_P[i], "tail": i % 2 == 1,
                          "fused": bool(self.s_fused[i]),
                          "launches": self.s_launches[i], "flight_steps": self.s_flight_steps[i],
@@ -109,6 +111,7 @@ def _load():
     lib.pr_set_tail_mapping.argtypes = [VP, C.c_int]
     lib.pr_set_fused.argtypes = [VP, C.c_int]
     lib.pr_set_list_order.argtypes = [VP, C.c_int]
+    lib.pr_set_split.argtypes = [VP, C.c_int]
     lib.pr_sim.argtypes = [VP, C.c_int, C.c_int, VP, VP, C.c_int, C.c_int, C.c_float, C.c_int, C.POINTER(SimParams)]
     lib.pr_design_stats.argtypes = [F32P, C.c_int, F32P]
     lib.pr_audit.argtypes = [VP, VP, VP, C.c_int, C.c_int, C.c_int, C.POINTER(SimParams), VP, VP, F32P, C.c_int]
@@ -374,6 +377,10 @@ class Planes:
         """Chunk 0's work lists in a stable order of mesh size (default) or flight order."""
         _ck(LIB.pr_set_list_order(self.h, int(sorted_by_size)), "pr_set_list_order")
 
+    def set_split(self, on=True):
+        """Small chunks flown one flight per 2- or 4-wave workgroup (default) or like the others."""
+        _ck(LIB.pr_set_split(self.h, int(on)), "pr_set_split")
+
     def set_graph(self, on=True):
         """Capture pr_fly's launch sequence as a hipGraph and replay it."""
         _ck(LIB.pr_set_graph(self.h, int(on)), "pr_set_graph")
@@ -518,6 +525,12 @@ class Generation:
         """Work-list order (pr_set_list_order) for every flight context of the generation."""
         for which in (0, 1, 2):
             _ck(LIB.pr_set_list_order(LIB.pr_gen_context(self.h, which), int(sorted_by_size)), "pr_set_list_order")
+
+    def set_split(self, on=True, holdout=False):
+        """Small-chunk mapping (pr_set_split) of the population context (default on) and of
+        the holdout contexts (default off: a pipelined holdout overlaps population flights)."""
+        for which, flag in ((0, on), (1, holdout), (2, holdout)):
+            _ck(LIB.pr_set_split(LIB.pr_gen_context(self.h, which), int(flag)), "pr_set_split")
 
     def set_graph(self, on=True):
         for which in (0, 1, 2):

@@ -170,6 +170,7 @@ def test_mapping_invariance(PA):
             continue
         pl = PA.Planes(len(gs), 2, 64)
         pl.set_tail_mapping(False)  # 64 flights would otherwise all take the one-per-wave tail mapping
+        pl.set_split(False)         # ... or the several-waves-per-flight small-chunk mapping
         pl.set_buckets([344], [L], [P])
         fit, steps = pl.evaluate(gs, seeds, params)
         if base is None:
@@ -195,6 +196,7 @@ def test_tail_mapping_equivalent(PA, n):
     for tail in (0, 1, 2):
         pl = PA.Planes(n, 8, n * 8)
         pl.set_tail_mapping(tail)
+        pl.set_split(False)
         fit, steps = pl.evaluate(gs, seeds, params)
         st = pl.stats()
         tail_steps = sum(b["flight_steps"] for b in st["buckets"] if b["tail"])
@@ -224,6 +226,7 @@ def test_fused_kernel_matches_per_bucket_launches(PA, buckets):
     for fused in (False, True):
         pl = PA.Planes(96, 8, 768)
         pl.set_tail_mapping(False)
+        pl.set_split(False)
         if buckets:
             pl.set_buckets(*buckets)
         pl.set_fused(fused)
@@ -424,6 +427,10 @@ def test_batched_audit_matches_single_design_flights_and_fp64(PA):
     for tail in (0, 2):
         pl.set_tail_mapping(tail)
         one.set_tail_mapping(tail)
+        # tail 0: one mapping per triangle count, so no small-chunk mapping either (whether
+        # a chunk is small depends on the batch); 2: the defaults
+        pl.set_split(tail == 2)
+        one.set_split(tail == 2)
         st, dist = pl.audit(gs, n_scen, base, params)
         for g in (0, 5, 11):
             fit, _ = one.evaluate(gs[g:g + 1], seeds, params)
@@ -484,6 +491,7 @@ def test_fine_meshes_largest_buckets_vs_fp64(PA, res):
     seeds = np.arange(1, 5, dtype=np.int32)
     params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=2400)
     pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=ch, max_sp=sp)
+    pl.set_split(False)  # 96 flights: every chunk would otherwise be a small-chunk one
     fit, steps = pl.evaluate(gs, seeds, params, res=r)
     mass, nt, nv, off = pl.read_prepared()
     for i in range(len(gs)):
@@ -521,6 +529,7 @@ def test_lds_panel_slots_vs_vgpr_mappings_and_fp64(PA, res):
     for name, table, fused in (("lds", lds_table, True), ("lds_own", lds_table, False), ("vgpr", vgpr_table, True)):
         pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=ch, max_sp=sp)
         pl.set_tail_mapping(False)
+        pl.set_split(False)
         pl.set_buckets(*table)
         pl.set_fused(fused)
         fit, steps = pl.evaluate(gs, seeds, params, res=r)
@@ -558,6 +567,7 @@ def test_pose_recording_with_lds_panel_slots(PA):
     seeds = np.array([3, 4], np.int32)
     pl = PA.Planes(len(gs), 2, 2 * len(gs), max_ch=ch, max_sp=sp)
     pl.set_tail_mapping(False)
+    pl.set_split(False)
     pl.set_buckets([528], [16], [33])
     fit, steps = pl.evaluate(gs, seeds, params, res=r)
     _, nt, _, _ = pl.read_prepared()
@@ -575,6 +585,48 @@ def test_pose_recording_with_lds_panel_slots(PA):
             # an ended flight keeps writing its final pose: x there is its fitness
             assert frames[f, dead[0], 0] == fr[f], (f, frames[f, dead[0]], fr[f], sr[f])
     pl.close()
+
+
+@pytest.mark.parametrize("n,res", [(24, (15, 16)), (100, (8, 12))])
+def test_split_small_chunks_vs_one_wave_mapping_and_fp64(PA, n, res):
+    """Chunks of at most one flight per 4 (2) SIMDs are flown one flight per workgroup of
+    4 (2) waves, the flight's panels over all their lanes and the partial sums added in LDS
+    each step.  24 x 4 flights: 4 waves per flight, meshes up to 1944 triangles (7.6 of the
+    8 panel slots per lane); 100 x 4: 2 waves per flight, 402 / 792 triangles.  With the
+    small-chunk kernel's work counters show that it flew every step; against the usual
+    mappings (pr_set_split 0): equal up to float reassociation; against the FP64 oracle:
+    within 1 %; repeated: bit-identical."""
+    ch, sp = res
+    gs = O.random_genomes(n, 61 + ch)
+    r = np.tile(np.array([[ch, sp]], np.int32), (len(gs), 1))
+    seeds = np.arange(1, 5, dtype=np.int32)
+    params = PA.SimParams.config(gust_rms_max=2.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=2400)
+    out = []
+    for split in (True, True, False):
+        pl = PA.Planes(len(gs), 4, 4 * len(gs), max_ch=ch, max_sp=sp)
+        pl.set_tail_mapping(False)
+        pl.set_split(split)
+        fit, steps = pl.evaluate(gs, seeds, params, res=r)
+        st = pl.stats()
+        out.append((fit.reshape(-1).copy(), steps.reshape(-1).copy(), st["split_flight_steps"], st["flight_steps"]))
+        if split:
+            _, nt, _, _ = pl.read_prepared()
+        pl.close()
+    (a, sa, ta, na), (a2, sa2, _, _), (b, sb, tb, nb) = out
+    assert ta == na == int(sa.sum()) > 0 and tb == 0 and nb == int(sb.sum())
+    assert np.array_equal(a, a2) and np.array_equal(sa, sa2)
+    if n * 4 <= 256:
+        assert nt.max() > 7 * 256  # the 4-wave workgroup's last panel slot in use
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
+    gi = np.repeat(np.arange(len(gs)), 4).astype(np.int32)
+    f64, _, _ = O.fly(gs, gi, np.tile(seeds, len(gs)), prec=64, preset=1, gust=2.0, warp=float(params.warp_max),
+                      wsd=2.0, max_steps=2400, ch=ch, sp=sp)
+    rel64 = np.abs(a - f64) / np.maximum(np.abs(f64), 1e-3)
+    print(f"{n * 4} flights, nt {nt.min()}..{nt.max()}: {np.mean(rel < 1e-3) * 100:.1f}% within 1e-3 of the "
+          f"one-wave mappings, {np.mean(rel64 < 0.01) * 100:.1f}% within 1% of FP64")
+    assert np.mean(rel < 1e-3) >= 0.9 and np.mean(rel < 1e-2) >= 0.97
+    assert np.mean(rel64 < 0.01) >= 0.95
+    assert np.all((a == -100) == (f64 == -100))
 
 
 def test_size_sorted_lists_order_and_results(PA):
