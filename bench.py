@@ -382,23 +382,6 @@ def audit_fp64_check(genomes, res, best, dist, n_scen=256):
             "fp64_wall_s": round(wall, 1)}
 
 
-def copy_probe(mib):
-    """A plain device-to-device copy of known size: reads mib MiB and writes mib MiB per
-    dispatch, so FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters' units."""
-    import torch
-
-    n = mib * (1 << 20) // 4
-    src = torch.arange(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    assert torch.equal(dst[-16:], src[-16:])
-    log(f"copy probe: 5 x {mib} MiB read + {mib} MiB written")
-    print(json.dumps({"copy_probe_mib": mib, "dispatches": 5, "bytes_read_per_dispatch": n * 4,
-                      "bytes_written_per_dispatch": n * 4}), flush=True)
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,11 +428,7 @@ def main():
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="c1-c3: finish each generation's holdout before the next generation starts")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="only run a plain device copy of this many MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
-    if args.copy_probe_mib > 0:
-        return copy_probe(args.copy_probe_mib)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
               f"(all-gather of fitness samples over RCCL) is not run here", file=sys.stderr)
