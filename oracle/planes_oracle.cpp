@@ -870,12 +870,14 @@ void or_wind_f64(double px, double py, double pz, double t, int seed, const floa
 }
 
 
-// Launch state of one flight (after sim.cu:70-109): out13 + printed mass20 + shift3
+// Launch state of one flight (after sim.cu:70-109): out13 + printed mass20 + shift3.
+// sc_override (optional, 26 floats): the scenario descriptor to launch under instead of seed's.
 #define LAUNCH_FN(NAME, R)                                                                               \
     void NAME(const float* g, int ch, int sp, int seed, int preset, float gust, float warp, float wsd,    \
-              int plane_index, float spread, R* st13, R* mp20, R* shift3, int* nt) {                    \
+              int plane_index, float spread, R* st13, R* mp20, R* shift3, int* nt,                     \
+              const float* sc_override) {                                                               \
         Prepared<R> p = prepare<R>(g, mkres(ch, sp));                                                   \
-        Flight<R> F = make_flight(p, seed, mkscfg(preset, gust, warp, wsd));                            \
+        Flight<R> F = make_flight(p, seed, mkscfg(preset, gust, warp, wsd), sc_override);               \
         SimCfg sc; sc.spread = spread;                                                                  \
         State<R> s; launch(F, s, plane_index, sc);                                                      \
         state_out(s, st13); mass_out(F.mp, mp20);                                                       \
@@ -1090,6 +1092,13 @@ void or_ga2_breed(void* h, const float* gs, const float* acc, const int* idx, in
             clampset(out, j, v);
         }
     }
+}
+
+// Raw 32-bit outputs of std::mt19937(seed) (the engine the GA draws from, sim.cu:398), so that
+// tests can check it against the standard's definition and its check value.
+void or_mt19937(unsigned seed, int n, uint32_t* out) {
+    std::mt19937 rng(seed);
+    for (int i = 0; i < n; i++) out[i] = (uint32_t)rng();
 }
 
 // percentile with linear interpolation (sim.cu:250-256)

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+# PLANES_ORACLE_SO: another build of the oracle (the planted-mistake tests load mutated builds)
+ORACLE_SO = os.environ.get("PLANES_ORACLE_SO") or os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libplanes_ref.so")
 
 N_GENES = 35
@@ -36,7 +37,7 @@ def _load():
         getattr(lib, f"or_panels_{fx}").argtypes = [F32P, C.c_int, C.c_int, P, P, P, C.c_int]
         getattr(lib, f"or_printed_{fx}").argtypes = [P, F32P, P, P]
         getattr(lib, f"or_launch_{fx}").argtypes = [F32P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
-                                                    C.c_float, C.c_int, C.c_float, P, P, P, IP]
+                                                    C.c_float, C.c_int, C.c_float, P, P, P, IP, C.c_void_p]
         getattr(lib, f"or_traj_{fx}").argtypes = [F32P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                                   C.c_float, C.c_int, C.c_int, P, C.c_void_p, C.c_int]
         getattr(lib, f"or_traj_{fx}").restype = C.c_int
@@ -167,15 +168,40 @@ def wind(p, t, seed, sc26, prec=32):
     return out
 
 
-def launch(g, seed, prec=32, preset=0, gust=-1.0, warp=-1.0, wsd=-1.0, plane_index=0, spread=0.0, ch=0, sp=0):
+def launch(g, seed, prec=32, preset=0, gust=-1.0, warp=-1.0, wsd=-1.0, plane_index=0, spread=0.0, ch=0, sp=0,
+           scenario_override=None):
+    """Launch state (pos3 q4 vel3 wb3), printed mass properties, CG shift and triangle count of
+    one flight; scenario_override: a 26-field descriptor to launch under instead of seed's."""
     g = np.ascontiguousarray(g, np.float32)
     st = np.zeros(13, _dt(prec))
     mp = np.zeros(20, _dt(prec))
     sh = np.zeros(3, _dt(prec))
     nt = C.c_int()
+    ov = None if scenario_override is None else np.ascontiguousarray(scenario_override, np.float32)
     getattr(lib(), f"or_launch_f{prec}")(g, ch, sp, int(seed), preset, gust, warp, wsd, plane_index, spread,
-                                         st, mp, sh, C.byref(nt))
+                                         st, mp, sh, C.byref(nt), None if ov is None else ov.ctypes.data)
     return st, mp, sh, nt.value
+
+
+def mt19937(seed, n):
+    """The first n raw outputs of std::mt19937(seed), the GA's engine."""
+    out = np.zeros(n, np.uint32)
+    lib().or_mt19937(C.c_uint(seed), n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def ga_breed(gs, acc, idx, gen, gens, seed=42, state=None):
+    """One breeding pass (sim.cu:472-490) with the reference's persistent distributions; state:
+    a handle from ga_new() to continue a stream, else a fresh mt19937(seed)."""
+    h = state if state is not None else lib().or_ga2_new(seed)
+    gs = np.ascontiguousarray(gs, np.float32)
+    pop = len(gs)
+    nxt = np.zeros_like(gs)
+    lib().or_ga2_breed(h, gs.reshape(-1), np.ascontiguousarray(acc, np.float32), np.ascontiguousarray(idx, np.int32),
+                       pop, gen, gens, nxt.reshape(-1))
+    if state is None:
+        lib().or_ga2_free(h)
+    return nxt
 
 
 def trajectory(g, seed, nsteps, prec=32, preset=0, gust=-1.0, warp=-1.0, wsd=-1.0, max_steps=0, ch=0, sp=0,
