@@ -223,6 +223,11 @@ pr_ctx* pr_gen_context(pr_gen* e, int which);
 int pr_gen_evaluate_async(pr_gen* e, const float* d_genomes, const int* d_res, int pop, const int* seeds,
                           int nseeds, int holdout_k, const pr_sim_params* p, float* acc, int* idx, float* samples);
 int pr_gen_collect(pr_gen* e, float* holdout_out, int* champion_out);
+/* Flight distances of genomes [first, first + count) of the last population batch (row-major,
+ * one row of nseeds per genome), read after the fact: they stay on the device until the next
+ * evaluation, so a caller can check a timed generation's own results without slowing it.
+ * No reference counterpart (run_ga keeps its samples on the host, sim.cu:436-439). */
+int pr_gen_read_samples(pr_gen* e, int first, int count, float* out);
 
 /* ---------------------------------------------------------------- surrogate MLP
  * BF16 MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU) on the matrix cores, FP32 accumulation,

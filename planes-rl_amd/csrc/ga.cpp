@@ -100,6 +100,7 @@ struct pr_gen {
     int oldest = -1;         // oldest uncollected slot (-1: none)
     std::vector<float> samples, acc_tmp;
     pr_stats st_main = {}, st_hold = {};
+    int last_pop = 0, last_nseeds = 0;  // shape of the last population batch (its samples stay in fit)
 };
 
 extern "C" {
@@ -235,6 +236,8 @@ static int gen_main(pr_gen* e, const float* d_genomes, const int* d_res, int pop
         if (samples) std::memcpy(samples, e->samples.data(), sizeof(float) * pop * nseeds);
     }
     pr_ga_rank(acc, pop, idx);
+    e->last_pop = pop;
+    e->last_nseeds = nseeds;
     return 0;
 }
 
@@ -318,6 +321,14 @@ int pr_gen_evaluate_async(pr_gen* e, const float* d_genomes, const int* d_res, i
     if (e->oldest < 0) e->oldest = s;
     e->next_slot = s ^ 1;
     return 0;
+}
+
+// Samples of genomes [first, first + count) of the last population batch, row-major by seed;
+// they stay on the device until the next evaluation, so reading them costs the evaluation nothing.
+int pr_gen_read_samples(pr_gen* e, int first, int count, float* out) {
+    if (!e || !out || first < 0 || count < 0 || e->last_pop == 0 || first + count > e->last_pop) return -1;
+    const size_t ns = (size_t)e->last_nseeds;
+    return pr_d2h(out, (const float*)e->fit.p + (size_t)first * ns, sizeof(float) * (size_t)count * ns);
 }
 
 // Result of the oldest outstanding pr_gen_evaluate_async holdout batch (waits for it).

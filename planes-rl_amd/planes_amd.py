@@ -119,6 +119,7 @@ def _load():
     lib.pr_gen_evaluate_async.argtypes = [VP, VP, VP, C.c_int, I32P, C.c_int, C.c_int, C.POINTER(SimParams), F32P,
                                           I32P, VP]
     lib.pr_gen_collect.argtypes = [VP, C.POINTER(C.c_float), C.POINTER(C.c_int)]
+    lib.pr_gen_read_samples.argtypes = [VP, C.c_int, C.c_int, VP]
     lib.pr_gen_context.restype = VP
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
     lib.pr_mlp_last_error.restype = C.c_char_p
@@ -562,6 +563,12 @@ class Generation:
         if samples is not None:
             out["samples"] = samples.reshape(pop, len(seeds))
         return out
+
+    def read_samples(self, first, count, nseeds):
+        """Flight distances [count x nseeds] of genomes first.. of the last population batch."""
+        out = np.zeros(count * nseeds, np.float32)
+        _ck(LIB.pr_gen_read_samples(self.h, int(first), int(count), out.ctypes.data), "pr_gen_read_samples")
+        return out.reshape(count, nseeds)
 
     def collect(self):
         """Holdout score and champion of the oldest outstanding evaluate_async."""

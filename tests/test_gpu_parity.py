@@ -587,13 +587,16 @@ def test_pose_recording_with_lds_panel_slots(PA):
     pl.close()
 
 
-@pytest.mark.parametrize("n,res", [(24, (15, 16)), (100, (8, 12))])
+@pytest.mark.parametrize("n,res", [(24, (15, 16)), (100, (8, 12)), (120, (15, 16)), (400, (8, 12))])
 def test_split_small_chunks_vs_one_wave_mapping_and_fp64(PA, n, res):
     """Chunks of at most one flight per 4 (2) SIMDs are flown one flight per workgroup of
     4 (2) waves, the flight's panels over all their lanes and the partial sums added in LDS
     each step.  24 x 4 flights: 4 waves per flight, meshes up to 1944 triangles (7.6 of the
-    8 panel slots per lane); 100 x 4: 2 waves per flight, 402 / 792 triangles.  With the
-    small-chunk kernel's work counters show that it flew every step; against the usual
+    8 panel slots per lane); 100 x 4: 2 waves per flight, 402 / 792 triangles; 120 x 4 (332
+    printable flights): 2 waves per flight on meshes up to 1944 triangles (all 16 panel slots
+    of the 2-wave kernel); 400 x 4 (1220 printable flights): chunk 0 flown by the fused kernel,
+    the later chunks, once at most 512 flights remain, by the small-chunk kernel (the hand-off
+    inside one evaluation).  The work counters show which kernel flew which steps; against the usual
     mappings (pr_set_split 0): equal up to float reassociation; against the FP64 oracle:
     within 1 %; repeated: bit-identical."""
     ch, sp = res
@@ -613,10 +616,17 @@ def test_split_small_chunks_vs_one_wave_mapping_and_fp64(PA, n, res):
             _, nt, _, _ = pl.read_prepared()
         pl.close()
     (a, sa, ta, na), (a2, sa2, _, _), (b, sb, tb, nb) = out
-    assert ta == na == int(sa.sum()) > 0 and tb == 0 and nb == int(sb.sum())
+    assert na == int(sa.sum()) > 0 and tb == 0 and nb == int(sb.sum())
+    nfl = 4 * int((nt[:len(gs)] > 0).sum())  # flights entering chunk 0 (unprintable designs do not fly)
+    if nfl <= 512:
+        assert ta == na  # every chunk small: the small-chunk kernel flew every step
+    else:
+        assert 0 < ta < na  # chunk 0 by the fused kernel, the late chunks by the small-chunk kernel
     assert np.array_equal(a, a2) and np.array_equal(sa, sa2)
-    if n * 4 <= 256:
+    if nfl <= 256:
         assert nt.max() > 7 * 256  # the 4-wave workgroup's last panel slot in use
+    elif nfl <= 512 and ch >= 15:
+        assert nt.max() > 15 * 128  # the 2-wave workgroup's last panel slot in use
     rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3)
     gi = np.repeat(np.arange(len(gs)), 4).astype(np.int32)
     f64, _, _ = O.fly(gs, gi, np.tile(seeds, len(gs)), prec=64, preset=1, gust=2.0, warp=float(params.warp_max),
