@@ -290,6 +290,22 @@ int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
  * [n_fly x 4]; null outputs are skipped; *n_fly = fly-set size. */
 int pr_pool_last(pr_pool* p, float* pool, float* fly, float* scores, float* targets, int* n_fly);
 
+/* ---------------------------------------------------------------- allocation-free evaluation
+ * No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP stream capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * evaluation as pr_prepare + pr_scenarios + pr_fly with layout 0 (flight f = genome f / n_seeds
+ * under seed f % n_seeds; results bit-identical to that path on a fresh context), at the
+ * reference mesh resolution, queued on `stream` (a hipStream_t; null = the null stream) and
+ * returning without waiting: d_fitness / d_steps are complete when the stream is.  Every
+ * intermediate buffer lives in one 32 MiB device array inside the library, which each device
+ * holds until the process ends; the call allocates nothing.  Limits: n_genomes <= 2048,
+ * n_seeds <= 16 (-1 and a message beyond them).  d_genomes [n_genomes*35], d_seeds
+ * [n_seeds] (device int32), d_fitness [n_genomes*n_seeds], d_steps (may be null) on the
+ * device.  Calls that overlap on one device race for the array: serialise them (one stream,
+ * or one host thread per device). */
+int pr_eval_noalloc(const float* d_genomes, int n_genomes, const int* d_seeds, int n_seeds, const pr_sim_params* p,
+                    float* d_fitness, int* d_steps, void* stream);
+
 /* Device memory helpers so that hosts without a GPU array library can drive the API. */
 int pr_dev_alloc(void** p, size_t bytes);
 int pr_dev_free(void* p);

@@ -316,7 +316,7 @@ int pr_fly_variants(int* n, int* L, int* P, int cap) {
 int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
     ARG(c && d_genomes, "pr_prepare: null argument");
     ARG(n > 0 && n <= c->cap_genomes, "pr_prepare: n outside 1..max_genomes");
-    HCK(hipEventRecord(c->prep0, c->stream));
+    if (c->prep0) HCK(hipEventRecord(c->prep0, c->stream));  // untimed in the allocation-free path
     HCK(launch_layout(d_genomes, d_res, 5, 8, n, c->d_ntraw, c->stream));
     HCK(launch_zero(c->d_ntraw + n, 1, c->stream));
     size_t tb = c->scan_tmp_bytes;
@@ -335,7 +335,7 @@ int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
     a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
     a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];
     HCK(launch_prepare(a, n, c->prep_fp64, c->stream));
-    HCK(hipEventRecord(c->prep1, c->stream));
+    if (c->prep1) HCK(hipEventRecord(c->prep1, c->stream));
     c->n_prepared = n;
     return 0;
 }
@@ -788,6 +788,111 @@ int pr_read_states(pr_ctx* c, int first, int n, float* out16) {
     HCK(hipStreamSynchronize(c->stream));
     HCK(hipMemcpy(out16, c->d_state + first, sizeof(FlightState) * n, hipMemcpyDeviceToHost));
     return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free evaluation
+// pr_eval_noalloc: pr_prepare + pr_scenarios + pr_fly (layout 0) for callers that cannot
+// allocate device memory (code under stream capture, tight serving loops).  Every buffer the
+// flight path needs beyond the caller's inputs and outputs -- mesh counts and offsets, mass
+// properties, packed panels, scenario descriptors, per-flight constants and states, work
+// lists, counters, the scan's workspace -- is carved from one file-scope device array.  Each
+// device holds its own copy of the array (at its own address, found once per device with
+// hipGetSymbolAddress) until the process ends.  Calls that overlap on one device race for it.
+static const int PR_NA_GENOMES = 2048;   // genomes per call (reference resolution CH=5, SP=8)
+static const int PR_NA_SEEDS = 16;       // scenarios per call
+static const int PR_NA_FLIGHTS = PR_NA_GENOMES * PR_NA_SEEDS;
+static const size_t PR_NA_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[PR_NA_BYTES];
+
+static const int PR_NA_MAX_DEVICES = 16;
+struct NoAllocShell {
+    int ready = 0;
+    pr_ctx ctx;
+};
+static NoAllocShell g_na[PR_NA_MAX_DEVICES];
+
+// carves the context's buffers out of the device's scratch array; 256-byte aligned pieces
+static int noalloc_setup(int dev, pr_ctx* c) {
+    unsigned char* base = nullptr;
+    HCK(hipGetSymbolAddress((void**)&base, HIP_SYMBOL(scratch)));
+    c->max_ch = 5;
+    c->max_sp = 8;
+    int nv, nt;
+    mesh_bounds(c->max_ch, c->max_sp, &nv, &nt);
+    c->nv_max = nv;
+    c->cap_genomes = PR_NA_GENOMES;
+    c->cap_panels = (long long)PR_NA_GENOMES * nt;
+    c->cap_scen = PR_NA_SEEDS;
+    c->cap_flights = PR_NA_FLIGHTS;
+    c->max_tiles = (PR_NA_FLIGHTS + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE;
+    int rc;
+    if ((rc = default_buckets(c))) return rc;
+    size_t used = 0;
+    bool fits = true;
+    auto take = [&](size_t bytes) -> void* {
+        void* p = base + used;
+        used += (bytes + 255) & ~(size_t)255;
+        if (used > PR_NA_BYTES) fits = false;
+        return p;
+    };
+    const int G = PR_NA_GENOMES, F = PR_NA_FLIGHTS;
+    c->d_ntraw = (int*)take(sizeof(int) * (G + 1));
+    c->d_off = (int*)take(sizeof(int) * (G + 1));
+    c->d_nt = (int*)take(sizeof(int) * G);
+    c->d_nv = (int*)take(sizeof(int) * G);
+    c->d_ext = (float*)take(sizeof(float) * 3 * G);
+    c->d_mass = (Mass*)take(sizeof(Mass) * G);
+    for (int k = 0; k < 7; k++) c->d_panel[k] = (float*)take(sizeof(float) * c->cap_panels);
+    c->d_seeds = (int*)take(sizeof(int) * PR_NA_SEEDS);
+    c->d_scen = (float*)take(sizeof(float) * NSF * PR_NA_SEEDS);
+    c->d_state = (FlightState*)take(sizeof(FlightState) * F);
+    c->d_steps_own = (int*)take(sizeof(int) * F);
+    c->d_fc = (FlightConst*)take(sizeof(FlightConst) * F);
+    for (int k = 0; k < 2; k++) c->d_lists[k] = (int*)take(sizeof(int) * (size_t)NB_MAX * F);
+    c->d_counts = (int*)take(sizeof(int) * PR_CNT * NB_MAX);
+    c->d_tcount = (int*)take(sizeof(int) * NB_MAX * (size_t)c->max_tiles);
+    c->d_perm = (int*)take(sizeof(int) * G);
+    c->d_bucket_nt = (int*)take(sizeof(int) * NB_MAX);
+    c->d_stats = (unsigned long long*)take(sizeof(unsigned long long) * PR_STATS_N);
+    c->d_cstats = (unsigned long long*)take(sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
+    c->d_chunk_n = (int*)take(sizeof(int) * PR_MAX_CHUNKS * NB_MAX);
+    HCK(hipcub::DeviceScan::ExclusiveSum(nullptr, c->scan_tmp_bytes, c->d_ntraw, c->d_off, G + 1, (hipStream_t)0));
+    c->d_scan_tmp = take(c->scan_tmp_bytes + 16);
+    if (!fits) return fail(-1, "pr_eval_noalloc: scratch layout exceeds the array (build error)");
+    c->stream = nullptr;  // set-up on the null stream; each call then runs on the caller's stream
+    if ((rc = upload_buckets(c))) return rc;
+    (void)dev;
+    return 0;
+}
+
+int pr_eval_noalloc(const float* d_genomes, int n_genomes, const int* d_seeds, int n_seeds, const pr_sim_params* p,
+                    float* d_fitness, int* d_steps, void* stream) {
+    ARG(d_genomes && d_seeds && p && d_fitness, "pr_eval_noalloc: null argument");
+    ARG(n_genomes >= 1 && n_genomes <= PR_NA_GENOMES,
+        "pr_eval_noalloc: n_genomes outside 1..2048 (the scratch array's capacity)");
+    ARG(n_seeds >= 1 && n_seeds <= PR_NA_SEEDS, "pr_eval_noalloc: n_seeds outside 1..16 (the scratch array's capacity)");
+    ARG(p->max_steps > 0 && p->chunk > 0, "pr_eval_noalloc: max_steps and chunk must be positive");
+    int dev = 0;
+    HCK(hipGetDevice(&dev));
+    ARG(dev >= 0 && dev < PR_NA_MAX_DEVICES, "pr_eval_noalloc: device index beyond 15");
+    NoAllocShell& sh = g_na[dev];
+    pr_ctx* c = &sh.ctx;
+    if (!sh.ready) {
+        int rc = noalloc_setup(dev, c);
+        if (rc) return rc;
+        sh.ready = 1;
+    }
+    c->stream = (hipStream_t)stream;
+    int rc;
+    if ((rc = pr_prepare(c, d_genomes, nullptr, n_genomes))) return rc;
+    if ((rc = pr_scenarios(c, d_seeds, n_seeds, p))) return rc;
+    FlyArgs a;
+    int* steps = d_steps ? d_steps : c->d_steps_own;
+    fill_fly_args(c, p, d_fitness, steps, a);
+    long long nf = 0;
+    if ((rc = init_flights(c, 0, n_genomes, n_seeds, p, d_fitness, steps, &nf))) return rc;
+    int ne = 0;
+    return issue_chunks(c, p, a, nf, false, &ne, false);
 }
 
 int pr_dev_alloc(void** p, size_t bytes) { HCK(hipMalloc(p, bytes)); return 0; }

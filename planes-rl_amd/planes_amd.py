@@ -120,6 +120,7 @@ def _load():
                                           I32P, VP]
     lib.pr_gen_collect.argtypes = [VP, C.POINTER(C.c_float), C.POINTER(C.c_int)]
     lib.pr_gen_read_samples.argtypes = [VP, C.c_int, C.c_int, VP]
+    lib.pr_eval_noalloc.argtypes = [VP, C.c_int, VP, C.c_int, C.POINTER(SimParams), VP, VP, VP]
     lib.pr_gen_context.restype = VP
     lib.pr_set_prepare_precision.argtypes = [VP, C.c_int]
     lib.pr_mlp_last_error.restype = C.c_char_p
