@@ -151,45 +151,62 @@ def pretrain_surrogate(PA, S, n_pairs, params, batch=250000, seeds_per=8, epochs
             "final_loss": round(float(loss), 4)}
 
 
-def pool_ab(PA, args):
-    """C4: GA generations/hour with surrogate pre-screening vs the plain batched sim.
-    plain: every generation flies the whole pool of mutants; surrogate: the network scores
-    the pool and only the elite + best-predicted candidates (plus an exploration sample)
-    are flown (n_fly); both keep the best simulated genomes as the next elite and report
-    the holdout champion (best holdout score so far).  The network is pretrained once on
-    args.pretrain sim pairs (setup, untimed, cost reported); each surrogate run starts from
-    those weights with a fresh optimizer and fine-tunes on its own flights.  With
-    --pretrain 0 the surrogate arm's first generation is instead a plain one whose flights
-    bootstrap the network (its time counts).  Both arms run args.reps GA seeds; the line
-    reports their totals and means (single runs differ by ~0.5 m of champion score).
-    The plain arm runs args.gens generations, the surrogate arm 3x as many (--sur-gens):
-    champion_ratio compares the champions after args.gens generations each;
-    time_to_plain_champion gives, per GA seed, the first surrogate generation whose
-    champion reaches 99 % of the plain arm's and the wall-time ratio of the two arms to
-    that point (the generations/hour gain at unchanged champion fitness)."""
-    n_pool, n_fly, n_elite, k = args.pool, args.fly, args.elite, 12
+ARMS = {
+    # name: (flies, selection) -- what each arm of the surrogate A/B flies per generation
+    "plain": "the whole pool (no pre-screen)",
+    "surrogate": "the elite + the network's choice: parent's simulated score + predicted parent-to-mutant change, "
+                 "with an exploration sample",
+    "strict": "the elite + the top candidates by the network's predicted score alone (no exploration sample)",
+    "uniform": "the elite + a uniform random sample of the mutants (no network)",
+    "parent": "the elite + the mutants of the best-scoring parents (parent's simulated score as the key, no network)",
+}
+
+
+def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, gens=4, sur_gens=None, pretrain=2_000_000,
+                 explore=0.3, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
+                 arms=("plain", "surrogate", "strict", "uniform", "parent")):
+    """GA generations/hour with surrogate pre-screening against the plain batched sim, and
+    against the same fly budget chosen without the network.  One generation: a pool of n_pool
+    Gaussian mutants (sigma 5 % of each gene's range) of an n_elite elite; every arm flies its
+    fly set under 12 environments (gusts <= 4 m/s RMS, 12 s cap), keeps the best simulated
+    genomes as the next elite and re-flies its top 8 on the 128-scenario holdout suite.  The
+    arms (ARMS): "plain" flies all n_pool candidates; the others fly n_fly.  The network is
+    pretrained once on `pretrain` sim pairs of the family generator (8 seeds each; setup,
+    untimed, its cost reported and also added to the network arms' time to the plain
+    champion), and each network run starts from those weights and fine-tunes on its own
+    flights.  With pretrain 0 a network arm's first generation is a plain one whose flights
+    bootstrap the network (timed).  Every arm runs `reps` GA seeds from the same initial
+    elite; the plain arm `gens` generations, the others `sur_gens` (default 3 x gens).
+    Reported per arm: generations/hour, champions (best holdout score so far) at the plain
+    arm's generation count and at the end, and per GA seed the first generation whose champion
+    reaches 99 % of that seed's plain champion, with the wall time of both arms to that point."""
+    sur_gens = sur_gens or 3 * gens
     params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=7200)
     g0, _, _ = PA.family_population(n_elite, seed=0)
-    pretrained = args.pretrain > 0
-    lr = args.lr if args.lr is not None else (1e-4 if pretrained else 2e-3)
-    tepochs = args.train_epochs if args.train_epochs is not None else (2 if pretrained else 4)
-    pre, p0 = None, None
-    if pretrained:
-        S = PA.Surrogate(max_batch=args.mlp_batch, seed=3)
-        pre = pretrain_surrogate(PA, S, args.pretrain, params)
+    pretrained = pretrain > 0
+    net_arms = [a for a in arms if a in ("surrogate", "strict")]
+    lr = lr if lr is not None else (1e-4 if pretrained else 2e-3)
+    tepochs = train_epochs if train_epochs is not None else (2 if pretrained else 4)
+    pre, p0, pre_s = None, None, 0.0
+    if pretrained and net_arms:
+        S = PA.Surrogate(max_batch=mlp_batch, seed=3)
+        pre = pretrain_surrogate(PA, S, pretrain, params)
         p0 = S.params()
+        pre_s = pre["sim_s"] + pre["train_s"]
         log(f"[surrogate] pretraining: {pre}")
-    runs = {"plain": [], "surrogate": []}
-    for rep in range(args.reps):
-        for mode in ("plain", "surrogate"):
-            ga = PA.PoolGA(n_elite, n_pool, n_fly if mode == "surrogate" else n_pool, k, buf_cap=args.buffer,
-                           sigma_frac=0.05, seed=7 + rep)
+    runs = {a: [] for a in arms}
+    for rep in range(reps):
+        for arm in arms:
+            ga = PA.PoolGA(n_elite, n_pool, n_pool if arm == "plain" else n_fly, 12, buf_cap=buffer, sigma_frac=0.05,
+                           seed=7 + rep)
             ga.set_elite(g0)
-            S = None
-            gens = []
-            if mode == "surrogate":
-                S = PA.Surrogate(max_batch=args.mlp_batch, seed=3)
-                ga.set_explore(args.explore)
+            S, out = None, []
+            if arm in ("uniform", "parent"):
+                ga.set_selection(arm)
+            elif arm in ("surrogate", "strict"):
+                S = PA.Surrogate(max_batch=mlp_batch, seed=3)
+                ga.set_explore(explore if arm == "surrogate" else 0.0)
+                ga.set_parent_delta(arm == "surrogate")
                 if pretrained:
                     S.set_params(p0)
                 else:
@@ -197,69 +214,94 @@ def pool_ab(PA, args):
                     r = ga.generation(0, params, None)
                     ga.set_collect(False)
                     t0 = time.perf_counter()
-                    r["loss"] = ga.train(S, epochs=args.boot_epochs, lr=lr)
+                    r["loss"] = ga.train(S, epochs=boot_epochs, lr=lr)
                     r["wall_ms"] += 1000 * (time.perf_counter() - t0)
-                    gens.append(r)
-                    log(f"[{mode} {rep}] gen 0 (bootstrap): best {r['best']:.3f} holdout {r['holdout']:.3f} "
-                        f"{r['wall_ms']:.0f} ms, training loss {r['loss']:.3f}")
-            n_gens = args.gens if mode == "plain" else max(args.gens, args.sur_gens or 3 * args.gens)
-            for gen in range(len(gens), n_gens):
+                    out.append(r)
+            n_gens = gens if arm == "plain" else max(gens, sur_gens)
+            for gen in range(len(out), n_gens):
                 r = ga.generation(gen, params, S, train_epochs=tepochs, lr=lr)
-                gens.append(r)
-                log(f"[{mode} {rep}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
+                out.append(r)
+                log(f"[{arm} {rep}] gen {gen}: best {r['best']:.3f} elite {r['elite_mean']:.3f} holdout "
                     f"{r['holdout']:.3f} (best {r['best_holdout']:.3f}) {r['wall_ms']:.0f} ms (fly {r['fly_ms']:.0f}, "
-                    f"mlp {r['mlp_ms']:.0f}) rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m "
-                    f"oversize acc {r['oversize_acc']:.3f}")
-            runs[mode].append(gens)
-    # time to the plain arm's champion: the first surrogate generation whose champion (best
-    # holdout so far) reaches 99 % of the same GA seed's plain champion after args.gens
-    # generations, and the wall time the two arms took to get there
-    match = []
-    for rep in range(args.reps):
-        pg, sg = runs["plain"][rep], runs["surrogate"][rep]
-        target, wp = pg[-1]["best_holdout"], sum(r["wall_ms"] for r in pg) / 1000.0
-        ws, hit = 0.0, None
-        for i, r in enumerate(sg):
-            ws += r["wall_ms"] / 1000.0
-            if r["best_holdout"] >= 0.99 * target:
-                hit = {"gen": i + 1, "wall_s": round(ws, 3)}
-                break
-        match.append({"plain_champion": round(target, 4), "plain_gens": len(pg), "plain_wall_s": round(wp, 3),
-                      "surrogate_reached_at": hit, "speedup": round(wp / hit["wall_s"], 2) if hit else None})
-    sur_equal = [gens[min(args.gens, len(gens)) - 1]["best_holdout"] for gens in runs["surrogate"]]
-    out = {}
-    for mode, reps in runs.items():
-        allg = [r for gens in reps for r in gens]
+                    f"mlp {r['mlp_ms']:.0f})" + (f" rank-corr {r['rank_corr']:.3f} MAE {r['mae']:.2f} m" if S else ""))
+            runs[arm].append(out)
+            del ga
+    summary = {}
+    for arm, reps_out in runs.items():
+        allg = [r for g in reps_out for r in g]
         wall = sum(r["wall_ms"] for r in allg) / 1000.0
         fin = lambda key: [r[key] for r in allg if r[key] == r[key]]
-        champs = [gens[-1]["best_holdout"] for gens in reps]
-        out[mode] = {"gens": len(allg), "reps": len(reps), "wall_s": round(wall, 3),
-                     "gens_per_hour": round(3600.0 * len(allg) / wall, 1),
-                     "champion_holdout_mean": round(float(np.mean(champs)), 4),
-                     "champion_holdout": [round(c, 4) for c in champs], "flights_per_gen": allg[-1]["flights"],
-                     "mean_rank_corr": round(float(np.mean(fin("rank_corr"))), 3) if fin("rank_corr") else None,
-                     "mean_mae_m": round(float(np.mean(fin("mae"))), 3) if fin("mae") else None,
-                     "mean_oversize_acc": round(float(np.mean(fin("oversize_acc"))), 3) if fin("oversize_acc") else None,
-                     "mlp_ms_per_gen": round(float(np.mean([r["mlp_ms"] for r in allg])), 1)}
-    ratio = out["surrogate"]["gens_per_hour"] / out["plain"]["gens_per_hour"]
-    champ = float(np.mean(sur_equal)) / out["plain"]["champion_holdout_mean"]
-    out["surrogate"]["champion_holdout_at_plain_gens"] = [round(c, 4) for c in sur_equal]
-    hits = [m["speedup"] for m in match if m["speedup"] is not None]
-    line = {"metric": METRIC, "value": out["surrogate"]["gens_per_hour"], "unit": "GA generations/hour",
+        d = {"flies": ARMS[arm], "gens": len(allg), "wall_s": round(wall, 3),
+             "gens_per_hour": round(3600.0 * len(allg) / wall, 1), "flights_per_gen": allg[-1]["flights"],
+             "champion_at_plain_gens": [round(g[min(gens, len(g)) - 1]["best_holdout"], 4) for g in reps_out],
+             "champion_final": [round(g[-1]["best_holdout"], 4) for g in reps_out],
+             "mlp_ms_per_gen": round(float(np.mean([r["mlp_ms"] for r in allg])), 1)}
+        if arm in net_arms:
+            d["mean_rank_corr"] = round(float(np.mean(fin("rank_corr"))), 3) if fin("rank_corr") else None
+            d["mean_mae_m"] = round(float(np.mean(fin("mae"))), 3) if fin("mae") else None
+        summary[arm] = d
+    if "plain" in runs:
+        for arm in arms:
+            if arm == "plain":
+                continue
+            per = []
+            for rep in range(reps):
+                pg, sg = runs["plain"][rep], runs[arm][rep]
+                target, wp = pg[-1]["best_holdout"], sum(r["wall_ms"] for r in pg) / 1000.0
+                ws, hit = 0.0, None
+                for i, r in enumerate(sg):
+                    ws += r["wall_ms"] / 1000.0
+                    if r["best_holdout"] >= 0.99 * target:
+                        hit = {"gen": i + 1, "wall_s": round(ws, 3)}
+                        break
+                m = {"plain_champion": round(target, 4), "plain_wall_s": round(wp, 3), "reached_at": hit,
+                     "speedup": round(wp / hit["wall_s"], 2) if hit else None}
+                if hit and arm in net_arms and pre_s > 0:
+                    m["speedup_with_pretraining"] = round(wp / (hit["wall_s"] + pre_s), 2)
+                per.append(m)
+            hits = [m["speedup"] for m in per if m["speedup"] is not None]
+            summary[arm]["time_to_plain_champion"] = {
+                "per_seed": per, "reached": f"{len(hits)}/{len(per)}",
+                "speedup_median": float(np.median(hits)) if hits else None}
+            summary[arm]["gens_per_hour_vs_plain"] = round(summary[arm]["gens_per_hour"] /
+                                                           summary["plain"]["gens_per_hour"], 2)
+            summary[arm]["champion_ratio_at_plain_gens"] = round(
+                float(np.mean(summary[arm]["champion_at_plain_gens"])) /
+                float(np.mean(summary["plain"]["champion_at_plain_gens"])), 4)
+    # what the network adds over the same fly budget chosen without it
+    gain = {}
+    if "surrogate" in summary:
+        s_end = float(np.mean(summary["surrogate"]["champion_final"]))
+        for base in ("uniform", "parent"):
+            if base in summary:
+                b_end = float(np.mean(summary[base]["champion_final"]))
+                gain[f"vs_{base}"] = {"champion_final_ratio": round(s_end / b_end, 4),
+                                      "champion_final_diff_m": round(s_end - b_end, 3),
+                                      "per_seed_diff_m": [round(a - b, 3) for a, b in
+                                                          zip(summary["surrogate"]["champion_final"],
+                                                              summary[base]["champion_final"])]}
+    return {"config": {"pool": n_pool, "fly": n_fly, "elite": n_elite, "environments": 12, "ga_seeds": reps,
+                       "plain_gens": gens, "other_gens": max(gens, sur_gens), "explore": explore,
+                       "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": buffer},
+                       "scenarios": "config preset, gusts <= 4 m/s RMS, warp +-1 deg, 12 s cap"},
+            "pretraining": pre, "arms": summary, "network_gain": gain}
+
+
+def pool_ab(PA, args):
+    """C4 as its own bench line: the surrogate A/B (surrogate_ab) with the command-line sizes;
+    value = the surrogate arm's generations/hour."""
+    ab = surrogate_ab(PA, n_pool=args.pool, n_fly=args.fly, n_elite=args.elite, reps=args.reps, gens=args.gens,
+                      sur_gens=args.sur_gens, pretrain=args.pretrain, explore=args.explore, buffer=args.buffer,
+                      mlp_batch=args.mlp_batch, train_epochs=args.train_epochs, lr=args.lr,
+                      boot_epochs=args.boot_epochs)
+    sur = ab["arms"]["surrogate"]
+    line = {"metric": METRIC, "value": sur["gens_per_hour"], "unit": "GA generations/hour",
             "n_gpus": 1, "steps": args.gens * args.reps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 sim, bf16 surrogate", "data": "synthetic",
-            "config": {"workload": f"C4: pool {n_pool} Gaussian mutants (sigma 5% of range) of a {n_elite} elite, "
-                                   f"surrogate flies the top {n_fly} ({args.explore:.0%} exploration); 12 environments, "
-                                   f"gusts <= 4 m/s RMS, 12 s cap; {args.reps} GA seeds, {args.gens} generations "
-                                   f"plain, {out['surrogate']['gens'] // args.reps} surrogate",
-                       "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": args.buffer},
-                       "parallelism": "single GPU"},
-            "plain": out["plain"], "surrogate": out["surrogate"], "pretraining": pre,
-            "speedup_gens_per_hour": round(ratio, 2),
-            "champion_ratio": round(champ, 4),
-            "time_to_plain_champion": {"per_seed": match, "reached": f"{len(hits)}/{len(match)}",
-                                       "speedup_min": min(hits) if hits else None,
-                                       "speedup_median": float(np.median(hits)) if hits else None}}
+            "config": {"workload": f"C4: pool {args.pool} Gaussian mutants (sigma 5% of range) of a {args.elite} "
+                                   f"elite, surrogate flies the top {args.fly}; 12 environments, gusts <= 4 m/s RMS, "
+                                   f"12 s cap; {args.reps} GA seeds", "parallelism": "single GPU"},
+            "surrogate_ab": ab}
     print(json.dumps(line), flush=True)
 
 
@@ -408,6 +450,14 @@ def main():
     ap.add_argument("--pretrain", type=int, default=2_000_000,
                     help="c4: sim pairs the surrogate is pretrained on (setup; 0: bootstrap generation instead)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--surrogate-ab", action=argparse.BooleanOptionalAction, default=None,
+                    help="c3: after the timed generations, the surrogate A/B at 100k x 12 (plain vs surrogate "
+                         "gens/hour, time to the plain champion, and the same fly budget chosen without the "
+                         "network) in the same line (default on for c3)")
+    ap.add_argument("--ab-reps", type=int, default=3, help="surrogate A/B in the c3 line: GA seeds per arm")
+    ap.add_argument("--ab-gens", type=int, default=4, help="surrogate A/B in the c3 line: plain-arm generations")
+    ap.add_argument("--ab-sur-gens", type=int, default=None,
+                    help="surrogate A/B in the c3 line: generations of the other arms (default 3 x --ab-gens)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
                     help="c1-c3: flight-kernel mapping per bucket from the autotuner (setup, untimed; "
                          "default on for c3, whose configuration asks for it)")
@@ -591,6 +641,13 @@ def main():
         f64 = fp64_check(genomes, res, w, timed_samples, timed_seeds)
         log(f"FP64 check: {f64}")
 
+    ab = None
+    if args.surrogate_ab if args.surrogate_ab is not None else args.workload == "c3":
+        t0 = time.perf_counter()
+        ab = surrogate_ab(PA, reps=args.ab_reps, gens=args.ab_gens, sur_gens=args.ab_sur_gens)
+        ab["wall_s"] = round(time.perf_counter() - t0, 1)
+        log(f"surrogate A/B: {json.dumps(ab['arms'].get('surrogate', {}))[:400]}")
+
     out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -600,6 +657,8 @@ def main():
                       "launch": "hipGraph replay per evaluation" if args.graph else "direct launches"},
            "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
            "roofline": roof, "cpu_baseline": cpu, "fp64_check": f64}
+    if ab is not None:
+        out["surrogate_ab"] = ab
     print(json.dumps(out), flush=True)
 
 

@@ -283,6 +283,13 @@ int pr_pool_set_explore(pr_pool* p, float frac);
  * (the error the network makes on both cancels); 0, and before the first generation: the
  * predicted score itself.  No reference counterpart (the reference has no surrogate). */
 int pr_pool_set_parent_delta(pr_pool* p, int on);
+/* Generations run without a model (model = null): 0 (default) fly the whole pool; 1 fly the
+ * elite + a uniform random sample of the mutants; 2 fly the elite + the mutants of the
+ * best-scoring elite parents (the parent's simulated score as the key; a uniform sample
+ * before the first generation).  Modes 1 and 2 fly n_fly genomes chosen without the
+ * network: the baselines against which the surrogate's selection is measured.  No reference
+ * counterpart (the reference has no candidate pool). */
+int pr_pool_set_selection(pr_pool* p, int mode);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 /* The last generation's pool [n_pool x 35], fly set [n_fly x 35] in flight order, its

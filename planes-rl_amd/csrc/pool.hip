@@ -92,6 +92,22 @@ __global__ void k_sort_keys(const float* __restrict__ pred, int stride, int col,
     vals[i] = i;
 }
 
+// keys for a selection without the network (the surrogate's ablations): mode 1, a uniform
+// random subset of the mutants (hashed from (seed, gen, i)); mode 2, the mutant's parent's
+// simulated score (escore; ties -- mutants of one parent -- keep the pool order, which is
+// independent of the parent because k_mutate draws parents by hash).  The elite first.
+__global__ void k_sort_keys_nonet(int n, int n_keep, int mode, uint32_t seed, uint32_t gen,
+                                  const float* __restrict__ escore, float* __restrict__ keys, int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float k;
+    if (i < n_keep) k = -3.0e38f;
+    else if (mode == 2 && escore) k = -escore[(int)(hash3(seed, gen, (uint32_t)i) % (uint32_t)n_keep)];
+    else k = -unit24(hash3(seed ^ 0x6a09e667u, gen, (uint32_t)i));
+    keys[i] = k;
+    vals[i] = i;
+}
+
 __global__ void k_gather_col(const float* __restrict__ pred, int stride, int col, const int* __restrict__ idx, int n,
                              float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,6 +180,7 @@ struct pr_pool {
     size_t sort_tmp_bytes = 0;
     int buf_n = 0, buf_head = 0;
     int collect = 0;  // plain generations also feed the replay buffer
+    int select_mode = 0;  // without a model: 0 fly the whole pool, 1 uniform subset, 2 parent-score subset
     float explore = 0.0f;  // fraction of the surrogate fly set drawn at random from the pool
     float best_holdout = -1e9f;
     std::vector<float> acc, samples, champion;
@@ -310,6 +327,20 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         PCK(hipDeviceSynchronize());
         mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
         fly = p->d_fly;
+    } else if (p->select_mode != 0) {
+        // the fly budget chosen without the network (ablation of the pre-screen)
+        const float* escore = p->has_escore ? p->d_escore : nullptr;
+        hipLaunchKernelGGL(k_sort_keys_nonet, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->n_pool, p->n_elite,
+                           p->select_mode, p->seed, (uint32_t)gen, escore, p->d_keys, p->d_vals);
+        PCK(hipGetLastError());
+        size_t tb = p->sort_tmp_bytes;
+        PCK(hipcub::DeviceRadixSort::SortPairs(p->d_sort_tmp, tb, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
+                                               p->n_pool));
+        nf = p->n_fly;
+        hipLaunchKernelGGL(k_gather, dim3((nf * pr::NG + 255) / 256), dim3(256), 0, 0, p->d_pool, p->d_vals2, nf,
+                           p->d_fly);
+        PCK(hipGetLastError());
+        fly = p->d_fly;
     }
     PCK(hipDeviceSynchronize());
     std::vector<int> seeds(p->nseeds);
@@ -425,6 +456,12 @@ int pr_pool_set_explore(pr_pool* p, float frac) {
 int pr_pool_set_parent_delta(pr_pool* p, int on) {
     if (!p) { g_pool_err = "pr_pool_set_parent_delta: null pool"; return -1; }
     p->parent_delta = on ? 1 : 0;
+    return 0;
+}
+
+int pr_pool_set_selection(pr_pool* p, int mode) {
+    if (!p || mode < 0 || mode > 2) { g_pool_err = "pr_pool_set_selection: mode 0, 1 or 2"; return -1; }
+    p->select_mode = mode;
     return 0;
 }
 

@@ -136,6 +136,7 @@ def _load():
     lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
     lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
     lib.pr_pool_set_parent_delta.argtypes = [VP, C.c_int]
+    lib.pr_pool_set_selection.argtypes = [VP, C.c_int]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_pool_last.argtypes = [VP, VP, VP, VP, VP, C.POINTER(C.c_int)]
     lib.pr_mlp_param_count.restype = C.c_int
@@ -867,6 +868,13 @@ class PoolGA:
         """Mutants ranked by parent's simulated score + predicted change (default) or by the
         predicted score itself."""
         self._ck(LIB.pr_pool_set_parent_delta(self.h, int(on)), "pr_pool_set_parent_delta")
+
+    def set_selection(self, mode):
+        """Fly-set choice of generations without a surrogate: "all" (the whole pool),
+        "uniform" (elite + a uniform sample of the mutants, n_fly in total) or "parent"
+        (elite + the mutants of the best-scoring parents) -- the no-network baselines."""
+        m = {"all": 0, "uniform": 1, "parent": 2}[mode]
+        self._ck(LIB.pr_pool_set_selection(self.h, m), "pr_pool_set_selection")
 
     def set_collect(self, on=True):
         """Plain generations also feed the surrogate's replay buffer (bootstrap)."""

@@ -246,3 +246,48 @@ def test_surrogate_generations_reproducible(PA):
     (e0, p0, r0), (e1, p1, r1) = res
     print("surrogate generations (holdout, loss, rank corr):", r0)
     assert np.array_equal(e0, e1) and np.array_equal(p0, p1) and r0 == r1
+
+
+def test_selection_without_network_uniform_and_parent_score(PA):
+    """The surrogate's ablation baselines fly the same budget chosen without the network:
+    "uniform" -- the elite, then the mutants in the order of a stable sort of hashed uniform
+    keys; "parent" -- the elite, then the mutants ordered by their parent's simulated score
+    (descending, ties in pool order), the parent drawn as the mutation hash draws it and the
+    scores those the previous generation simulated; before any generation, "parent" falls back
+    to the uniform keys.  The flown scores are those of an independent evaluation."""
+    n_elite, n_pool, n_fly, k, seed = 32, 8192, 1024, 4, 11
+    params = PA.SimParams.config(**PARAMS)
+    g0, _, _ = PA.family_population(n_elite, seed=6)
+    i = np.arange(n_pool)
+    for mode in ("uniform", "parent"):
+        ga = PA.PoolGA(n_elite, n_pool, n_fly, k, buf_cap=4 * n_fly, sigma_frac=0.05, seed=seed)
+        ga.set_elite(g0)
+        ga.set_selection(mode)
+        gen = 2
+        r = ga.generation(gen, params)
+        assert r["n_fly"] == n_fly and r["flights"] == n_fly * k + 8 * 128
+        last = ga.last()
+        pool, fly, sc = last["pool"], last["fly"], last["scores"]
+        u = unit24(hash3(seed ^ 0x6A09E667, gen, i))
+        keys = np.where(i < n_elite, np.float32(-3.0e38), -u).astype(np.float32)
+        assert np.array_equal(fly, pool[np.argsort(keys, kind="stable")[:n_fly]]), mode
+        _, want = independent_scores(PA, fly, gen, k, params)
+        assert np.array_equal(sc, want)
+        E = ga.elite()
+        rows = {fly[j].tobytes(): j for j in range(len(fly))}
+        escore = sc[np.array([rows[e.tobytes()] for e in E])].astype(np.float32)
+        ga.generation(gen + 1, params)
+        last = ga.last()
+        pool, fly = last["pool"], last["fly"]
+        if mode == "uniform":
+            u = unit24(hash3(seed ^ 0x6A09E667, gen + 1, i))
+            keys = np.where(i < n_elite, np.float32(-3.0e38), -u).astype(np.float32)
+        else:
+            par = (hash3(seed, gen + 1, i) % n_elite).astype(np.int64)
+            keys = np.where(i < n_elite, np.float32(-3.0e38), -escore[par]).astype(np.float32)
+        assert np.array_equal(fly, pool[np.argsort(keys, kind="stable")[:n_fly]]), mode
+        if mode == "parent":  # the chosen mutants descend from the best-scoring parents
+            par = (hash3(seed, gen + 1, i) % n_elite).astype(np.int64)
+            chosen = np.argsort(keys, kind="stable")[n_elite:n_fly]
+            assert escore[par[chosen]].min() >= np.sort(escore)[len(escore) // 2]
+        del ga
