@@ -164,7 +164,7 @@ ARMS = {
 
 def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, gens=4, sur_gens=None, pretrain=2_000_000,
                  explore=0.3, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
-                 arms=("plain", "surrogate", "strict", "uniform", "parent")):
+                 arms=("plain", "surrogate", "strict", "uniform", "parent"), delta_mode=True, pretrained_state=None):
     """GA generations/hour with surrogate pre-screening against the plain batched sim, and
     against the same fly budget chosen without the network.  One generation: a pool of n_pool
     Gaussian mutants (sigma 5 % of each gene's range) of an n_elite elite; every arm flies its
@@ -189,11 +189,14 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
     tepochs = train_epochs if train_epochs is not None else (2 if pretrained else 4)
     pre, p0, pre_s = None, None, 0.0
     if pretrained and net_arms:
-        S = PA.Surrogate(max_batch=mlp_batch, seed=3)
-        pre = pretrain_surrogate(PA, S, pretrain, params)
-        p0 = S.params()
+        if pretrained_state is not None:  # weights pretrained by an earlier call (same settings)
+            pre, p0 = pretrained_state
+        else:
+            S = PA.Surrogate(max_batch=mlp_batch, seed=3)
+            pre = pretrain_surrogate(PA, S, pretrain, params)
+            p0 = S.params()
+            log(f"[surrogate] pretraining: {pre}")
         pre_s = pre["sim_s"] + pre["train_s"]
-        log(f"[surrogate] pretraining: {pre}")
     runs = {a: [] for a in arms}
     for rep in range(reps):
         for arm in arms:
@@ -206,7 +209,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
             elif arm in ("surrogate", "strict"):
                 S = PA.Surrogate(max_batch=mlp_batch, seed=3)
                 ga.set_explore(explore if arm == "surrogate" else 0.0)
-                ga.set_parent_delta(arm == "surrogate")
+                ga.set_parent_delta(delta_mode if arm == "surrogate" else False)
                 if pretrained:
                     S.set_params(p0)
                 else:
@@ -280,11 +283,16 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
                                       "per_seed_diff_m": [round(a - b, 3) for a, b in
                                                           zip(summary["surrogate"]["champion_final"],
                                                               summary[base]["champion_final"])]}
+    for arm in net_arms:
+        w = [(r["w_parent"], r["w_pred"], r["w_parent_pred"]) for g in runs[arm] for r in g
+             if r.get("w_parent") == r.get("w_parent")]
+        if w:  # least-squares weights of the mutants' scores on (parent score, prediction, parent's prediction)
+            summary[arm]["mean_key_fit"] = [round(float(v), 3) for v in np.mean(np.array(w), 0)]
     return {"config": {"pool": n_pool, "fly": n_fly, "elite": n_elite, "environments": 12, "ga_seeds": reps,
                        "plain_gens": gens, "other_gens": max(gens, sur_gens), "explore": explore,
                        "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": buffer},
                        "scenarios": "config preset, gusts <= 4 m/s RMS, warp +-1 deg, 12 s cap"},
-            "pretraining": pre, "arms": summary, "network_gain": gain}
+            "pretraining": pre, "arms": summary, "network_gain": gain, "_pretrained_state": (pre, p0)}
 
 
 def pool_ab(PA, args):
@@ -294,6 +302,7 @@ def pool_ab(PA, args):
                       sur_gens=args.sur_gens, pretrain=args.pretrain, explore=args.explore, buffer=args.buffer,
                       mlp_batch=args.mlp_batch, train_epochs=args.train_epochs, lr=args.lr,
                       boot_epochs=args.boot_epochs)
+    ab.pop("_pretrained_state")
     sur = ab["arms"]["surrogate"]
     line = {"metric": METRIC, "value": sur["gens_per_hour"], "unit": "GA generations/hour",
             "n_gpus": 1, "steps": args.gens * args.reps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
@@ -645,6 +654,7 @@ def main():
     if args.surrogate_ab if args.surrogate_ab is not None else args.workload == "c3":
         t0 = time.perf_counter()
         ab = surrogate_ab(PA, reps=args.ab_reps, gens=args.ab_gens, sur_gens=args.ab_sur_gens)
+        ab.pop("_pretrained_state")
         ab["wall_s"] = round(time.perf_counter() - t0, 1)
         log(f"surrogate A/B: {json.dumps(ab['arms'].get('surrogate', {}))[:400]}")
 

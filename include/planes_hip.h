@@ -258,11 +258,14 @@ int pr_mlp_train(pr_mlp* m, const float* d_genomes, const float* d_targets, int 
  * best surrogate-predicted score (n_fly in total); new elite = best simulated scores;
  * top-8 holdout champion; with a model, the flown pairs join a replay buffer of buf_cap
  * and the network trains on it.  The simulation alone ranks the elite and the champion.
- * pr_pool_generation out[14]: best score, elite mean, holdout champion score, wall ms,
+ * pr_pool_generation out[17]: best score, elite mean, holdout champion score, wall ms,
  * flight ms, surrogate ms, flights, plane-flight-s, training loss, fly-set size, best
  * holdout score so far; on the fly set's exploration sample (uniform draws from the pool):
  * Spearman rho of predicted vs simulated scores, mean absolute error (m) over the members
- * both call flyable, accuracy of the predicted oversize flag. */
+ * both call flyable, accuracy of the predicted oversize flag, and the least-squares weights
+ * of the mutants' simulated scores on (parent's simulated score, mutant's predicted score,
+ * parent's predicted score) (NaN until a generation with the elite's scores; the key of
+ * pr_pool_set_parent_delta 2). */
 typedef struct pr_pool pr_pool;
 int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds, int buf_cap, float sigma_frac,
                    unsigned seed);
@@ -280,7 +283,10 @@ int pr_pool_set_collect(pr_pool* p, int on);
 int pr_pool_set_explore(pr_pool* p, float frac);
 /* 1 (default): once a generation has simulated the elite, a mutant's selection key is its
  * parent's simulated score plus the network's predicted change from parent to mutant
- * (the error the network makes on both cancels); 0, and before the first generation: the
+ * (the error the network makes on both cancels); 2: a fitted combination a * parent's
+ * simulated score + b * mutant's predicted score + c * parent's predicted score, (a, b, c)
+ * from the least squares of the previous generation's exploration sample's simulated scores
+ * on those three ((1, 1, -1) until a fit exists); 0, and before the first generation: the
  * predicted score itself.  No reference counterpart (the reference has no surrogate). */
 int pr_pool_set_parent_delta(pr_pool* p, int on);
 /* Generations run without a model (model = null): 0 (default) fly the whole pool; 1 fly the

@@ -35,6 +35,11 @@ __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
 }
 __device__ __forceinline__ float unit24(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
 
+// hash3 on the host (the parent draw of k_mutate, for the key calibration)
+static inline uint32_t host_hash3(uint32_t a, uint32_t b, uint32_t c) {
+    return pr::mix32(a * 0x9e3779b9u ^ pr::mix32(b * 0x85ebca6bu ^ pr::mix32(c + 0x27d4eb2fu)));
+}
+
 // pool[i] = elite[i] for i < n_elite, else a mutant of a random elite parent
 __global__ void k_mutate(const float* __restrict__ elite, int n_elite, float* __restrict__ pool, int n_pool,
                          const float* __restrict__ lo, const float* __restrict__ hi, float sigma_frac, uint32_t seed,
@@ -77,16 +82,20 @@ __global__ void k_gather(const float* __restrict__ src, const int* __restrict__ 
 // escore (non-null): the elite's simulated scores; a mutant's key is then its parent's
 // simulated score plus the predicted change from parent to mutant (the network's error
 // common to both cancels), with the parent drawn as k_mutate draws it
+// cw: the key's weights (a, b, c) of the parent's simulated score, the mutant's predicted
+// score and the parent's predicted score -- (1, 1, -1): the parent's score plus the predicted
+// change; fitted: least squares of the mutants' simulated scores on the three over the last
+// exploration sample
 __global__ void k_sort_keys(const float* __restrict__ pred, int stride, int col, int n, int n_keep,
                             float p_explore, uint32_t seed, uint32_t gen, const float* __restrict__ escore,
-                            float* __restrict__ keys, int* __restrict__ vals) {
+                            float ca, float cb, float cc, float* __restrict__ keys, int* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool explore = i >= n_keep && unit24(hash3(seed ^ 0x2545f491u, gen, (uint32_t)i)) < p_explore;
     float s = pred[(size_t)i * stride + col];
     if (escore && i >= n_keep) {
         const int par = (int)(hash3(seed, gen, (uint32_t)i) % (uint32_t)n_keep);
-        s = escore[par] + (s - pred[(size_t)par * stride + col]);
+        s = ca * escore[par] + (cb * s + cc * pred[(size_t)par * stride + col]);
     }
     keys[i] = i < n_keep ? -3.0e38f : explore ? -2.0e38f : -s;
     vals[i] = i;
@@ -164,6 +173,8 @@ __global__ void k_targets(const float* __restrict__ samples, int n, int k, float
 
 }  // namespace
 
+static const float kDeltaWeights[3] = {1.0f, 1.0f, -1.0f};
+
 struct pr_pool {
     int n_elite = 0, n_pool = 0, n_fly = 0, nseeds = 0, buf_cap = 0;
     float sigma = 0.05f;
@@ -174,7 +185,10 @@ struct pr_pool {
     float *d_buf_g = nullptr, *d_buf_t = nullptr;
     float* d_escore = nullptr;  // [n_elite] simulated scores of the current elite
     int has_escore = 0;         // set by a generation, cleared by pr_pool_set_elite
-    int parent_delta = 1;       // mutants' keys from the parent's simulated score (when known)
+    int parent_delta = 1;       // mutants' keys from the parent's simulated score (when known); 2: calibrated
+    float cw[3] = {1.0f, 1.0f, -1.0f};  // key weights of (parent score, prediction, parent's prediction)
+    int cw_fitted = 0;          // cw comes from a fit (mode 2) rather than the default
+    std::vector<float> escore_h;  // host copy of the elite's scores the last selection used
     int *d_vals = nullptr, *d_vals2 = nullptr, *d_idx = nullptr;
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
@@ -261,6 +275,8 @@ int pr_pool_set_elite(pr_pool* p, const float* genomes, int n) {
     if (!p || n < p->n_elite) { g_pool_err = "pr_pool_set_elite: need n >= n_elite"; return -1; }
     PCK(hipMemcpy(p->d_elite, genomes, sizeof(float) * pr::NG * p->n_elite, hipMemcpyHostToDevice));
     p->has_escore = 0;
+    p->escore_h.clear();
+    p->cw_fitted = 0;
     return 0;
 }
 
@@ -314,8 +330,10 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         const float p_explore =
             p->n_pool > p->n_elite ? p->explore * (float)(p->n_fly - p->n_elite) / (float)(p->n_pool - p->n_elite) : 0.f;
         const float* escore = p->has_escore && p->parent_delta ? p->d_escore : nullptr;
+        const float* w = p->parent_delta == 2 && p->cw_fitted ? p->cw : kDeltaWeights;
         hipLaunchKernelGGL(k_sort_keys, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->n_pool,
-                           p->n_elite, p_explore, p->seed, (uint32_t)gen, escore, p->d_keys, p->d_vals);
+                           p->n_elite, p_explore, p->seed, (uint32_t)gen, escore, w[0], w[1], w[2], p->d_keys,
+                           p->d_vals);
         PCK(hipGetLastError());
         size_t tb = p->sort_tmp_bytes;
         PCK(hipcub::DeviceRadixSort::SortPairs(p->d_sort_tmp, tb, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
@@ -351,6 +369,8 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     float holdout = 0;
     int champ = 0;
     const bool collect = model != nullptr || p->collect;
+    const bool had_escore = p->has_escore != 0;  // this generation's keys used the elite's scores
+    std::vector<float> escore_prev;
     PRC(pr_gen_evaluate(p->gen_eval, fly, nullptr, nf, seeds.data(), p->nseeds, 8, prm, p->acc.data(), p->idx.data(),
                         collect ? p->samples.data() : nullptr, &holdout, &champ));
     pr_stats st_main, st_hold;
@@ -364,6 +384,8 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         std::vector<float> es(p->n_elite);
         for (int k = 0; k < p->n_elite; k++) es[k] = p->acc[p->idx[k]];
         PCK(hipMemcpy(p->d_escore, es.data(), sizeof(float) * p->n_elite, hipMemcpyHostToDevice));
+        escore_prev.swap(p->escore_h);  // the scores this generation's selection used
+        p->escore_h = es;
         p->has_escore = 1;
     }
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
@@ -373,6 +395,8 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     // exploration sample of the fly set (uniform draws from the pool, so unbiased by the
     // selection; the surrogate-selected part over-represents over-predicted candidates)
     double rho = std::nan(""), mae = std::nan(""), acc_over = std::nan("");
+    int fit_ok = 0;
+    float fit_w[3] = {0.0f, 0.0f, 0.0f};
     if (model) {
         hipLaunchKernelGGL(k_gather_col, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->d_vals2, nf,
                            p->d_keys);
@@ -384,6 +408,54 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         for (int i = 0; i < nf; i++)
             if (key[i] == -2.0e38f) { a.push_back(pf[i]); b.push_back(p->acc[i]); }
         rho = spearman(a, b);
+        // fit of the selection key (mode 2): over the exploration sample, least squares of the
+        // mutants' simulated scores on (the parent's score the key used, the mutant's predicted
+        // score, the parent's predicted score) with an intercept; flyable members only
+        fit_ok = 0;
+        if (had_escore && (int)escore_prev.size() == p->n_elite) {
+            std::vector<int> pidx(nf);
+            PCK(hipMemcpy(pidx.data(), p->d_vals2, sizeof(int) * nf, hipMemcpyDeviceToHost));
+            std::vector<float> pp(p->n_elite);
+            hipLaunchKernelGGL(k_gather_col, dim3((p->n_elite + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4,
+                               p->d_vals2, p->n_elite, p->d_keys);
+            PCK(hipGetLastError());
+            PCK(hipMemcpy(pp.data(), p->d_keys, sizeof(float) * p->n_elite, hipMemcpyDeviceToHost));
+            double A[4][5] = {};
+            long long m = 0;
+            for (int i = p->n_elite; i < nf; i++) {
+                if (key[i] != -2.0e38f) continue;
+                const int par = (int)(host_hash3(p->seed, (uint32_t)gen, (uint32_t)pidx[i]) % (uint32_t)p->n_elite);
+                const float xe = escore_prev[par], xc = pf[i], xp = pp[par], y = p->acc[i];
+                if (xe == -100.0f || xc == -100.0f || xp == -100.0f || y == -100.0f) continue;
+                const double v[5] = {1.0, xe, xc, xp, y};
+                for (int r = 0; r < 4; r++)
+                    for (int c = 0; c < 5; c++) A[r][c] += v[r] * v[c];
+                m++;
+            }
+            // solve the 4x4 normal equations (Gaussian elimination, partial pivoting)
+            if (m >= 256) {
+                bool ok = true;
+                for (int c = 0; c < 4 && ok; c++) {
+                    int piv = c;
+                    for (int r = c + 1; r < 4; r++) if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+                    if (std::fabs(A[piv][c]) < 1e-12 * (1.0 + std::fabs(A[0][0]))) { ok = false; break; }
+                    for (int k = 0; k < 5; k++) std::swap(A[c][k], A[piv][k]);
+                    for (int r = 0; r < 4; r++) {
+                        if (r == c) continue;
+                        const double f = A[r][c] / A[c][c];
+                        for (int k = c; k < 5; k++) A[r][k] -= f * A[c][k];
+                    }
+                }
+                if (ok) {
+                    for (int k = 0; k < 3; k++) fit_w[k] = (float)(A[k + 1][4] / A[k + 1][k + 1]);
+                    fit_ok = 1;
+                    if (p->parent_delta == 2) {
+                        for (int k = 0; k < 3; k++) p->cw[k] = fit_w[k];
+                        p->cw_fitted = 1;
+                    }
+                }
+            }
+        }
         // the predicted score is -100 where the oversize head says so: regression error over
         // the members both call flyable, and the oversize flag's accuracy over all of them
         if (!a.empty()) {
@@ -430,6 +502,7 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     out[11] = rho;
     out[12] = mae;
     out[13] = acc_over;
+    for (int k = 0; k < 3; k++) out[14 + k] = fit_ok ? (double)fit_w[k] : std::nan("");
     return 0;
 }
 
@@ -454,8 +527,8 @@ int pr_pool_set_explore(pr_pool* p, float frac) {
 // 1: plain generations (model = null) also add their flown pairs to the replay buffer,
 // e.g. to bootstrap the surrogate before pre-screening starts.
 int pr_pool_set_parent_delta(pr_pool* p, int on) {
-    if (!p) { g_pool_err = "pr_pool_set_parent_delta: null pool"; return -1; }
-    p->parent_delta = on ? 1 : 0;
+    if (!p || on < 0 || on > 2) { g_pool_err = "pr_pool_set_parent_delta: null pool or mode outside 0..2"; return -1; }
+    p->parent_delta = on;
     return 0;
 }
 

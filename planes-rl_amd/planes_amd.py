@@ -831,7 +831,7 @@ class PoolGA:
     pre-screened (fly the elite + the best-predicted candidates)."""
 
     FIELDS = ("best", "elite_mean", "holdout", "wall_ms", "fly_ms", "mlp_ms", "flights", "pfs", "loss", "n_fly",
-              "best_holdout", "rank_corr", "mae", "oversize_acc")
+              "best_holdout", "rank_corr", "mae", "oversize_acc", "w_parent", "w_pred", "w_parent_pred")
 
     def __init__(self, n_elite, n_pool, n_fly, nseeds, buf_cap=2_000_000, sigma_frac=0.05, seed=1):
         self.h = VP()
@@ -864,10 +864,13 @@ class PoolGA:
         """Fraction of the surrogate fly set drawn at random from the pool."""
         self._ck(LIB.pr_pool_set_explore(self.h, float(frac)), "pr_pool_set_explore")
 
-    def set_parent_delta(self, on=True):
-        """Mutants ranked by parent's simulated score + predicted change (default) or by the
-        predicted score itself."""
-        self._ck(LIB.pr_pool_set_parent_delta(self.h, int(on)), "pr_pool_set_parent_delta")
+    def set_parent_delta(self, mode=True):
+        """Mutants ranked by parent's simulated score + predicted change (True / 1, default), by
+        a fitted combination ("fitted" / 2: weights of the parent's score, the mutant's and the
+        parent's predicted score from the last exploration sample's least squares), or by the
+        predicted score itself (False / 0)."""
+        m = 2 if mode == "fitted" else int(bool(mode))
+        self._ck(LIB.pr_pool_set_parent_delta(self.h, m), "pr_pool_set_parent_delta")
 
     def set_selection(self, mode):
         """Fly-set choice of generations without a surrogate: "all" (the whole pool),
