@@ -1,16 +1,23 @@
 // mlp.hip -- surrogate fitness model: MLP 35 -> 256 -> 256 -> 256 -> 4 (ReLU hidden layers)
 // trained on the (genome, fitness) pairs the flight simulation produces.
 //
-// Arithmetic: BF16 operands on the matrix cores (v_mfma_f32_16x16x32_bf16, gfx950), FP32
+// Arithmetic: BF16 operands on the matrix cores (v_mfma_f32_32x32x16_bf16, gfx950), FP32
 // accumulation, FP32 master weights and Adam state; activations stored as BF16 between
 // layers.  Heads: 0 mean distance, 1 CVaR of the worst quarter, 2 worst distance (all in
 // units of 10 m, regression, masked for unprintable designs), 3 oversize logit (BCE).
 //
-// Every GEMM is one kernel, gemm_bf16<TA, TB, EPI>: a wave computes one 16x16 output tile,
-// stepping K by 32 with one MFMA per step; the epilogue either applies bias + ReLU and
-// stores BF16 (forward), multiplies by the ReLU mask of the layer input and stores BF16
-// (input gradients), stores FP32, or stores FP32 partial products per batch slice (weight
-// gradients, then summed in slice order: training is deterministic).
+// Two GEMM kernels, both with K-contiguous operands loaded 16 bytes per lane:
+//   gemm_nt     C = A * Bt^T for the forward pass (A = activations, Bt = the layer's
+//               transposed weight copy; epilogue bias + ReLU -> BF16, or bias -> FP32) and
+//               the input gradients (A = output gradient, Bt = the weights; epilogue the ReLU
+//               mask of the layer input -> BF16); training also writes each BF16 result
+//               transposed, the layout the weight gradients read;
+//   gemm_wgrad  dW = Xin^T dY over the batch from those transposes (features x batch), eight
+//               waves per 32 x 32 tile on batch slices summed in wave order through LDS, the
+//               bias gradient (column sums of dY) from an all-ones MFMA operand alongside:
+//               training is deterministic.
+// A training step is 14 launches (embedding, 4 forward, loss, 4 weight + bias gradients,
+// 3 input gradients, Adam); prediction runs 65536 rows per pass on buffers of its own.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,66 +42,219 @@ const int LN[NL] = {HID, HID, HID, NOP};
 __device__ __forceinline__ __bf16 to_bf(float x) { return (__bf16)x; }  // round to nearest even
 __device__ __forceinline__ float from_bf(__bf16 x) { return (float)x; }
 
-enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2, EPI_PART_F32 = 3 };
+enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2 };
 
-// C[M x N] = A(m,k) * B(k,n) over k in [0, K) (split over blockIdx.z for EPI_PART_F32, which
-// writes split z's partial product to C + z * ldc * gridDim.x * 16 for an ordered sum after).
-// A(m,k) = TA ? A[k*lda + m] : A[m*lda + k];  B(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n].
-// M, N multiples of 16, K a multiple of 32 (per split).
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(64) void gemm_bf16(const __bf16* __restrict__ A, int lda, const __bf16* __restrict__ B,
-                                                int ldb, int K, int k_per_split, const float* __restrict__ bias,
-                                                const __bf16* __restrict__ mask, int ldm, void* __restrict__ C,
-                                                int ldc) {
-    const int l = threadIdx.x;
-    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
-    const int kb = blockIdx.z * k_per_split, ke = min(K, kb + k_per_split);
-    const int mr = m0 + (l & 15), nc = n0 + (l & 15), kq = 8 * (l >> 4);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 32) {
-        bf8 a, b;
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+
+// C[M x N] = A[M x K] * B[K x N] with both operands K-contiguous: A row-major (lda), and B
+// given as its transpose Bt[N x K] (row-major, ldb): B(k, n) = Bt[n * ldb + k].  The forward
+// pass (B = the layer's transposed weight copy) and the input gradient (dY * W^T: Bt = W
+// itself) have this form.  A workgroup of WGM x WGN waves computes a (32 WM WGM) x
+// (32 WN WGN) tile; each wave a (32 WM) x (32 WN) tile of v_mfma_f32_32x32x16_bf16 products,
+// its operands one 16-byte load per lane and MFMA tile (lane l: row l % 32, k = k0 + 8 (l / 32)
+// .. + 7).  M, N multiples of the workgroup tile, K a multiple of 16.  Accumulator element r
+// of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 tile.
+// BF16 epilogues optionally also write the transpose, Ct[n * ldct + m] (the operand layout
+// the weight gradients read), four rows at a time as one 8-byte store.
+template <int EPI, int WGM, int WGN, int WM, int WN>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_nt(const __bf16* __restrict__ A, int lda,
+                                                          const __bf16* __restrict__ Bt, int ldb, int K,
+                                                          const float* __restrict__ bias,
+                                                          const __bf16* __restrict__ mask, int ldm, void* __restrict__ C,
+                                                          int ldc, __bf16* __restrict__ Ct, int ldct) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * (32 * WM * WGM) + (w / WGN) * (32 * WM);
+    const int n0 = blockIdx.y * (32 * WN * WGN) + (w % WGN) * (32 * WN);
+    const int lr = l & 31, kq = 8 * (l >> 5);
+    f16v acc[WM][WN];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int k = k0 + kq + j;
-            a[j] = TA ? A[(size_t)k * lda + mr] : A[(size_t)mr * lda + k];
-            b[j] = TB ? B[(size_t)nc * ldb + k] : B[(size_t)k * ldb + nc];
+    for (int i = 0; i < WM; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+    const __bf16* pa = A + (size_t)(m0 + lr) * lda + kq;
+    const __bf16* pb = Bt + (size_t)(n0 + lr) * ldb + kq;
+    // operands of the next D k-steps in flight (a ring of registers, refilled D steps ahead):
+    // the loop issues loads D steps before their MFMAs instead of waiting on each
+    constexpr int D = (WM + WN) <= 2 ? 8 : 4;
+    const int ns = K / 16;
+    bf8 ra[D][WM], rb[D][WN];
+#pragma unroll
+    for (int d = 0; d < D; d++)
+        if (d < ns) {
+#pragma unroll
+            for (int i = 0; i < WM; i++) ra[d][i] = *reinterpret_cast<const bf8*>(pa + (size_t)32 * i * lda + 16 * d);
+#pragma unroll
+            for (int j = 0; j < WN; j++) rb[d][j] = *reinterpret_cast<const bf8*>(pb + (size_t)32 * j * ldb + 16 * d);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-    }
-    // D layout: lane l holds rows m0 + 4*(l/16) + j, column n0 + l%16
+    for (int s0 = 0; s0 < ns; s0 += D) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int m = m0 + 4 * (l >> 4) + j, n = nc;
-        float v = acc[j];
-        if (EPI == EPI_BIAS_RELU_BF16) {
-            v = fmaxf(v + bias[n], 0.0f);
-            reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = to_bf(v);
-        } else if (EPI == EPI_BIAS_F32) {
-            reinterpret_cast<float*>(C)[(size_t)m * ldc + n] = v + bias[n];
-        } else if (EPI == EPI_MASK_BF16) {
-            const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
-            reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = to_bf(v * keep);
-        } else {
-            const size_t plane = (size_t)gridDim.x * 16 * ldc;
-            reinterpret_cast<float*>(C)[blockIdx.z * plane + (size_t)m * ldc + n] = v;
+        for (int d = 0; d < D; d++) {
+            const int st = s0 + d;
+            if (st < ns) {
+                bf8 a[WM], b[WN];
+#pragma unroll
+                for (int i = 0; i < WM; i++) a[i] = ra[d][i];
+#pragma unroll
+                for (int j = 0; j < WN; j++) b[j] = rb[d][j];
+                if (st + D < ns) {
+                    const int k1 = 16 * (st + D);
+#pragma unroll
+                    for (int i = 0; i < WM; i++) ra[d][i] = *reinterpret_cast<const bf8*>(pa + (size_t)32 * i * lda + k1);
+#pragma unroll
+                    for (int j = 0; j < WN; j++) rb[d][j] = *reinterpret_cast<const bf8*>(pb + (size_t)32 * j * ldb + k1);
+                }
+#pragma unroll
+                for (int i = 0; i < WM; i++)
+#pragma unroll
+                    for (int j = 0; j < WN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < WM; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int mg = m0 + 32 * i + 8 * g + 4 * (l >> 5), n = n0 + 32 * j + lr;
+                __bf16 o[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int m = mg + q;
+                    float v = acc[i][j][4 * g + q];
+                    if (EPI == EPI_BIAS_RELU_BF16) {
+                        o[q] = to_bf(fmaxf(v + bias[n], 0.0f));
+                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
+                    } else if (EPI == EPI_BIAS_F32) {
+                        reinterpret_cast<float*>(C)[(size_t)m * ldc + n] = v + bias[n];
+                    } else if (EPI == EPI_MASK_BF16) {
+                        const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
+                        o[q] = to_bf(v * keep);
+                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
+                    }
+                }
+                if ((EPI == EPI_BIAS_RELU_BF16 || EPI == EPI_MASK_BF16) && Ct) {
+                    bf4 t = {o[0], o[1], o[2], o[3]};
+                    *reinterpret_cast<bf4*>(Ct + (size_t)n * ldct + mg) = t;
+                }
+            }
+}
+
+// Weight (and bias) gradients: dW[M x N] (FP32) = A[M x Kd] * Bt[N x Kd]^T over the batch
+// Kd, with A = the layer input transposed (features x batch) and Bt = the output gradient
+// transposed -- both batch-contiguous, so the operands are 16-byte loads as in gemm_nt.  The
+// batch is split gridDim.z ways (so that the few 32 x 32 tiles of a layer fill the GPU); a
+// workgroup of 8 waves computes one tile of one split: wave w sums its slice, the eight
+// partial tiles are added in wave order through LDS, and split z writes C + z split_stride
+// (the Adam step adds the splits in order: deterministic, no atomics).  With BIAS, the
+// workgroups of row tile 0 also form db[n] = sum over the batch of dY[b][n]: an MFMA with an
+// all-ones A operand on the same Bt operands gives the column sums of the slice, added in
+// wave order likewise.  Kd a multiple of 128 gridDim.z.
+// k-loop of one wave of gemm_wgrad over its batch slice (ns k-steps), operands D steps ahead
+template <bool DOB>
+__device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, int ns, f16v& acc, f16v& accb) {
+    constexpr int D = 8;
+    bf8 ones;
+#pragma unroll
+    for (int j = 0; j < 8; j++) ones[j] = to_bf(1.0f);
+    bf8 ra[D], rb[D];
+#pragma unroll
+    for (int d = 0; d < D; d++)
+        if (d < ns) {
+            ra[d] = *reinterpret_cast<const bf8*>(pa + 16 * d);
+            rb[d] = *reinterpret_cast<const bf8*>(pb + 16 * d);
+        }
+    for (int s0 = 0; s0 < ns; s0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const int st = s0 + d;
+            if (st < ns) {
+                const bf8 a = ra[d], b = rb[d];
+                if (st + D < ns) {
+                    ra[d] = *reinterpret_cast<const bf8*>(pa + 16 * (st + D));
+                    rb[d] = *reinterpret_cast<const bf8*>(pb + 16 * (st + D));
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                if (DOB) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ones, b, accb, 0, 0, 0);
+            }
         }
     }
 }
 
+// Weight (and bias) gradients: dW[M x N] (FP32) = A[M x Kd] * Bt[N x Kd]^T over the batch
+// Kd, with A = the layer input transposed (features x batch) and Bt = the output gradient
+// transposed -- both batch-contiguous, so the operands are 16-byte loads as in gemm_nt.  The
+// batch is split gridDim.z ways (so that the few 32 x 32 tiles of a layer fill the GPU); a
+// workgroup of 8 waves computes one tile of one split: wave w sums its slice, the eight
+// partial tiles are added in wave order through LDS, and split z writes C + z split_stride
+// (the Adam step adds the splits in order: deterministic, no atomics).  With BIAS, the
+// workgroups of row tile 0 also form db[n] = sum over the batch of dY[b][n]: an MFMA with an
+// all-ones A operand on the same Bt operands gives the column sums of the slice, added in
+// wave order likewise.  Kd a multiple of 128 gridDim.z.
+template <bool BIAS>
+__global__ __launch_bounds__(512) void gemm_wgrad(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, int ld,
+                                                  int Kd, float* __restrict__ C, int ldc, float* __restrict__ db,
+                                                  size_t split_stride) {
+    __shared__ float part[8][16][64];
+    __shared__ float bpart[8][32];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    // batch split blockIdx.z of gridDim.z, wave w its eighth; each split writes its own copy
+    const int ks = Kd / (8 * gridDim.z), kb = (blockIdx.z * 8 + w) * ks;
+    C += blockIdx.z * split_stride;
+    db += blockIdx.z * split_stride;
+    const int lr = l & 31, kq = 8 * (l >> 5);
+    const __bf16* pa = A + (size_t)(m0 + lr) * ld + kb + kq;
+    const __bf16* pb = Bt + (size_t)(n0 + lr) * ld + kb + kq;
+    const bool dob = BIAS && blockIdx.x == 0;
+    f16v acc, accb;
+#pragma unroll
+    for (int r = 0; r < 16; r++) { acc[r] = 0.0f; accb[r] = 0.0f; }
+    if (dob) wgrad_slice<true>(pa, pb, ks / 16, acc, accb);
+    else wgrad_slice<false>(pa, pb, ks / 16, acc, accb);
+#pragma unroll
+    for (int r = 0; r < 16; r++) part[w][r][l] = acc[r];
+    if (dob && l < 32) bpart[w][l] = accb[0];  // row 0 of ones * B: the column sums
+    __syncthreads();
+#pragma unroll
+    for (int rr = 2 * w; rr < 2 * w + 2; rr++) {
+        float v = part[0][rr][l];
+#pragma unroll
+        for (int u = 1; u < 8; u++) v += part[u][rr][l];
+        const int m = m0 + 8 * (rr >> 2) + 4 * (l >> 5) + (rr & 3), n = n0 + lr;
+        C[(size_t)m * ldc + n] = v;
+    }
+    if (dob && w == 0 && l < 32) {
+        float v = bpart[0][l];
+#pragma unroll
+        for (int u = 1; u < 8; u++) v += bpart[u][l];
+        db[n0 + l] = v;
+    }
+}
+
+// rows of a batch are padded to this multiple (the gemm_nt workgroup tiles' row count)
+constexpr int ROW_PAD = 128;
+
 // genomes [n x 35] f32 -> X [n_pad x 64] bf16, genes scaled to [0, 1] by the bounds
+// (Xt, optional: the transpose [64 x n_pad], the weight gradient's operand layout)
 __global__ void k_embed(const float* __restrict__ g, int n, int n_pad, const float* __restrict__ lo,
-                        const float* __restrict__ inv_range, __bf16* __restrict__ X) {
+                        const float* __restrict__ inv_range, __bf16* __restrict__ X, __bf16* __restrict__ Xt) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pad * K0) return;
     int r = i / K0, k = i - r * K0;
     float v = 0.0f;
     if (r < n && k < NIN) v = (g[(size_t)r * NIN + k] - lo[k]) * inv_range[k];
     X[i] = to_bf(v);
+    if (Xt) Xt[(size_t)k * n_pad + r] = to_bf(v);
 }
 
 // loss and output gradient.  out [n_pad x 16] f32, targets [n x 4] raw (m, m, m, 0/1)
 __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, int n, int n_pad, float inv_n,
-                       __bf16* __restrict__ dout, float* __restrict__ dout_f, float* __restrict__ loss_part) {
+                       __bf16* __restrict__ dout, __bf16* __restrict__ doutT, float* __restrict__ loss_part) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;  // every lane stays for the wave reduction
     float l = 0.0f;
     float d[NOUT] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -118,7 +278,7 @@ __global__ void k_loss(const float* __restrict__ out, const float* __restrict__ 
         for (int k = 0; k < NOP; k++) {
             const float dk = k < NOUT ? d[k] : 0.0f;
             dout[(size_t)r * NOP + k] = to_bf(dk);
-            if (dout_f) dout_f[(size_t)r * NOP + k] = dk;
+            doutT[(size_t)k * n_pad + r] = to_bf(dk);
         }
     }
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
@@ -133,59 +293,46 @@ __global__ void k_loss_fin(const float* __restrict__ loss_part, int nblk, float 
     if (threadIdx.x == 0) *loss = l * inv_n;
 }
 
-// bias gradient = column sums of a bf16 [rows x cols] gradient, in two deterministic passes:
-// block (x, y) sums columns 64x..64x+63 over row slice y (4 waves on interleaved rows, then
-// an LDS sum in wave order) into part[y][cols]; k_colsum_fin adds the COLSUM_SLICES partials
-// of each column in slice order.
-constexpr int COLSUM_SLICES = 32;
-constexpr int WG_SPLIT = 512;  // batch rows per weight-gradient partial product
-__global__ __launch_bounds__(256) void k_colsum_part(const __bf16* __restrict__ G, int rows, int cols,
-                                                     float* __restrict__ part) {
-    __shared__ float red[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int w = threadIdx.x >> 6;
-    const int per = (rows + COLSUM_SLICES - 1) / COLSUM_SLICES;
-    const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
-    float s = 0.0f;
-    if (c < cols)
-        for (int r = r0 + w; r < r1; r += 4) s += from_bf(G[(size_t)r * cols + c]);
-    red[w][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (w == 0 && c < cols) {
-        const int l = threadIdx.x;
-        part[(size_t)blockIdx.y * cols + c] = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
-    }
-}
-
-__global__ void k_colsum_fin(const float* __restrict__ part, int cols, float* __restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    float s = 0.0f;
-    for (int y = 0; y < COLSUM_SLICES; y++) s += part[(size_t)y * cols + c];
-    out[c] = s;
-}
-
-// out[i] = sum over the splits of part[z * count + i], in split order (deterministic)
+// out[i] = the nsplit copies of part (count apart) added in split order
 __global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int count, float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    float s = 0.0f;
-    for (int z = 0; z < nsplit; z++) s += part[(size_t)z * count + i];
+    float s = part[i];
+    for (int z = 1; z < nsplit; z++) s += part[(size_t)z * count + i];
     out[i] = s;
 }
 
 // Adam step on the FP32 master parameters, writing their BF16 copy too. Padding parameters
 // (W1 rows >= 35, W4 / b4 columns >= 4) take no step, so they stay zero.
-__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                       __bf16* __restrict__ wb, int n, int off_w1, int off_w4, int off_b4, float lr, float b1, float b2,
-                       float eps, float c1, float c2) {
+// the parameter block's geometry: weights of layer L at off_w[L] ([k x n] row-major), and
+// the transposed BF16 copy's element for parameter i (-1 for biases)
+struct ParamLayout {
+    int off_w[4], k[4], n[4];
+};
+__device__ __forceinline__ int transposed_index(const ParamLayout& P, int i) {
+#pragma unroll
+    for (int L = 0; L < 4; L++) {
+        const int t = i - P.off_w[L];
+        if (t >= 0 && t < P.k[L] * P.n[L]) {
+            const int kk = t / P.n[L], nn = t - kk * P.n[L];
+            return P.off_w[L] + nn * P.k[L] + kk;
+        }
+    }
+    return -1;
+}
+
+// g: nsplit gradient copies n apart, added in split order
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, int nsplit, float* __restrict__ m,
+                       float* __restrict__ v, __bf16* __restrict__ wb, __bf16* __restrict__ wbt, ParamLayout P, int n,
+                       int off_w1, int off_w4, int off_b4, float lr, float b1, float b2, float eps, float c1, float c2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool pad = (i >= off_w1 + NIN * HID && i < off_w1 + K0 * HID) ||
                      (i >= off_w4 && i < off_w4 + HID * NOP && (i - off_w4) % NOP >= NOUT) ||
                      (i >= off_b4 && i < off_b4 + NOP && i - off_b4 >= NOUT);
     if (pad) return;
-    const float gi = g[i];
+    float gi = g[i];
+    for (int z = 1; z < nsplit; z++) gi += g[(size_t)z * n + i];
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
@@ -193,11 +340,18 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
     w[i] = wi;
     wb[i] = to_bf(wi);
+    const int it = transposed_index(P, i);
+    if (it >= 0) wbt[it] = to_bf(wi);
 }
 
-__global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict__ y) {
+// BF16 copies of the weights: row-major and transposed
+__global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict__ y, __bf16* __restrict__ yt,
+                          ParamLayout P) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = to_bf(x[i]);
+    if (i >= n) return;
+    y[i] = to_bf(x[i]);
+    const int it = transposed_index(P, i);
+    if (it >= 0) yt[it] = to_bf(x[i]);
 }
 
 // robust score from the three regression heads; -100 when the oversize head says so
@@ -224,22 +378,33 @@ struct pr_mlp {
     int max_batch = 0;
     size_t off_w[NL], off_b[NL], n_params = 0;
     float* w = nullptr;   // master FP32 parameters
-    float* g = nullptr;   // gradients
+    float* g = nullptr;   // gradients (the splits summed; for pr_mlp_loss_grad)
+    float* gpart = nullptr;  // [WG_SPLITS x n_params] gradient copies of the batch splits
+    int nsplit = 1;          // splits the last backward pass wrote
     float* m = nullptr;   // Adam first moment
     float* v = nullptr;   // Adam second moment
     __bf16* wb = nullptr; // BF16 copy of the weights (biases stay FP32)
+    __bf16* wbt = nullptr;  // BF16 copy of each layer's weights transposed ([n x k]), for gemm_nt
+    ParamLayout layout = {};
+    // prediction: its own buffers, PRED_ROWS rows per pass (fewer, larger launches)
+    __bf16* Xp = nullptr;
+    __bf16* Hp[2] = {};
+    float* outp = nullptr;
     __bf16* X = nullptr;  // [B x 64]
     __bf16* H[3] = {};    // [B x 256] post-ReLU activations
     __bf16* dH[3] = {};   // [B x 256] gradients
-    float* out = nullptr; // [B x 16]
+    // transposes (features x B) of the layer inputs and output gradients: the operands of the
+    // weight gradients
+    __bf16* Xt = nullptr;
+    __bf16* Ht[3] = {};
+    __bf16* dHt[3] = {};
+    __bf16* doutT = nullptr;  // [32 x B]
+    float* out = nullptr; // [B x 32]
     __bf16* dout = nullptr;
-    float* dout_f = nullptr;
     float* loss = nullptr;
     float* loss_part = nullptr;  // [max_batch / 64] per-block loss sums
     float* lo = nullptr;
     float* inv_range = nullptr;
-    float* colsum_part = nullptr;  // [COLSUM_SLICES x 256] bias-gradient partial sums
-    float* wgrad_part = nullptr;   // [batch splits x 256 x 256] weight-gradient partial products
     int step = 0;
     hipStream_t s = nullptr;
 };
@@ -250,7 +415,9 @@ struct pr_mlp {
         if (e_ != hipSuccess) { g_mlp_err = std::string(#x) + ": " + hipGetErrorString(e_); return (int)e_; } \
     } while (0)
 
-static int pad32(int n) { return (n + 31) / 32 * 32; }
+static int pad_rows(int n) { return (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD; }
+constexpr int PRED_ROWS = 65536;
+constexpr int WG_SPLITS = 4;  // batch splits of the weight gradients (their copies summed by the Adam step)
 
 // deterministic He-uniform initialisation from a 32-bit hash of (seed, index)
 static float init_u(unsigned seed, size_t i) {
@@ -259,92 +426,106 @@ static float init_u(unsigned seed, size_t i) {
     return (float)(x >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
 }
 
-static int forward(pr_mlp* q, int B, bool keep_out) {
-    // layer 1..3: H = ReLU(X W + b) (BF16); layer 4: out = H3 W4 + b4 (FP32)
+// C[M x N] = A[M x K] * (Bt[N x K])^T through gemm_nt, with the workgroup tile chosen for the
+// size: 128 x 128 for large batches (M >= 32768), 64 x 64 otherwise (a 4096-row batch then
+// spreads over 256 workgroups), 64 x 32 for the 32-wide output layer.
+template <int EPI>
+static void launch_nt(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* Bt, int ldb,
+                      const float* bias, const __bf16* mask, int ldm, void* C, int ldc, __bf16* Ct, int ldct) {
+    if (N % 128 == 0 && M >= 32768 && M % 128 == 0)
+        hipLaunchKernelGGL((gemm_nt<EPI, 2, 2, 2, 2>), dim3(M / 128, N / 128), dim3(256), 0, st, A, lda, Bt, ldb, K,
+                           bias, mask, ldm, C, ldc, Ct, ldct);
+    else if (N % 64 == 0)
+        hipLaunchKernelGGL((gemm_nt<EPI, 2, 2, 1, 1>), dim3(M / 64, N / 64), dim3(256), 0, st, A, lda, Bt, ldb, K, bias,
+                           mask, ldm, C, ldc, Ct, ldct);
+    else
+        hipLaunchKernelGGL((gemm_nt<EPI, 2, 1, 1, 1>), dim3(M / 64, N / 32), dim3(128), 0, st, A, lda, Bt, ldb, K, bias,
+                           mask, ldm, C, ldc, Ct, ldct);
+}
+
+// One layer's forward GEMM: [B x K] x W[K x N] (its transposed copy) + bias (ReLU, BF16 for the
+// hidden layers, FP32 for the output layer); Ht: the activations' transpose as well (training)
+static void launch_layer(pr_mlp* q, int L, const __bf16* in, int B, void* out, __bf16* Ht) {
+    const __bf16* Wt = q->wbt + q->off_w[L];
+    const float* b = q->w + q->off_b[L];
+    if (L < NL - 1)
+        launch_nt<EPI_BIAS_RELU_BF16>(q->s, B, LN[L], LK[L], in, LK[L], Wt, LK[L], b, nullptr, 0, out, LN[L], Ht, B);
+    else
+        launch_nt<EPI_BIAS_F32>(q->s, B, LN[L], LK[L], in, LK[L], Wt, LK[L], b, nullptr, 0, out, NOP, nullptr, 0);
+}
+
+static int forward(pr_mlp* q, int B) {
+    // layer 1..3: H = ReLU(X W + b) (BF16, and its transpose); layer 4: out = H3 W4 + b4 (FP32)
     const __bf16* in = q->X;
-    int ldin = K0;
     for (int L = 0; L < NL; L++) {
-        const __bf16* W = q->wb + q->off_w[L];
-        const float* b = q->w + q->off_b[L];
-        dim3 grid(B / 16, LN[L] / 16, 1);
-        if (L < NL - 1) {
-            hipLaunchKernelGGL((gemm_bf16<false, false, EPI_BIAS_RELU_BF16>), grid, dim3(64), 0, q->s, in, ldin, W,
-                               LN[L], LK[L], LK[L], b, nullptr, 0, (void*)q->H[L], LN[L]);
-            in = q->H[L];
-            ldin = LN[L];
-        } else {
-            hipLaunchKernelGGL((gemm_bf16<false, false, EPI_BIAS_F32>), grid, dim3(64), 0, q->s, in, ldin, W, LN[L],
-                               LK[L], LK[L], b, nullptr, 0, (void*)q->out, NOP);
-        }
+        launch_layer(q, L, in, B, L < NL - 1 ? (void*)q->H[L] : (void*)q->out, L < NL - 1 ? q->Ht[L] : nullptr);
         MCK(hipGetLastError());
+        if (L < NL - 1) in = q->H[L];
     }
-    (void)keep_out;
     return 0;
 }
 
-// forward + loss + backward into q->g (FP32 gradients of the mean loss over n samples)
-static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, float* loss_out) {
-    const int B = pad32(n);
+// forward pass of a prediction chunk on the prediction buffers (activations ping-pong)
+static int forward_pred(pr_mlp* q, int B) {
+    const __bf16* in = q->Xp;
+    for (int L = 0; L < NL; L++) {
+        void* out = L < NL - 1 ? (void*)q->Hp[L & 1] : (void*)q->outp;
+        launch_layer(q, L, in, B, out, nullptr);
+        MCK(hipGetLastError());
+        if (L < NL - 1) in = q->Hp[L & 1];
+    }
+    return 0;
+}
+
+// forward + loss + backward into q->gpart (FP32 gradients of the mean loss over n samples, one
+// copy per batch split of the weight gradients: q->nsplit);
+// want_loss: also reduce the mean loss into q->loss (the caller reads it)
+static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, bool want_loss) {
+    const int B = pad_rows(n);
     hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes, n, B, q->lo, q->inv_range,
-                       q->X);
+                       q->X, q->Xt);
     MCK(hipGetLastError());
-    int rc = forward(q, B, true);
+    int rc = forward(q, B);
     if (rc) return rc;
-    // every gradient entry is written below (weights by k_sum_splits, biases by k_colsum_fin)
     hipLaunchKernelGGL(k_loss, dim3((B + 63) / 64), dim3(64), 0, q->s, q->out, d_targets, n, B, 1.0f / n, q->dout,
-                       q->dout_f, q->loss_part);
+                       q->doutT, q->loss_part);
     MCK(hipGetLastError());
-    hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, q->s, q->loss_part, (B + 63) / 64, 1.0f / n, q->loss);
-    MCK(hipGetLastError());
-    // backward, layer 4 .. 1
+    if (want_loss) {
+        hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, q->s, q->loss_part, (B + 63) / 64, 1.0f / n, q->loss);
+        MCK(hipGetLastError());
+    }
+    // backward, layer 4 .. 1; every gradient entry is written (weights and biases by gemm_wgrad)
     const __bf16* dY = q->dout;
+    const __bf16* dYt = q->doutT;
     int ldy = NOP;
     for (int L = NL - 1; L >= 0; L--) {
-        const __bf16* Xin = L == 0 ? q->X : q->H[L - 1];
+        const __bf16* Xt = L == 0 ? q->Xt : q->Ht[L - 1];
         const int K = LK[L], N = LN[L];
-        // dW[K x N] = Xin^T [K x B] * dY [B x N]: partial products per batch split of
-        // WG_SPLIT rows, then summed in split order (deterministic, no atomics); the batch is
-        // padded to a multiple of 32 with zero rows (zero activations and gradients)
-        const int nsplit = (B + WG_SPLIT - 1) / WG_SPLIT;
-        dim3 gw(K / 16, N / 16, nsplit);
-        hipLaunchKernelGGL((gemm_bf16<true, false, EPI_PART_F32>), gw, dim3(64), 0, q->s, Xin, K, dY, ldy, B, WG_SPLIT,
-                           nullptr, nullptr, 0, (void*)q->wgrad_part, N);
-        MCK(hipGetLastError());
-        hipLaunchKernelGGL(k_sum_splits, dim3((K * N + 255) / 256), dim3(256), 0, q->s, q->wgrad_part, nsplit, K * N,
-                           q->g + q->off_w[L]);
-        MCK(hipGetLastError());
-        // db = column sums of dY
-        hipLaunchKernelGGL(k_colsum_part, dim3((N + 63) / 64, COLSUM_SLICES), dim3(256), 0, q->s, dY, B, N,
-                           q->colsum_part);
-        MCK(hipGetLastError());
-        hipLaunchKernelGGL(k_colsum_fin, dim3((N + 63) / 64), dim3(64), 0, q->s, q->colsum_part, N,
-                           q->g + q->off_b[L]);
+        // dW [K x N] = Xin^T dY and db = column sums of dY, over the (zero-padded) batch
+        const int z = B % (128 * WG_SPLITS) == 0 ? WG_SPLITS : 1;
+        hipLaunchKernelGGL(gemm_wgrad<true>, dim3(K / 32, N / 32, z), dim3(512), 0, q->s, Xt, dYt, B, B,
+                           q->gpart + q->off_w[L], N, q->gpart + q->off_b[L], q->n_params);
+        q->nsplit = z;
         MCK(hipGetLastError());
         if (L > 0) {
-            // dXin [B x K] = dY [B x N] * W^T, masked by ReLU'(Xin) (Xin > 0)
+            // dXin [B x K] = dY [B x N] W^T, masked by ReLU'(Xin) (Xin > 0); W [K x N] row-major is
+            // the transposed operand gemm_nt takes; the transpose feeds the next weight gradient
             const __bf16* W = q->wb + q->off_w[L];
-            dim3 gx(B / 16, K / 16, 1);
-            hipLaunchKernelGGL((gemm_bf16<false, true, EPI_MASK_BF16>), gx, dim3(64), 0, q->s, dY, ldy, W, N, N, N,
-                               nullptr, Xin, K, (void*)q->dH[L - 1], K);
+            launch_nt<EPI_MASK_BF16>(q->s, B, K, N, dY, ldy, W, N, nullptr, q->H[L - 1], K, (void*)q->dH[L - 1], K,
+                                     q->dHt[L - 1], B);
             MCK(hipGetLastError());
             dY = q->dH[L - 1];
+            dYt = q->dHt[L - 1];
             ldy = K;
         }
-    }
-    if (loss_out) {
-        MCK(hipMemcpyAsync(loss_out, q->loss, sizeof(float), hipMemcpyDeviceToHost, q->s));
-        MCK(hipStreamSynchronize(q->s));
     }
     return 0;
 }
 
 static int refresh_bf16(pr_mlp* q) {
-    for (int L = 0; L < NL; L++) {
-        int n = LK[L] * LN[L];
-        hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, q->s, q->w + q->off_w[L], n,
-                           q->wb + q->off_w[L]);
-        MCK(hipGetLastError());
-    }
+    const int n = (int)q->n_params;
+    hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, q->s, q->w, n, q->wb, q->wbt, q->layout);
+    MCK(hipGetLastError());
     return 0;
 }
 
@@ -366,24 +547,34 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     for (int L = 0; L < NL; L++) {
         q->off_w[L] = o; o += (size_t)LK[L] * LN[L];
         q->off_b[L] = o; o += LN[L];
+        q->layout.off_w[L] = (int)q->off_w[L];
+        q->layout.k[L] = LK[L];
+        q->layout.n[L] = LN[L];
     }
     q->n_params = o;
-    const int B = q->max_batch;
+    const int B = pad_rows(q->max_batch);
 #define MALLOC(p, bytes) MCK(hipMalloc((void**)&(p), (bytes)))
     MCK(hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
     MALLOC(q->w, 4 * o); MALLOC(q->g, 4 * o); MALLOC(q->m, 4 * o); MALLOC(q->v, 4 * o);
+    MALLOC(q->gpart, 4 * o * WG_SPLITS);
     MALLOC(q->wb, 2 * o);
+    MALLOC(q->wbt, 2 * o);
+    MALLOC(q->Xp, 2 * (size_t)PRED_ROWS * K0);
+    for (int k = 0; k < 2; k++) MALLOC(q->Hp[k], 2 * (size_t)PRED_ROWS * HID);
+    MALLOC(q->outp, 4 * (size_t)PRED_ROWS * NOP);
     MALLOC(q->X, 2 * (size_t)B * K0);
-    for (int k = 0; k < 3; k++) { MALLOC(q->H[k], 2 * (size_t)B * HID); MALLOC(q->dH[k], 2 * (size_t)B * HID); }
+    MALLOC(q->Xt, 2 * (size_t)B * K0);
+    for (int k = 0; k < 3; k++) {
+        MALLOC(q->H[k], 2 * (size_t)B * HID); MALLOC(q->dH[k], 2 * (size_t)B * HID);
+        MALLOC(q->Ht[k], 2 * (size_t)B * HID); MALLOC(q->dHt[k], 2 * (size_t)B * HID);
+    }
     MALLOC(q->out, 4 * (size_t)B * NOP);
     MALLOC(q->dout, 2 * (size_t)B * NOP);
-    MALLOC(q->dout_f, 4 * (size_t)B * NOP);
+    MALLOC(q->doutT, 2 * (size_t)B * NOP);
     MALLOC(q->loss, 4);
-    MALLOC(q->loss_part, 4 * (size_t)(B / 64 + 1));
+    MALLOC(q->loss_part, 4 * (size_t)(B / 64 + 1));  // B is a multiple of 128
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
-    MALLOC(q->colsum_part, 4 * (size_t)COLSUM_SLICES * HID);
-    MALLOC(q->wgrad_part, 4 * (size_t)((B + WG_SPLIT - 1) / WG_SPLIT) * HID * HID);
 #undef MALLOC
     float lo[NIN], hi[NIN], ir[NIN];
     pr_gene_bounds(lo, hi);
@@ -413,9 +604,9 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
 void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
-    void* bufs[] = {q->w, q->g, q->m, q->v, q->wb, q->X, q->out, q->dout, q->dout_f, q->loss, q->loss_part, q->lo, q->inv_range,
-                    q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2], q->colsum_part,
-                    q->wgrad_part};
+    void* bufs[] = {q->w, q->g, q->gpart, q->m, q->v, q->wb, q->wbt, q->X, q->Xt, q->out, q->dout, q->doutT, q->loss,
+                    q->loss_part, q->lo, q->inv_range, q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2],
+                    q->Ht[0], q->Ht[1], q->Ht[2], q->dHt[0], q->dHt[1], q->dHt[2], q->Xp, q->Hp[0], q->Hp[1], q->outp};
     for (void* b : bufs) if (b) (void)hipFree(b);
     if (q->s) (void)hipStreamDestroy(q->s);
     delete q;
@@ -443,15 +634,15 @@ int pr_mlp_set_params(pr_mlp* q, const float* in, int n) {
 // (metres), oversize probability, predicted robust score (-100 if predicted oversize).
 int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
     if (!q || n < 1) { g_mlp_err = "pr_mlp_predict: n >= 1"; return -1; }
-    for (int s0 = 0; s0 < n; s0 += q->max_batch) {
-        const int nb = std::min(q->max_batch, n - s0);
-        const int B = pad32(nb);
+    for (int s0 = 0; s0 < n; s0 += PRED_ROWS) {
+        const int nb = std::min(PRED_ROWS, n - s0);
+        const int B = pad_rows(nb);
         hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes + (size_t)s0 * NIN, nb,
-                           B, q->lo, q->inv_range, q->X);
+                           B, q->lo, q->inv_range, q->Xp, nullptr);
         MCK(hipGetLastError());
-        int rc = forward(q, B, false);
+        int rc = forward_pred(q, B);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_head_scores, dim3((nb + 255) / 256), dim3(256), 0, q->s, q->out, nb,
+        hipLaunchKernelGGL(k_head_scores, dim3((nb + 255) / 256), dim3(256), 0, q->s, q->outp, nb,
                            d_pred + (size_t)s0 * 5);
         MCK(hipGetLastError());
     }
@@ -463,9 +654,19 @@ int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
 // parameter layout of pr_mlp_get_params.
 int pr_mlp_loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, float* loss, float* grads) {
     if (!q || n < 1 || n > q->max_batch) { g_mlp_err = "pr_mlp_loss_grad: 1 <= n <= max_batch"; return -1; }
-    int rc = loss_grad(q, d_genomes, d_targets, n, loss);
+    int rc = loss_grad(q, d_genomes, d_targets, n, true);
     if (rc) return rc;
-    if (grads) MCK(hipMemcpy(grads, q->g, 4 * q->n_params, hipMemcpyDeviceToHost));
+    if (loss) {
+        MCK(hipMemcpyAsync(loss, q->loss, sizeof(float), hipMemcpyDeviceToHost, q->s));
+        MCK(hipStreamSynchronize(q->s));
+    }
+    if (grads) {
+        hipLaunchKernelGGL(k_sum_splits, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->gpart,
+                           q->nsplit, (int)q->n_params, q->g);
+        MCK(hipGetLastError());
+        MCK(hipStreamSynchronize(q->s));
+        MCK(hipMemcpy(grads, q->g, 4 * q->n_params, hipMemcpyDeviceToHost));
+    }
     MCK(hipStreamSynchronize(q->s));
     return 0;
 }
@@ -480,14 +681,16 @@ int pr_mlp_train(pr_mlp* q, const float* d_genomes, const float* d_targets, int 
     for (int e = 0; e < epochs; e++) {
         for (int s0 = 0; s0 + 32 <= n; s0 += batch) {
             const int nb = std::min(batch, n - s0) / 32 * 32;
-            int rc = loss_grad(q, d_genomes + (size_t)s0 * NIN, d_targets + (size_t)s0 * NOUT, nb, nullptr);
+            const bool last = e == epochs - 1 && s0 + batch + 32 > n;  // the loss the caller reads
+            int rc = loss_grad(q, d_genomes + (size_t)s0 * NIN, d_targets + (size_t)s0 * NOUT, nb, last);
             if (rc) return rc;
             q->step++;
             const float c1 = 1.0f / (1.0f - std::pow(b1, (float)q->step));
             const float c2 = 1.0f / (1.0f - std::pow(b2, (float)q->step));
-            hipLaunchKernelGGL(k_adam, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->w, q->g, q->m,
-                               q->v, q->wb, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3], (int)q->off_b[3], lr,
-                               b1, b2, eps, c1, c2);
+            hipLaunchKernelGGL(k_adam, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->w, q->gpart,
+                               q->nsplit, q->m,
+                               q->v, q->wb, q->wbt, q->layout, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3],
+                               (int)q->off_b[3], lr, b1, b2, eps, c1, c2);
             MCK(hipGetLastError());
         }
     }

@@ -73,6 +73,25 @@ def test_mlp_forward_matches_cpu_model():
 
 
 @pytest.mark.gpu
+def test_mlp_prediction_in_chunks_matches_cpu_model():
+    """Prediction runs in passes of 65536 rows on its own buffers: rows at the start, across
+    the first pass boundary and at the end of a 70000-row call equal the CPU model's (BF16
+    mimic), and a row's prediction does not depend on the call it is part of."""
+    import planes_amd as PA
+    g, t, lo, hi = _data(70000, 8)
+    p = _params(9)
+    S = PA.Surrogate(max_batch=512)
+    S.set_params(p)
+    pred = S.predict(g)
+    rows = np.r_[0:64, 65500:65600, 69936:70000]
+    out, _ = M.forward(p, g[rows], lo, hi, bf=True)
+    ref = np.concatenate([10 * out[:, :3], 1 / (1 + np.exp(-out[:, 3:4]))], 1)
+    err = np.abs(pred[rows, :4] - ref) / (np.abs(ref) + 0.05)
+    assert err.max() < 2e-3, err.max()
+    assert np.array_equal(S.predict(g[65500:65600]), pred[65500:65600])
+
+
+@pytest.mark.gpu
 def test_mlp_gradients_vs_cpu_model():
     import planes_amd as PA
     g, t, lo, hi = _data(512, 6)
