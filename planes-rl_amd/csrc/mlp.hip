@@ -17,7 +17,8 @@
 //               bias gradient (column sums of dY) from an all-ones MFMA operand alongside:
 //               training is deterministic.
 // A training step is 14 launches (embedding, 4 forward, loss, 4 weight + bias gradients,
-// 3 input gradients, Adam); prediction runs 65536 rows per pass on buffers of its own.
+// 3 input gradients, Adam); prediction runs 65536 rows per pass on buffers of its own, its
+// first layer embedding the genomes on the way into LDS (gemm_lds).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -46,6 +47,44 @@ enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2 };
 
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+
+// Epilogue of a wave's (32 WM) x (32 WN) accumulator tiles at (m0, n0): bias + ReLU -> BF16,
+// bias -> FP32, or the ReLU mask -> BF16; BF16 results also transposed into Ct when given,
+// four rows at a time as one 8-byte store.
+template <int EPI, int WM, int WN>
+__device__ __forceinline__ void store_tiles(const f16v (&acc)[WM][WN], int m0, int n0, int l,
+                                            const float* __restrict__ bias, const __bf16* __restrict__ mask, int ldm,
+                                            void* __restrict__ C, int ldc, __bf16* __restrict__ Ct, int ldct) {
+    const int lr = l & 31;
+#pragma unroll
+    for (int i = 0; i < WM; i++)
+#pragma unroll
+        for (int j = 0; j < WN; j++)
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int mg = m0 + 32 * i + 8 * g + 4 * (l >> 5), n = n0 + 32 * j + lr;
+                __bf16 o[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int m = mg + q;
+                    float v = acc[i][j][4 * g + q];
+                    if (EPI == EPI_BIAS_RELU_BF16) {
+                        o[q] = to_bf(fmaxf(v + bias[n], 0.0f));
+                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
+                    } else if (EPI == EPI_BIAS_F32) {
+                        reinterpret_cast<float*>(C)[(size_t)m * ldc + n] = v + bias[n];
+                    } else if (EPI == EPI_MASK_BF16) {
+                        const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
+                        o[q] = to_bf(v * keep);
+                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
+                    }
+                }
+                if ((EPI == EPI_BIAS_RELU_BF16 || EPI == EPI_MASK_BF16) && Ct) {
+                    bf4 t = {o[0], o[1], o[2], o[3]};
+                    *reinterpret_cast<bf4*>(Ct + (size_t)n * ldct + mg) = t;
+                }
+            }
+}
 
 // C[M x N] = A[M x K] * B[K x N] with both operands K-contiguous: A row-major (lda), and B
 // given as its transpose Bt[N x K] (row-major, ldb): B(k, n) = Bt[n * ldb + k].  The forward
@@ -114,46 +153,94 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_nt(const __bf16* __restri
             }
         }
     }
+    store_tiles<EPI, WM, WN>(acc, m0, n0, l, bias, mask, ldm, C, ldc, Ct, ldct);
+}
+
+// gemm_nt with the operands staged through LDS: a workgroup of WGM x WGN waves computes a
+// (32 WM WGM) x (32 WN WGN) tile; each K chunk of 64 of its A rows and Bt rows is loaded with
+// 16-byte vector loads covering whole 128-byte rows (each operand element read from global
+// memory once per workgroup instead of once per wave), written to one of two LDS buffers (rows
+// padded by 16 bytes), and the waves' MFMA operands are 16-byte LDS reads.  The next chunk's
+// global loads are in flight while the current one is multiplied; one barrier per chunk.
+// M, N multiples of the workgroup tile, K a multiple of 64.
+template <int EPI, int WGM, int WGN, int WM, int WN>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_lds(const __bf16* __restrict__ A, int lda,
+                                                           const __bf16* __restrict__ Bt, int ldb, int K,
+                                                           const float* __restrict__ bias,
+                                                           const __bf16* __restrict__ mask, int ldm,
+                                                           void* __restrict__ C, int ldc, __bf16* __restrict__ Ct,
+                                                           int ldct) {
+    constexpr int NT = 64 * WGM * WGN;
+    constexpr int BM = 32 * WM * WGM, BN = 32 * WN * WGN, BK = 64, PITCH = BK + 8;
+    constexpr int VA = BM * 8 / NT, VB = BN * 8 / NT;  // staging vectors per thread and chunk
+    static_assert(VA >= 1 && VB >= 1 && VA * NT == BM * 8 && VB * NT == BN * 8, "staging tiles");
+    __shared__ __attribute__((aligned(16))) __bf16 sA[2][BM * PITCH];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[2][BN * PITCH];
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int bm0 = blockIdx.x * BM, bn0 = blockIdx.y * BN;
+    const int wm0 = (w / WGN) * (32 * WM), wn0 = (w % WGN) * (32 * WN);
+    const int lr = l & 31, kq = 8 * (l >> 5);
+    // staging: thread t moves vectors t + NT v (8 per 128-byte row: row = idx / 8, column 8 (idx % 8))
+    bf8 na[VA], nb[VB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int v = 0; v < VA; v++) {
+            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
+            na[v] = *reinterpret_cast<const bf8*>(A + (size_t)(bm0 + r) * lda + k0 + c);
+        }
+#pragma unroll
+        for (int v = 0; v < VB; v++) {
+            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
+            nb[v] = *reinterpret_cast<const bf8*>(Bt + (size_t)(bn0 + r) * ldb + k0 + c);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int v = 0; v < VA; v++) {
+            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
+            *reinterpret_cast<bf8*>(&sA[buf][r * PITCH + c]) = na[v];
+        }
+#pragma unroll
+        for (int v = 0; v < VB; v++) {
+            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
+            *reinterpret_cast<bf8*>(&sB[buf][r * PITCH + c]) = nb[v];
+        }
+    };
+    f16v acc[WM][WN];
 #pragma unroll
     for (int i = 0; i < WM; i++)
 #pragma unroll
         for (int j = 0; j < WN; j++)
 #pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const int mg = m0 + 32 * i + 8 * g + 4 * (l >> 5), n = n0 + 32 * j + lr;
-                __bf16 o[4];
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+    const int nch = K / BK;
+    load(0);
+    store(0);
+    for (int c = 0; c < nch; c++) {
+        __syncthreads();
+        if (c + 1 < nch) load((c + 1) * BK);
+        const __bf16* a_s = sA[c & 1];
+        const __bf16* b_s = sB[c & 1];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int m = mg + q;
-                    float v = acc[i][j][4 * g + q];
-                    if (EPI == EPI_BIAS_RELU_BF16) {
-                        o[q] = to_bf(fmaxf(v + bias[n], 0.0f));
-                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
-                    } else if (EPI == EPI_BIAS_F32) {
-                        reinterpret_cast<float*>(C)[(size_t)m * ldc + n] = v + bias[n];
-                    } else if (EPI == EPI_MASK_BF16) {
-                        const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
-                        o[q] = to_bf(v * keep);
-                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
-                    }
-                }
-                if ((EPI == EPI_BIAS_RELU_BF16 || EPI == EPI_MASK_BF16) && Ct) {
-                    bf4 t = {o[0], o[1], o[2], o[3]};
-                    *reinterpret_cast<bf4*>(Ct + (size_t)n * ldct + mg) = t;
-                }
-            }
+        for (int ks = 0; ks < BK; ks += 16) {
+            bf8 a[WM], b[WN];
+#pragma unroll
+            for (int i = 0; i < WM; i++)
+                a[i] = *reinterpret_cast<const bf8*>(&a_s[(wm0 + 32 * i + lr) * PITCH + ks + kq]);
+#pragma unroll
+            for (int j = 0; j < WN; j++)
+                b[j] = *reinterpret_cast<const bf8*>(&b_s[(wn0 + 32 * j + lr) * PITCH + ks + kq]);
+#pragma unroll
+            for (int i = 0; i < WM; i++)
+#pragma unroll
+                for (int j = 0; j < WN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (c + 1 < nch) store((c + 1) & 1);
+    }
+    store_tiles<EPI, WM, WN>(acc, bm0 + wm0, bn0 + wn0, l, bias, mask, ldm, C, ldc, Ct, ldct);
 }
 
-// Weight (and bias) gradients: dW[M x N] (FP32) = A[M x Kd] * Bt[N x Kd]^T over the batch
-// Kd, with A = the layer input transposed (features x batch) and Bt = the output gradient
-// transposed -- both batch-contiguous, so the operands are 16-byte loads as in gemm_nt.  The
-// batch is split gridDim.z ways (so that the few 32 x 32 tiles of a layer fill the GPU); a
-// workgroup of 8 waves computes one tile of one split: wave w sums its slice, the eight
-// partial tiles are added in wave order through LDS, and split z writes C + z split_stride
-// (the Adam step adds the splits in order: deterministic, no atomics).  With BIAS, the
-// workgroups of row tile 0 also form db[n] = sum over the batch of dY[b][n]: an MFMA with an
-// all-ones A operand on the same Bt operands gives the column sums of the slice, added in
-// wave order likewise.  Kd a multiple of 128 gridDim.z.
 // k-loop of one wave of gemm_wgrad over its batch slice (ns k-steps), operands D steps ahead
 template <bool DOB>
 __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, int ns, f16v& acc, f16v& accb) {
@@ -354,17 +441,183 @@ __global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict
     if (it >= 0) yt[it] = to_bf(x[i]);
 }
 
-// robust score from the three regression heads; -100 when the oversize head says so
-__global__ void k_head_scores(const float* __restrict__ out, int n, float* __restrict__ pred) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const float* o = out + (size_t)r * NOP;
-    float s = 10.0f * (0.65f * o[0] + 0.25f * o[1] + 0.10f * o[2]);
-    pred[(size_t)r * 5 + 0] = 10.0f * o[0];
-    pred[(size_t)r * 5 + 1] = 10.0f * o[1];
-    pred[(size_t)r * 5 + 2] = 10.0f * o[2];
-    pred[(size_t)r * 5 + 3] = 1.0f / (1.0f + expf(-o[3]));
-    pred[(size_t)r * 5 + 4] = o[3] > 0.0f ? -100.0f : s;
+// the five prediction values of one row from its four head outputs: mean, CVaR and worst
+// distance (metres), oversize probability, and the robust score from the three regression
+// heads (-100 when the oversize head says so)
+__device__ __forceinline__ void head_values(const float (&o)[NOUT], float* __restrict__ pr) {
+    const float s = 10.0f * (0.65f * o[0] + 0.25f * o[1] + 0.10f * o[2]);
+    pr[0] = 10.0f * o[0];
+    pr[1] = 10.0f * o[1];
+    pr[2] = 10.0f * o[2];
+    pr[3] = 1.0f / (1.0f + expf(-o[3]));
+    pr[4] = o[3] > 0.0f ? -100.0f : s;
+}
+
+// ---- prediction in one pass: k_mlp_predict ----
+// A workgroup of four waves takes 128 rows through the whole network: the genomes are
+// embedded straight into LDS, each layer reads its input activations from an LDS buffer and,
+// once every wave has read them, overwrites them with its output, and the heads are evaluated
+// in registers, so the only HBM traffic is the genomes in and the five values per row out.
+// One activation buffer (66 KB) lets two workgroups share a CU, each one's loads, epilogues
+// and barriers overlapping the other's products.  Each layer is computed transposed,
+// C^T = W^T H^T: the MFMA's A operand is a 32-feature tile of the layer's transposed weight
+// copy (16-byte loads from L2, where every workgroup finds the 311 KB of weights), its B
+// operand a 32-row tile of activations (16-byte LDS reads), and accumulator element r of lane l
+// is then feature 8 (r / 4) + 4 (l / 32) + r % 4 of row l % 32 -- four consecutive features of
+// one row, one 8-byte LDS write in the row-major layout the next layer reads.  Wave w computes
+// features 64 w .. 64 w + 63 of all 128 rows (2 x 4 tiles); the output layer (32 features,
+// four of them live) has one tile, and wave w takes rows 32 w .. 32 w + 31.  The weight stream
+// runs through a ring of PD k-steps in registers that, near the end of a layer, is refilled
+// from the next layer's weights.
+constexpr int PB_ROWS = 128;           // rows per workgroup
+constexpr int PB_XP = K0 + 8;          // LDS row pitch of the embedded inputs (bf16; 16-byte multiple)
+constexpr int PB_HP = HID + 8;         // LDS row pitch of the hidden activations
+constexpr int PD = 4;                  // weight k-steps in flight
+// LDS copy of the biases and gene bounds: b1, b2, b3 [256 each], b4 [4], lo [35], 1/range [35]
+constexpr int PB_B4 = 3 * HID, PB_LO = PB_B4 + NOUT, PB_IR = PB_LO + NIN, PB_CONST = PB_IR + NIN;
+
+// one wave's k-loop of a layer: acc[i][j] += (W^T tile i) x (activation tile j)^T over KIN;
+// ring slot s % PD holds k-step s of this layer on entry and k-step s of the next layer on exit
+template <int KIN, int NA, int NB, int NA_NEXT>
+__device__ __forceinline__ void fused_layer_k(f16v (&acc)[NA][NB], bf8 (&ring)[PD][2], const __bf16* __restrict__ pw,
+                                              int ldw, const __bf16* __restrict__ pw_next, int ldw_next,
+                                              const __bf16* __restrict__ act, int pitch, int j0, int lr, int kq) {
+    constexpr int NS = KIN / 16;
+    static_assert(NS % PD == 0, "k-steps a multiple of the ring depth");
+#pragma unroll
+    for (int i = 0; i < NA; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int d = s % PD;
+        bf8 a[NA];
+#pragma unroll
+        for (int i = 0; i < NA; i++) a[i] = ring[d][i];
+        if (s + PD < NS) {
+#pragma unroll
+            for (int i = 0; i < NA; i++)
+                ring[d][i] = *reinterpret_cast<const bf8*>(pw + (size_t)32 * i * ldw + 16 * (s + PD));
+        } else if (NA_NEXT > 0) {
+#pragma unroll
+            for (int i = 0; i < NA_NEXT; i++)
+                ring[d][i] = *reinterpret_cast<const bf8*>(pw_next + (size_t)32 * i * ldw_next + 16 * (s + PD - NS));
+        }
+        bf8 b[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            b[j] = *reinterpret_cast<const bf8*>(act + (32 * (j0 + j) + lr) * pitch + 16 * s + kq);
+#pragma unroll
+        for (int i = 0; i < NA; i++)
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// bias + ReLU of a hidden layer's transposed tiles -> BF16 rows of the LDS activation buffer
+// (features n0 .. n0 + 63 of rows 0 .. 127); bias from the LDS copy
+__device__ __forceinline__ void fused_relu_to_lds(const f16v (&acc)[2][4], const float* __restrict__ bias, int n0,
+                                                  __bf16* __restrict__ out, int l) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int n = n0 + 32 * i + 8 * g + 4 * (l >> 5);
+            const f4 bv = *reinterpret_cast<const f4*>(bias + n);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                bf4 o;
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[q] = to_bf(fmaxf(acc[i][j][4 * g + q] + bv[q], 0.0f));
+                *reinterpret_cast<bf4*>(out + (32 * j + (l & 31)) * PB_HP + n) = o;
+            }
+        }
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict__ genomes, int n,
+                                                        const __bf16* __restrict__ wbt, ParamLayout P,
+                                                        const float* __restrict__ w, const float* __restrict__ lo,
+                                                        const float* __restrict__ inv_range,
+                                                        float* __restrict__ pred) {
+    __shared__ __attribute__((aligned(16))) __bf16 h[PB_ROWS * PB_HP];  // activations (in place)
+    __shared__ __attribute__((aligned(16))) float cst[PB_CONST];
+    const int t = threadIdx.x, l = t & 63, wv = t >> 6, lr = l & 31, kq = 8 * (l >> 5);
+    const int m0 = blockIdx.x * PB_ROWS;
+    const int n0 = 64 * wv;  // this wave's features in the hidden layers
+    const __bf16* pw0 = wbt + P.off_w[0] + (size_t)(n0 + lr) * K0 + kq;
+    const __bf16* pw1 = wbt + P.off_w[1] + (size_t)(n0 + lr) * HID + kq;
+    const __bf16* pw2 = wbt + P.off_w[2] + (size_t)(n0 + lr) * HID + kq;
+    const __bf16* pw3 = wbt + P.off_w[3] + (size_t)lr * HID + kq;
+    bf8 ring[PD][2];
+#pragma unroll
+    for (int d = 0; d < PD; d++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) ring[d][i] = *reinterpret_cast<const bf8*>(pw0 + (size_t)32 * i * K0 + 16 * d);
+
+    // the block's genomes: 128 x 35 consecutive floats, coalesced, all loads issued at once
+    constexpr int NE = (PB_ROWS * NIN + 255) / 256;
+    const int live = min(PB_ROWS, n - m0) * NIN;
+    const float* gb = genomes + (size_t)m0 * NIN;
+    float gv[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = t + 256 * i;
+        gv[i] = e < live ? gb[e] : 0.0f;
+    }
+    for (int i = t; i < PB_CONST; i += 256) {
+        float v;
+        if (i < PB_B4) v = w[P.off_w[i >> 8] + P.k[i >> 8] * P.n[i >> 8] + (i & (HID - 1))];
+        else if (i < PB_LO) v = w[P.off_w[3] + P.k[3] * P.n[3] + (i - PB_B4)];
+        else if (i < PB_IR) v = lo[i - PB_LO];
+        else v = inv_range[i - PB_IR];
+        cst[i] = v;
+    }
+    // padding columns 35..63 of the embedded inputs are zero
+    for (int e = t; e < PB_ROWS * 32; e += 256) {
+        const int r = e >> 5, k = 32 + (e & 31);
+        if (k >= NIN) h[r * PB_XP + k] = to_bf(0.0f);
+    }
+    __syncthreads();
+    // embedding: genes scaled to [0, 1] by the bounds; rows beyond n stay zero
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = t + 256 * i;
+        if (e < PB_ROWS * NIN) {
+            const int r = e / NIN, k = e - r * NIN;
+            h[r * PB_XP + k] = to_bf(e < live ? (gv[i] - cst[PB_LO + k]) * cst[PB_IR + k] : 0.0f);
+        }
+    }
+    __syncthreads();
+
+    f16v acc[2][4];
+    fused_layer_k<K0, 2, 4, 2>(acc, ring, pw0, K0, pw1, HID, h, PB_XP, 0, lr, kq);
+    __syncthreads();  // every wave has read the inputs: overwrite them
+    fused_relu_to_lds(acc, cst, n0, h, l);
+    __syncthreads();
+    fused_layer_k<HID, 2, 4, 2>(acc, ring, pw1, HID, pw2, HID, h, PB_HP, 0, lr, kq);
+    __syncthreads();
+    fused_relu_to_lds(acc, cst + HID, n0, h, l);
+    __syncthreads();
+    fused_layer_k<HID, 2, 4, 1>(acc, ring, pw2, HID, pw3, HID, h, PB_HP, 0, lr, kq);
+    __syncthreads();
+    fused_relu_to_lds(acc, cst + 2 * HID, n0, h, l);
+    __syncthreads();
+    f16v acc3[1][1];
+    fused_layer_k<HID, 1, 1, 0>(acc3, ring, pw3, HID, nullptr, 0, h, PB_HP, wv, lr, kq);
+    // lanes 0..31 hold the four live outputs of row 32 wv + lane in accumulator elements 0..3
+    const int m = m0 + 32 * wv + l;
+    if (l < 32 && m < n) {
+        float o[NOUT];
+#pragma unroll
+        for (int q = 0; q < NOUT; q++) o[q] = acc3[0][0][q] + cst[PB_B4 + q];
+        float pr[5];
+        head_values(o, pr);
+#pragma unroll
+        for (int q = 0; q < 5; q++) pred[(size_t)m * 5 + q] = pr[q];
+    }
 }
 
 
@@ -386,10 +639,6 @@ struct pr_mlp {
     __bf16* wb = nullptr; // BF16 copy of the weights (biases stay FP32)
     __bf16* wbt = nullptr;  // BF16 copy of each layer's weights transposed ([n x k]), for gemm_nt
     ParamLayout layout = {};
-    // prediction: its own buffers, PRED_ROWS rows per pass (fewer, larger launches)
-    __bf16* Xp = nullptr;
-    __bf16* Hp[2] = {};
-    float* outp = nullptr;
     __bf16* X = nullptr;  // [B x 64]
     __bf16* H[3] = {};    // [B x 256] post-ReLU activations
     __bf16* dH[3] = {};   // [B x 256] gradients
@@ -416,7 +665,6 @@ struct pr_mlp {
     } while (0)
 
 static int pad_rows(int n) { return (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD; }
-constexpr int PRED_ROWS = 65536;
 constexpr int WG_SPLITS = 4;  // batch splits of the weight gradients (their copies summed by the Adam step)
 
 // deterministic He-uniform initialisation from a 32-bit hash of (seed, index)
@@ -432,8 +680,11 @@ static float init_u(unsigned seed, size_t i) {
 template <int EPI>
 static void launch_nt(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* Bt, int ldb,
                       const float* bias, const __bf16* mask, int ldm, void* C, int ldc, __bf16* Ct, int ldct) {
-    if (N % 128 == 0 && M >= 32768 && M % 128 == 0)
-        hipLaunchKernelGGL((gemm_nt<EPI, 2, 2, 2, 2>), dim3(M / 128, N / 128), dim3(256), 0, st, A, lda, Bt, ldb, K,
+    if (N % 128 == 0 && M >= 32768 && M % 128 == 0 && K % 64 == 0)
+        hipLaunchKernelGGL((gemm_lds<EPI, 2, 2, 2, 2>), dim3(M / 128, N / 128), dim3(256), 0, st, A, lda, Bt, ldb, K,
+                           bias, mask, ldm, C, ldc, Ct, ldct);
+    else if (N % 64 == 0 && M % 64 == 0 && K % 64 == 0)
+        hipLaunchKernelGGL((gemm_lds<EPI, 2, 2, 1, 1>), dim3(M / 64, N / 64), dim3(256), 0, st, A, lda, Bt, ldb, K,
                            bias, mask, ldm, C, ldc, Ct, ldct);
     else if (N % 64 == 0)
         hipLaunchKernelGGL((gemm_nt<EPI, 2, 2, 1, 1>), dim3(M / 64, N / 64), dim3(256), 0, st, A, lda, Bt, ldb, K, bias,
@@ -461,18 +712,6 @@ static int forward(pr_mlp* q, int B) {
         launch_layer(q, L, in, B, L < NL - 1 ? (void*)q->H[L] : (void*)q->out, L < NL - 1 ? q->Ht[L] : nullptr);
         MCK(hipGetLastError());
         if (L < NL - 1) in = q->H[L];
-    }
-    return 0;
-}
-
-// forward pass of a prediction chunk on the prediction buffers (activations ping-pong)
-static int forward_pred(pr_mlp* q, int B) {
-    const __bf16* in = q->Xp;
-    for (int L = 0; L < NL; L++) {
-        void* out = L < NL - 1 ? (void*)q->Hp[L & 1] : (void*)q->outp;
-        launch_layer(q, L, in, B, out, nullptr);
-        MCK(hipGetLastError());
-        if (L < NL - 1) in = q->Hp[L & 1];
     }
     return 0;
 }
@@ -559,9 +798,6 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     MALLOC(q->gpart, 4 * o * WG_SPLITS);
     MALLOC(q->wb, 2 * o);
     MALLOC(q->wbt, 2 * o);
-    MALLOC(q->Xp, 2 * (size_t)PRED_ROWS * K0);
-    for (int k = 0; k < 2; k++) MALLOC(q->Hp[k], 2 * (size_t)PRED_ROWS * HID);
-    MALLOC(q->outp, 4 * (size_t)PRED_ROWS * NOP);
     MALLOC(q->X, 2 * (size_t)B * K0);
     MALLOC(q->Xt, 2 * (size_t)B * K0);
     for (int k = 0; k < 3; k++) {
@@ -606,7 +842,7 @@ void pr_mlp_destroy(pr_mlp* q) {
     if (q->s) (void)hipStreamSynchronize(q->s);
     void* bufs[] = {q->w, q->g, q->gpart, q->m, q->v, q->wb, q->wbt, q->X, q->Xt, q->out, q->dout, q->doutT, q->loss,
                     q->loss_part, q->lo, q->inv_range, q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2],
-                    q->Ht[0], q->Ht[1], q->Ht[2], q->dHt[0], q->dHt[1], q->dHt[2], q->Xp, q->Hp[0], q->Hp[1], q->outp};
+                    q->Ht[0], q->Ht[1], q->Ht[2], q->dHt[0], q->dHt[1], q->dHt[2]};
     for (void* b : bufs) if (b) (void)hipFree(b);
     if (q->s) (void)hipStreamDestroy(q->s);
     delete q;
@@ -634,18 +870,9 @@ int pr_mlp_set_params(pr_mlp* q, const float* in, int n) {
 // (metres), oversize probability, predicted robust score (-100 if predicted oversize).
 int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
     if (!q || n < 1) { g_mlp_err = "pr_mlp_predict: n >= 1"; return -1; }
-    for (int s0 = 0; s0 < n; s0 += PRED_ROWS) {
-        const int nb = std::min(PRED_ROWS, n - s0);
-        const int B = pad_rows(nb);
-        hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes + (size_t)s0 * NIN, nb,
-                           B, q->lo, q->inv_range, q->Xp, nullptr);
-        MCK(hipGetLastError());
-        int rc = forward_pred(q, B);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_head_scores, dim3((nb + 255) / 256), dim3(256), 0, q->s, q->outp, nb,
-                           d_pred + (size_t)s0 * 5);
-        MCK(hipGetLastError());
-    }
+    hipLaunchKernelGGL(k_mlp_predict, dim3((n + PB_ROWS - 1) / PB_ROWS), dim3(256), 0, q->s, d_genomes, n, q->wbt,
+                       q->layout, q->w, q->lo, q->inv_range, d_pred);
+    MCK(hipGetLastError());
     MCK(hipStreamSynchronize(q->s));
     return 0;
 }
