@@ -6,19 +6,19 @@
 // layers.  Heads: 0 mean distance, 1 CVaR of the worst quarter, 2 worst distance (all in
 // units of 10 m, regression, masked for unprintable designs), 3 oversize logit (BCE).
 //
-// Two GEMM kernels, both with K-contiguous operands loaded 16 bytes per lane:
-//   gemm_nt     C = A * Bt^T for the forward pass (A = activations, Bt = the layer's
-//               transposed weight copy; epilogue bias + ReLU -> BF16, or bias -> FP32) and
-//               the input gradients (A = output gradient, Bt = the weights; epilogue the ReLU
-//               mask of the layer input -> BF16); training also writes each BF16 result
-//               transposed, the layout the weight gradients read;
-//   gemm_wgrad  dW = Xin^T dY over the batch from those transposes (features x batch), eight
-//               waves per 32 x 32 tile on batch slices summed in wave order through LDS, the
-//               bias gradient (column sums of dY) from an all-ones MFMA operand alongside:
-//               training is deterministic.
-// A training step is 14 launches (embedding, 4 forward, loss, 4 weight + bias gradients,
-// 3 input gradients, Adam); prediction runs 65536 rows per pass on buffers of its own, its
-// first layer embedding the genomes on the way into LDS (gemm_lds).
+// Four kernels do the work, every MFMA operand a 16-byte load per lane (K-contiguous):
+//   k_mlp_predict  one pass per 128 rows: embedding, four layers and heads, the activations
+//                  in LDS between layers;
+//   k_mlp_fwd_bwd  a training batch's forward pass, loss, output gradient and the input
+//                  gradients back through the hidden layers, 32 rows per workgroup, the
+//                  activations (the ReLU masks) in LDS; it writes each layer input and
+//                  output gradient transposed (features x batch), the operands of
+//   k_wgrad_all    dW = Xin^T dY for all four layers in one launch: eight waves per 32 x 32
+//                  tile on batch slices summed in wave order through LDS, the bias gradient
+//                  (column sums of dY) from an all-ones MFMA operand alongside, the batch split
+//                  four ways into copies that
+//   k_adam         adds in order before its step: training is deterministic.
+// A training step is three launches (four when the loss is read).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -43,205 +43,10 @@ const int LN[NL] = {HID, HID, HID, NOP};
 __device__ __forceinline__ __bf16 to_bf(float x) { return (__bf16)x; }  // round to nearest even
 __device__ __forceinline__ float from_bf(__bf16 x) { return (float)x; }
 
-enum Epi { EPI_BIAS_RELU_BF16 = 0, EPI_BIAS_F32 = 1, EPI_MASK_BF16 = 2 };
-
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
-// Epilogue of a wave's (32 WM) x (32 WN) accumulator tiles at (m0, n0): bias + ReLU -> BF16,
-// bias -> FP32, or the ReLU mask -> BF16; BF16 results also transposed into Ct when given,
-// four rows at a time as one 8-byte store.
-template <int EPI, int WM, int WN>
-__device__ __forceinline__ void store_tiles(const f16v (&acc)[WM][WN], int m0, int n0, int l,
-                                            const float* __restrict__ bias, const __bf16* __restrict__ mask, int ldm,
-                                            void* __restrict__ C, int ldc, __bf16* __restrict__ Ct, int ldct) {
-    const int lr = l & 31;
-#pragma unroll
-    for (int i = 0; i < WM; i++)
-#pragma unroll
-        for (int j = 0; j < WN; j++)
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const int mg = m0 + 32 * i + 8 * g + 4 * (l >> 5), n = n0 + 32 * j + lr;
-                __bf16 o[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int m = mg + q;
-                    float v = acc[i][j][4 * g + q];
-                    if (EPI == EPI_BIAS_RELU_BF16) {
-                        o[q] = to_bf(fmaxf(v + bias[n], 0.0f));
-                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
-                    } else if (EPI == EPI_BIAS_F32) {
-                        reinterpret_cast<float*>(C)[(size_t)m * ldc + n] = v + bias[n];
-                    } else if (EPI == EPI_MASK_BF16) {
-                        const float keep = from_bf(mask[(size_t)m * ldm + n]) > 0.0f ? 1.0f : 0.0f;
-                        o[q] = to_bf(v * keep);
-                        reinterpret_cast<__bf16*>(C)[(size_t)m * ldc + n] = o[q];
-                    }
-                }
-                if ((EPI == EPI_BIAS_RELU_BF16 || EPI == EPI_MASK_BF16) && Ct) {
-                    bf4 t = {o[0], o[1], o[2], o[3]};
-                    *reinterpret_cast<bf4*>(Ct + (size_t)n * ldct + mg) = t;
-                }
-            }
-}
-
-// C[M x N] = A[M x K] * B[K x N] with both operands K-contiguous: A row-major (lda), and B
-// given as its transpose Bt[N x K] (row-major, ldb): B(k, n) = Bt[n * ldb + k].  The forward
-// pass (B = the layer's transposed weight copy) and the input gradient (dY * W^T: Bt = W
-// itself) have this form.  A workgroup of WGM x WGN waves computes a (32 WM WGM) x
-// (32 WN WGN) tile; each wave a (32 WM) x (32 WN) tile of v_mfma_f32_32x32x16_bf16 products,
-// its operands one 16-byte load per lane and MFMA tile (lane l: row l % 32, k = k0 + 8 (l / 32)
-// .. + 7).  M, N multiples of the workgroup tile, K a multiple of 16.  Accumulator element r
-// of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 tile.
-// BF16 epilogues optionally also write the transpose, Ct[n * ldct + m] (the operand layout
-// the weight gradients read), four rows at a time as one 8-byte store.
-template <int EPI, int WGM, int WGN, int WM, int WN>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_nt(const __bf16* __restrict__ A, int lda,
-                                                          const __bf16* __restrict__ Bt, int ldb, int K,
-                                                          const float* __restrict__ bias,
-                                                          const __bf16* __restrict__ mask, int ldm, void* __restrict__ C,
-                                                          int ldc, __bf16* __restrict__ Ct, int ldct) {
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int m0 = blockIdx.x * (32 * WM * WGM) + (w / WGN) * (32 * WM);
-    const int n0 = blockIdx.y * (32 * WN * WGN) + (w % WGN) * (32 * WN);
-    const int lr = l & 31, kq = 8 * (l >> 5);
-    f16v acc[WM][WN];
-#pragma unroll
-    for (int i = 0; i < WM; i++)
-#pragma unroll
-        for (int j = 0; j < WN; j++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
-    const __bf16* pa = A + (size_t)(m0 + lr) * lda + kq;
-    const __bf16* pb = Bt + (size_t)(n0 + lr) * ldb + kq;
-    // operands of the next D k-steps in flight (a ring of registers, refilled D steps ahead):
-    // the loop issues loads D steps before their MFMAs instead of waiting on each
-    constexpr int D = (WM + WN) <= 2 ? 8 : 4;
-    const int ns = K / 16;
-    bf8 ra[D][WM], rb[D][WN];
-#pragma unroll
-    for (int d = 0; d < D; d++)
-        if (d < ns) {
-#pragma unroll
-            for (int i = 0; i < WM; i++) ra[d][i] = *reinterpret_cast<const bf8*>(pa + (size_t)32 * i * lda + 16 * d);
-#pragma unroll
-            for (int j = 0; j < WN; j++) rb[d][j] = *reinterpret_cast<const bf8*>(pb + (size_t)32 * j * ldb + 16 * d);
-        }
-    for (int s0 = 0; s0 < ns; s0 += D) {
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            const int st = s0 + d;
-            if (st < ns) {
-                bf8 a[WM], b[WN];
-#pragma unroll
-                for (int i = 0; i < WM; i++) a[i] = ra[d][i];
-#pragma unroll
-                for (int j = 0; j < WN; j++) b[j] = rb[d][j];
-                if (st + D < ns) {
-                    const int k1 = 16 * (st + D);
-#pragma unroll
-                    for (int i = 0; i < WM; i++) ra[d][i] = *reinterpret_cast<const bf8*>(pa + (size_t)32 * i * lda + k1);
-#pragma unroll
-                    for (int j = 0; j < WN; j++) rb[d][j] = *reinterpret_cast<const bf8*>(pb + (size_t)32 * j * ldb + k1);
-                }
-#pragma unroll
-                for (int i = 0; i < WM; i++)
-#pragma unroll
-                    for (int j = 0; j < WN; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-            }
-        }
-    }
-    store_tiles<EPI, WM, WN>(acc, m0, n0, l, bias, mask, ldm, C, ldc, Ct, ldct);
-}
-
-// gemm_nt with the operands staged through LDS: a workgroup of WGM x WGN waves computes a
-// (32 WM WGM) x (32 WN WGN) tile; each K chunk of 64 of its A rows and Bt rows is loaded with
-// 16-byte vector loads covering whole 128-byte rows (each operand element read from global
-// memory once per workgroup instead of once per wave), written to one of two LDS buffers (rows
-// padded by 16 bytes), and the waves' MFMA operands are 16-byte LDS reads.  The next chunk's
-// global loads are in flight while the current one is multiplied; one barrier per chunk.
-// M, N multiples of the workgroup tile, K a multiple of 64.
-template <int EPI, int WGM, int WGN, int WM, int WN>
-__global__ __launch_bounds__(64 * WGM * WGN) void gemm_lds(const __bf16* __restrict__ A, int lda,
-                                                           const __bf16* __restrict__ Bt, int ldb, int K,
-                                                           const float* __restrict__ bias,
-                                                           const __bf16* __restrict__ mask, int ldm,
-                                                           void* __restrict__ C, int ldc, __bf16* __restrict__ Ct,
-                                                           int ldct) {
-    constexpr int NT = 64 * WGM * WGN;
-    constexpr int BM = 32 * WM * WGM, BN = 32 * WN * WGN, BK = 64, PITCH = BK + 8;
-    constexpr int VA = BM * 8 / NT, VB = BN * 8 / NT;  // staging vectors per thread and chunk
-    static_assert(VA >= 1 && VB >= 1 && VA * NT == BM * 8 && VB * NT == BN * 8, "staging tiles");
-    __shared__ __attribute__((aligned(16))) __bf16 sA[2][BM * PITCH];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[2][BN * PITCH];
-    const int t = threadIdx.x, l = t & 63, w = t >> 6;
-    const int bm0 = blockIdx.x * BM, bn0 = blockIdx.y * BN;
-    const int wm0 = (w / WGN) * (32 * WM), wn0 = (w % WGN) * (32 * WN);
-    const int lr = l & 31, kq = 8 * (l >> 5);
-    // staging: thread t moves vectors t + NT v (8 per 128-byte row: row = idx / 8, column 8 (idx % 8))
-    bf8 na[VA], nb[VB];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int v = 0; v < VA; v++) {
-            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
-            na[v] = *reinterpret_cast<const bf8*>(A + (size_t)(bm0 + r) * lda + k0 + c);
-        }
-#pragma unroll
-        for (int v = 0; v < VB; v++) {
-            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
-            nb[v] = *reinterpret_cast<const bf8*>(Bt + (size_t)(bn0 + r) * ldb + k0 + c);
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int v = 0; v < VA; v++) {
-            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
-            *reinterpret_cast<bf8*>(&sA[buf][r * PITCH + c]) = na[v];
-        }
-#pragma unroll
-        for (int v = 0; v < VB; v++) {
-            const int idx = t + NT * v, r = idx >> 3, c = 8 * (idx & 7);
-            *reinterpret_cast<bf8*>(&sB[buf][r * PITCH + c]) = nb[v];
-        }
-    };
-    f16v acc[WM][WN];
-#pragma unroll
-    for (int i = 0; i < WM; i++)
-#pragma unroll
-        for (int j = 0; j < WN; j++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
-    const int nch = K / BK;
-    load(0);
-    store(0);
-    for (int c = 0; c < nch; c++) {
-        __syncthreads();
-        if (c + 1 < nch) load((c + 1) * BK);
-        const __bf16* a_s = sA[c & 1];
-        const __bf16* b_s = sB[c & 1];
-#pragma unroll
-        for (int ks = 0; ks < BK; ks += 16) {
-            bf8 a[WM], b[WN];
-#pragma unroll
-            for (int i = 0; i < WM; i++)
-                a[i] = *reinterpret_cast<const bf8*>(&a_s[(wm0 + 32 * i + lr) * PITCH + ks + kq]);
-#pragma unroll
-            for (int j = 0; j < WN; j++)
-                b[j] = *reinterpret_cast<const bf8*>(&b_s[(wn0 + 32 * j + lr) * PITCH + ks + kq]);
-#pragma unroll
-            for (int i = 0; i < WM; i++)
-#pragma unroll
-                for (int j = 0; j < WN; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        if (c + 1 < nch) store((c + 1) & 1);
-    }
-    store_tiles<EPI, WM, WN>(acc, bm0 + wm0, bn0 + wn0, l, bias, mask, ldm, C, ldc, Ct, ldct);
-}
-
-// k-loop of one wave of gemm_wgrad over its batch slice (ns k-steps), operands D steps ahead
+// k-loop of one wave of a weight-gradient tile over its batch slice (ns k-steps), operands D steps ahead
 template <bool DOB>
 __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, int ns, f16v& acc, f16v& accb) {
     constexpr int D = 8;
@@ -274,30 +79,26 @@ __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, 
 
 // Weight (and bias) gradients: dW[M x N] (FP32) = A[M x Kd] * Bt[N x Kd]^T over the batch
 // Kd, with A = the layer input transposed (features x batch) and Bt = the output gradient
-// transposed -- both batch-contiguous, so the operands are 16-byte loads as in gemm_nt.  The
-// batch is split gridDim.z ways (so that the few 32 x 32 tiles of a layer fill the GPU); a
-// workgroup of 8 waves computes one tile of one split: wave w sums its slice, the eight
-// partial tiles are added in wave order through LDS, and split z writes C + z split_stride
-// (the Adam step adds the splits in order: deterministic, no atomics).  With BIAS, the
-// workgroups of row tile 0 also form db[n] = sum over the batch of dY[b][n]: an MFMA with an
-// all-ones A operand on the same Bt operands gives the column sums of the slice, added in
-// wave order likewise.  Kd a multiple of 128 gridDim.z.
-template <bool BIAS>
-__global__ __launch_bounds__(512) void gemm_wgrad(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, int ld,
-                                                  int Kd, float* __restrict__ C, int ldc, float* __restrict__ db,
-                                                  size_t split_stride) {
-    __shared__ float part[8][16][64];
-    __shared__ float bpart[8][32];
+// transposed -- both batch-contiguous, so the operands are 16-byte loads.  The batch is split
+// nz ways (so that the few 32 x 32 tiles of a layer fill the GPU); a workgroup of 8 waves
+// computes one tile of one split: wave w sums its slice, the eight partial tiles are added in
+// wave order through LDS, and split z writes C + z split_stride (the Adam step adds the splits
+// in order: deterministic, no atomics).  The workgroups of row tile 0 also form db[n] = sum
+// over the batch of dY[b][n]: an MFMA with an all-ones A operand on the same Bt operands gives
+// the column sums of the slice, added in wave order likewise.  Kd a multiple of 128 nz.
+__device__ __forceinline__ void wgrad_tile(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, int ld, int Kd,
+                                           float* __restrict__ C, int ldc, float* __restrict__ db,
+                                           size_t split_stride, int m0, int n0, int z, int nz, float (*part)[16][64],
+                                           float (*bpart)[32]) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
-    // batch split blockIdx.z of gridDim.z, wave w its eighth; each split writes its own copy
-    const int ks = Kd / (8 * gridDim.z), kb = (blockIdx.z * 8 + w) * ks;
-    C += blockIdx.z * split_stride;
-    db += blockIdx.z * split_stride;
+    // batch split z of nz, wave w its eighth; each split writes its own copy
+    const int ks = Kd / (8 * nz), kb = (z * 8 + w) * ks;
+    C += z * split_stride;
+    db += z * split_stride;
     const int lr = l & 31, kq = 8 * (l >> 5);
     const __bf16* pa = A + (size_t)(m0 + lr) * ld + kb + kq;
     const __bf16* pb = Bt + (size_t)(n0 + lr) * ld + kb + kq;
-    const bool dob = BIAS && blockIdx.x == 0;
+    const bool dob = m0 == 0;
     f16v acc, accb;
 #pragma unroll
     for (int r = 0; r < 16; r++) { acc[r] = 0.0f; accb[r] = 0.0f; }
@@ -323,54 +124,37 @@ __global__ __launch_bounds__(512) void gemm_wgrad(const __bf16* __restrict__ A, 
     }
 }
 
-// rows of a batch are padded to this multiple (the gemm_nt workgroup tiles' row count)
+// the operands and outputs of one layer's weight gradient
+struct WgradLayer {
+    const __bf16* A;   // layer input, transposed [k x batch]
+    const __bf16* Bt;  // output gradient, transposed [n x batch]
+    float* C;          // dW [k x n] of split 0
+    float* db;         // db [n] of split 0
+    int mt, nt;        // 32 x 32 tiles along k and n
+};
+struct WgradAll {
+    WgradLayer L[NL];  // in launch order
+    int first[NL + 1]; // first workgroup of each layer (of the nz splits), and the total
+};
+
+// every layer's weight and bias gradients in one launch: workgroup b belongs to the layer whose
+// range holds it, then decodes (split, row tile, column tile)
+__global__ __launch_bounds__(512) void k_wgrad_all(WgradAll W, int ld, int nz, size_t split_stride) {
+    __shared__ float part[8][16][64];
+    __shared__ float bpart[8][32];
+    int L = 0;
+#pragma unroll
+    for (int u = 1; u < NL; u++)
+        if ((int)blockIdx.x >= W.first[u]) L = u;
+    const WgradLayer& G = W.L[L];
+    const int b = blockIdx.x - W.first[L];
+    const int z = b % nz, tile = b / nz, mi = tile / G.nt, ni = tile - mi * G.nt;
+    wgrad_tile(G.A, G.Bt, ld, ld, G.C, 32 * G.nt, G.db, split_stride, 32 * mi, 32 * ni, z, nz, part, bpart);
+}
+
+// rows of a training batch are padded to this multiple (four training blocks: the weight
+// gradients split the batch into four 32-row-aligned parts)
 constexpr int ROW_PAD = 128;
-
-// genomes [n x 35] f32 -> X [n_pad x 64] bf16, genes scaled to [0, 1] by the bounds
-// (Xt, optional: the transpose [64 x n_pad], the weight gradient's operand layout)
-__global__ void k_embed(const float* __restrict__ g, int n, int n_pad, const float* __restrict__ lo,
-                        const float* __restrict__ inv_range, __bf16* __restrict__ X, __bf16* __restrict__ Xt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_pad * K0) return;
-    int r = i / K0, k = i - r * K0;
-    float v = 0.0f;
-    if (r < n && k < NIN) v = (g[(size_t)r * NIN + k] - lo[k]) * inv_range[k];
-    X[i] = to_bf(v);
-    if (Xt) Xt[(size_t)k * n_pad + r] = to_bf(v);
-}
-
-// loss and output gradient.  out [n_pad x 16] f32, targets [n x 4] raw (m, m, m, 0/1)
-__global__ void k_loss(const float* __restrict__ out, const float* __restrict__ tgt, int n, int n_pad, float inv_n,
-                       __bf16* __restrict__ dout, __bf16* __restrict__ doutT, float* __restrict__ loss_part) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // every lane stays for the wave reduction
-    float l = 0.0f;
-    float d[NOUT] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (r < n) {
-        const float* o = out + (size_t)r * NOP;
-        const float* t = tgt + (size_t)r * NOUT;
-        const float over = t[3];
-        if (over < 0.5f) {
-            for (int k = 0; k < 3; k++) {
-                const float e = o[k] - t[k] * 0.1f;
-                l += e * e;
-                d[k] = 2.0f * e * inv_n;
-            }
-        }
-        const float z = o[3];
-        const float p = 1.0f / (1.0f + expf(-z));
-        l += fmaxf(z, 0.0f) - z * over + log1pf(expf(-fabsf(z)));
-        d[3] = (p - over) * inv_n;
-    }
-    if (r < n_pad) {
-        for (int k = 0; k < NOP; k++) {
-            const float dk = k < NOUT ? d[k] : 0.0f;
-            dout[(size_t)r * NOP + k] = to_bf(dk);
-            doutT[(size_t)k * n_pad + r] = to_bf(dk);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = l;  // one 64-row block per partial
-}
 
 // mean loss = the partials summed in a fixed order (lane-strided, then a butterfly), times 1/n
 __global__ void k_loss_fin(const float* __restrict__ loss_part, int nblk, float inv_n, float* __restrict__ loss) {
@@ -389,28 +173,40 @@ __global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int cou
     out[i] = s;
 }
 
-// Adam step on the FP32 master parameters, writing their BF16 copy too. Padding parameters
-// (W1 rows >= 35, W4 / b4 columns >= 4) take no step, so they stay zero.
-// the parameter block's geometry: weights of layer L at off_w[L] ([k x n] row-major), and
-// the transposed BF16 copy's element for parameter i (-1 for biases)
+// The parameter block's geometry: the weights of layer L at off_w[L], [k x n] row-major.
+// The kernels read the weights as MFMA A operands from two BF16 copies in fragment order: a
+// fragment is one k-step of one 32-row tile, the 64 lanes' 16-byte operands back to back
+// (lane l: row 32 t + l % 32, reduction index 16 s + 8 (l / 32) .. + 7), so that every weight
+// load of a wave is 1 KB contiguous.  The forward copy's rows are the layer's outputs n
+// (reduction over k: the transposed weights), the backward copy's rows its inputs k
+// (reduction over n: the input gradient dY W^T); fragment (t, s) of layer L sits at
+// off_w[L] + (t * steps + s) * FRAG, steps = the reduction length / 16.
 struct ParamLayout {
     int off_w[4], k[4], n[4];
 };
-__device__ __forceinline__ int transposed_index(const ParamLayout& P, int i) {
+constexpr int FRAG = 512;  // bf16 elements per fragment
+__device__ __forceinline__ int frag_offset(int row, int red, int steps) {
+    return ((row >> 5) * steps + (red >> 4)) * FRAG + ((row & 31) + 32 * ((red >> 3) & 1)) * 8 + (red & 7);
+}
+// positions of parameter i in the forward and backward copies (-1: a bias)
+__device__ __forceinline__ void frag_index(const ParamLayout& P, int i, int& fwd, int& bwd) {
+    fwd = bwd = -1;
 #pragma unroll
     for (int L = 0; L < 4; L++) {
         const int t = i - P.off_w[L];
         if (t >= 0 && t < P.k[L] * P.n[L]) {
             const int kk = t / P.n[L], nn = t - kk * P.n[L];
-            return P.off_w[L] + nn * P.k[L] + kk;
+            fwd = P.off_w[L] + frag_offset(nn, kk, P.k[L] >> 4);
+            bwd = P.off_w[L] + frag_offset(kk, nn, P.n[L] >> 4);
         }
     }
-    return -1;
 }
 
+// Adam step on the FP32 master parameters, writing their BF16 fragment copies too. Padding
+// parameters (W1 rows >= 35, W4 / b4 columns >= 4) take no step, so they stay zero.
 // g: nsplit gradient copies n apart, added in split order
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, int nsplit, float* __restrict__ m,
-                       float* __restrict__ v, __bf16* __restrict__ wb, __bf16* __restrict__ wbt, ParamLayout P, int n,
+                       float* __restrict__ v, __bf16* __restrict__ wf, __bf16* __restrict__ wbk, ParamLayout P, int n,
                        int off_w1, int off_w4, int off_b4, float lr, float b1, float b2, float eps, float c1, float c2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -426,19 +222,19 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, int n
     v[i] = vi;
     const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
     w[i] = wi;
-    wb[i] = to_bf(wi);
-    const int it = transposed_index(P, i);
-    if (it >= 0) wbt[it] = to_bf(wi);
+    int f, b;
+    frag_index(P, i, f, b);
+    if (f >= 0) { wf[f] = to_bf(wi); wbk[b] = to_bf(wi); }
 }
 
-// BF16 copies of the weights: row-major and transposed
-__global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict__ y, __bf16* __restrict__ yt,
+// the BF16 fragment copies of the weights
+__global__ void k_to_bf16(const float* __restrict__ x, int n, __bf16* __restrict__ wf, __bf16* __restrict__ wbk,
                           ParamLayout P) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    y[i] = to_bf(x[i]);
-    const int it = transposed_index(P, i);
-    if (it >= 0) yt[it] = to_bf(x[i]);
+    int f, b;
+    frag_index(P, i, f, b);
+    if (f >= 0) { wf[f] = to_bf(x[i]); wbk[b] = to_bf(x[i]); }
 }
 
 // the five prediction values of one row from its four head outputs: mean, CVaR and worst
@@ -477,10 +273,15 @@ constexpr int PD = 4;                  // weight k-steps in flight
 constexpr int PB_B4 = 3 * HID, PB_LO = PB_B4 + NOUT, PB_IR = PB_LO + NIN, PB_CONST = PB_IR + NIN;
 
 // one wave's k-loop of a layer: acc[i][j] += (W^T tile i) x (activation tile j)^T over KIN;
-// ring slot s % PD holds k-step s of this layer on entry and k-step s of the next layer on exit
+// ring slot s % PD holds k-step s of this layer on entry and k-step s of the next layer on exit.
+// The activation operands of step s + 1 are read from LDS during step s.  A scheduling barrier
+// closes every step: without it the compiler sinks the ring's loads next to their MFMAs and
+// every step waits out an L2 round trip.
+// pw, pw_next: the lane's operand in the layer's first fragment of this wave's first tile;
+// tiles tstride apart, k-steps FRAG apart.
 template <int KIN, int NA, int NB, int NA_NEXT>
 __device__ __forceinline__ void fused_layer_k(f16v (&acc)[NA][NB], bf8 (&ring)[PD][2], const __bf16* __restrict__ pw,
-                                              int ldw, const __bf16* __restrict__ pw_next, int ldw_next,
+                                              int tstride, const __bf16* __restrict__ pw_next, int tstride_next,
                                               const __bf16* __restrict__ act, int pitch, int j0, int lr, int kq) {
     constexpr int NS = KIN / 16;
     static_assert(NS % PD == 0, "k-steps a multiple of the ring depth");
@@ -490,6 +291,10 @@ __device__ __forceinline__ void fused_layer_k(f16v (&acc)[NA][NB], bf8 (&ring)[P
         for (int j = 0; j < NB; j++)
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+    const __bf16* pa = act + (32 * j0 + lr) * pitch + kq;
+    bf8 b[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) b[j] = *reinterpret_cast<const bf8*>(pa + 32 * j * pitch);
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const int d = s % PD;
@@ -499,21 +304,27 @@ __device__ __forceinline__ void fused_layer_k(f16v (&acc)[NA][NB], bf8 (&ring)[P
         if (s + PD < NS) {
 #pragma unroll
             for (int i = 0; i < NA; i++)
-                ring[d][i] = *reinterpret_cast<const bf8*>(pw + (size_t)32 * i * ldw + 16 * (s + PD));
+                ring[d][i] = *reinterpret_cast<const bf8*>(pw + i * tstride + FRAG * (s + PD));
         } else if (NA_NEXT > 0) {
 #pragma unroll
             for (int i = 0; i < NA_NEXT; i++)
-                ring[d][i] = *reinterpret_cast<const bf8*>(pw_next + (size_t)32 * i * ldw_next + 16 * (s + PD - NS));
+                ring[d][i] = *reinterpret_cast<const bf8*>(pw_next + i * tstride_next + FRAG * (s + PD - NS));
         }
-        bf8 b[NB];
+        bf8 bn[NB];
+        if (s + 1 < NS) {
 #pragma unroll
-        for (int j = 0; j < NB; j++)
-            b[j] = *reinterpret_cast<const bf8*>(act + (32 * (j0 + j) + lr) * pitch + 16 * s + kq);
+            for (int j = 0; j < NB; j++) bn[j] = *reinterpret_cast<const bf8*>(pa + 32 * j * pitch + 16 * (s + 1));
+        }
 #pragma unroll
         for (int i = 0; i < NA; i++)
 #pragma unroll
             for (int j = 0; j < NB; j++)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (s + 1 < NS) {
+#pragma unroll
+            for (int j = 0; j < NB; j++) b[j] = bn[j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -538,7 +349,7 @@ __device__ __forceinline__ void fused_relu_to_lds(const f16v (&acc)[2][4], const
 }
 
 __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict__ genomes, int n,
-                                                        const __bf16* __restrict__ wbt, ParamLayout P,
+                                                        const __bf16* __restrict__ wf, ParamLayout P,
                                                         const float* __restrict__ w, const float* __restrict__ lo,
                                                         const float* __restrict__ inv_range,
                                                         float* __restrict__ pred) {
@@ -547,15 +358,17 @@ __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict_
     const int t = threadIdx.x, l = t & 63, wv = t >> 6, lr = l & 31, kq = 8 * (l >> 5);
     const int m0 = blockIdx.x * PB_ROWS;
     const int n0 = 64 * wv;  // this wave's features in the hidden layers
-    const __bf16* pw0 = wbt + P.off_w[0] + (size_t)(n0 + lr) * K0 + kq;
-    const __bf16* pw1 = wbt + P.off_w[1] + (size_t)(n0 + lr) * HID + kq;
-    const __bf16* pw2 = wbt + P.off_w[2] + (size_t)(n0 + lr) * HID + kq;
-    const __bf16* pw3 = wbt + P.off_w[3] + (size_t)lr * HID + kq;
+    // this wave's first fragment of each layer (forward copy; tiles of 32 outputs)
+    constexpr int T0 = K0 / 16 * FRAG, TH = HID / 16 * FRAG;  // tile strides
+    const __bf16* pw0 = wf + P.off_w[0] + (n0 >> 5) * T0 + 8 * l;
+    const __bf16* pw1 = wf + P.off_w[1] + (n0 >> 5) * TH + 8 * l;
+    const __bf16* pw2 = wf + P.off_w[2] + (n0 >> 5) * TH + 8 * l;
+    const __bf16* pw3 = wf + P.off_w[3] + 8 * l;
     bf8 ring[PD][2];
 #pragma unroll
     for (int d = 0; d < PD; d++)
 #pragma unroll
-        for (int i = 0; i < 2; i++) ring[d][i] = *reinterpret_cast<const bf8*>(pw0 + (size_t)32 * i * K0 + 16 * d);
+        for (int i = 0; i < 2; i++) ring[d][i] = *reinterpret_cast<const bf8*>(pw0 + i * T0 + FRAG * d);
 
     // the block's genomes: 128 x 35 consecutive floats, coalesced, all loads issued at once
     constexpr int NE = (PB_ROWS * NIN + 255) / 256;
@@ -593,20 +406,20 @@ __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict_
     __syncthreads();
 
     f16v acc[2][4];
-    fused_layer_k<K0, 2, 4, 2>(acc, ring, pw0, K0, pw1, HID, h, PB_XP, 0, lr, kq);
+    fused_layer_k<K0, 2, 4, 2>(acc, ring, pw0, T0, pw1, TH, h, PB_XP, 0, lr, kq);
     __syncthreads();  // every wave has read the inputs: overwrite them
     fused_relu_to_lds(acc, cst, n0, h, l);
     __syncthreads();
-    fused_layer_k<HID, 2, 4, 2>(acc, ring, pw1, HID, pw2, HID, h, PB_HP, 0, lr, kq);
+    fused_layer_k<HID, 2, 4, 2>(acc, ring, pw1, TH, pw2, TH, h, PB_HP, 0, lr, kq);
     __syncthreads();
     fused_relu_to_lds(acc, cst + HID, n0, h, l);
     __syncthreads();
-    fused_layer_k<HID, 2, 4, 1>(acc, ring, pw2, HID, pw3, HID, h, PB_HP, 0, lr, kq);
+    fused_layer_k<HID, 2, 4, 1>(acc, ring, pw2, TH, pw3, TH, h, PB_HP, 0, lr, kq);
     __syncthreads();
     fused_relu_to_lds(acc, cst + 2 * HID, n0, h, l);
     __syncthreads();
     f16v acc3[1][1];
-    fused_layer_k<HID, 1, 1, 0>(acc3, ring, pw3, HID, nullptr, 0, h, PB_HP, wv, lr, kq);
+    fused_layer_k<HID, 1, 1, 0>(acc3, ring, pw3, TH, nullptr, 0, h, PB_HP, wv, lr, kq);
     // lanes 0..31 hold the four live outputs of row 32 wv + lane in accumulator elements 0..3
     const int m = m0 + 32 * wv + l;
     if (l < 32 && m < n) {
@@ -618,6 +431,224 @@ __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict_
 #pragma unroll
         for (int q = 0; q < 5; q++) pred[(size_t)m * 5 + q] = pr[q];
     }
+}
+
+
+// ---- a training step's forward pass, loss and input gradients in one pass: k_mlp_fwd_bwd ----
+// A workgroup of four waves takes 32 rows of the batch: embedding, the four layers, the loss
+// and its output gradient, and the input gradients back through layers 4, 3 and 2 (ReLU
+// masks from the activations, which stay in LDS), with the layers computed transposed as in
+// k_mlp_predict (wave w: features 64 w .. 64 w + 63; the A operand the transposed weight copy
+// in the forward pass, the row-major weights in the backward pass).  What leaves the block is
+// what the weight gradients read: the layer inputs and output gradients transposed
+// (features x batch), X^T, H1^T..H3^T, dY4^T, dH3^T..dH1^T, plus one loss partial.
+constexpr int TB_ROWS = 32;      // rows per workgroup
+constexpr int TB_DP = NOP + 8;   // LDS row pitch of the output gradient
+
+// the transposed global copies a block writes (each [features x B], B the padded batch)
+struct TrainOut {
+    __bf16* xt;      // [64 x B]
+    __bf16* ht[3];   // [256 x B] hidden activations
+    __bf16* dht[3];  // [256 x B] their gradients
+    __bf16* doutt;   // [32 x B] output gradient
+};
+
+// epilogue of a 32-row block's transposed tiles (features n0 .. n0 + 63): MASK false:
+// ReLU(acc + bias); MASK true: acc where the activation in `mask` (LDS rows) is positive, else
+// 0.  The BF16 result goes to the LDS rows of `out` (when given) and, transposed, to the global
+// copy gt [features x ldt] at column m0.
+template <bool MASK>
+__device__ __forceinline__ void train_epilogue(const f16v (&acc)[2][1], const float* __restrict__ bias,
+                                               const __bf16* __restrict__ mask, int n0, __bf16* __restrict__ out,
+                                               __bf16* __restrict__ gt, int ldt, int m0, int l) {
+    const int m = l & 31;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int n = n0 + 32 * i + 8 * g + 4 * (l >> 5);
+            bf4 o;
+            if (MASK) {
+                const bf4 mk = *reinterpret_cast<const bf4*>(mask + m * PB_HP + n);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float keep = from_bf(mk[q]) > 0.0f ? 1.0f : 0.0f;
+                    o[q] = to_bf(acc[i][0][4 * g + q] * keep);
+                }
+            } else {
+                const f4 bv = *reinterpret_cast<const f4*>(bias + n);
+#pragma unroll
+                for (int q = 0; q < 4; q++) o[q] = to_bf(fmaxf(acc[i][0][4 * g + q] + bv[q], 0.0f));
+            }
+            if (out) *reinterpret_cast<bf4*>(out + m * PB_HP + n) = o;
+#pragma unroll
+            for (int q = 0; q < 4; q++) gt[(size_t)(n + q) * ldt + m0 + m] = o[q];
+        }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ genomes,
+                                                     const float* __restrict__ tgt, int n, int B, float inv_n,
+                                                     const __bf16* __restrict__ wf, const __bf16* __restrict__ wbk,
+                                                     ParamLayout P, const float* __restrict__ w,
+                                                     const float* __restrict__ lo,
+                                                     const float* __restrict__ inv_range, TrainOut T,
+                                                     float* __restrict__ loss_part) {
+    __shared__ __attribute__((aligned(16))) __bf16 xs[TB_ROWS * PB_XP];     // embedded inputs
+    __shared__ __attribute__((aligned(16))) __bf16 hs[3][TB_ROWS * PB_HP];  // H1..H3 (post-ReLU)
+    __shared__ __attribute__((aligned(16))) __bf16 ds[TB_ROWS * PB_HP];     // dH3, then dH2 (in place)
+    __shared__ __attribute__((aligned(16))) __bf16 dos[TB_ROWS * TB_DP];    // output gradient
+    __shared__ __attribute__((aligned(16))) float cst[PB_CONST];
+    __shared__ float red[4][NOUT][32];  // the output layer's four k-quarter partial sums
+    const int t = threadIdx.x, l = t & 63, wv = t >> 6, lr = l & 31, kq = 8 * (l >> 5);
+    const int m0 = blockIdx.x * TB_ROWS, n0 = 64 * wv;
+    // this wave's first fragment of each layer: forward copy (tiles of outputs), backward copy
+    // (tiles of inputs: W3 and W2 rows)
+    constexpr int T0 = K0 / 16 * FRAG, TH = HID / 16 * FRAG, T4 = NOP / 16 * FRAG;  // tile strides
+    const __bf16* pf0 = wf + P.off_w[0] + (n0 >> 5) * T0 + 8 * l;
+    const __bf16* pf1 = wf + P.off_w[1] + (n0 >> 5) * TH + 8 * l;
+    const __bf16* pf2 = wf + P.off_w[2] + (n0 >> 5) * TH + 8 * l;
+    const __bf16* pb2 = wbk + P.off_w[2] + (n0 >> 5) * TH + 8 * l;
+    const __bf16* pb1 = wbk + P.off_w[1] + (n0 >> 5) * TH + 8 * l;
+    bf8 ring[PD][2];
+#pragma unroll
+    for (int d = 0; d < PD; d++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) ring[d][i] = *reinterpret_cast<const bf8*>(pf0 + i * T0 + FRAG * d);
+
+    // genomes of the block (element e: row e % 32, gene e / 32), targets of wave 0's rows
+    constexpr int NE = TB_ROWS * K0 / 256;
+    float gv[NE];
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = t + 256 * i, m = e & 31, k = e >> 5;
+        gv[i] = (k < NIN && m0 + m < n) ? genomes[(size_t)(m0 + m) * NIN + k] : 0.0f;
+    }
+    float tg[NOUT] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (t < 32 && m0 + t < n) {
+#pragma unroll
+        for (int q = 0; q < NOUT; q++) tg[q] = tgt[(size_t)(m0 + t) * NOUT + q];
+    }
+    for (int i = t; i < PB_CONST; i += 256) {
+        float v;
+        if (i < PB_B4) v = w[P.off_w[i >> 8] + P.k[i >> 8] * P.n[i >> 8] + (i & (HID - 1))];
+        else if (i < PB_LO) v = w[P.off_w[3] + P.k[3] * P.n[3] + (i - PB_B4)];
+        else if (i < PB_IR) v = lo[i - PB_LO];
+        else v = inv_range[i - PB_IR];
+        cst[i] = v;
+    }
+    __syncthreads();
+    // embedding (genes scaled to [0, 1] by the bounds, zero beyond gene 35 and row n) -> LDS
+    // rows and X^T
+#pragma unroll
+    for (int i = 0; i < NE; i++) {
+        const int e = t + 256 * i, m = e & 31, k = e >> 5;
+        const __bf16 x = to_bf((k < NIN && m0 + m < n) ? (gv[i] - cst[PB_LO + k]) * cst[PB_IR + k] : 0.0f);
+        xs[m * PB_XP + k] = x;
+        T.xt[(size_t)k * B + m0 + m] = x;
+    }
+    __syncthreads();
+
+    f16v acc[2][1];
+    fused_layer_k<K0, 2, 1, 2>(acc, ring, pf0, T0, pf1, TH, xs, PB_XP, 0, lr, kq);
+    train_epilogue<false>(acc, cst, nullptr, n0, hs[0], T.ht[0], B, m0, l);
+    __syncthreads();
+    fused_layer_k<HID, 2, 1, 2>(acc, ring, pf1, TH, pf2, TH, hs[0], PB_HP, 0, lr, kq);
+    train_epilogue<false>(acc, cst + HID, nullptr, n0, hs[1], T.ht[1], B, m0, l);
+    __syncthreads();
+    // the ring continues with the backward pass through layer 3 (W3 rows); layer 4 forward and
+    // backward load their few operands directly
+    fused_layer_k<HID, 2, 1, 2>(acc, ring, pf2, TH, pb2, TH, hs[1], PB_HP, 0, lr, kq);
+    bf8 a4[4];  // output layer, this wave's k-quarter 64 wv .. 64 wv + 63 (W4^T rows 0..31)
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++)
+        a4[s4] = *reinterpret_cast<const bf8*>(wf + P.off_w[3] + (4 * wv + s4) * FRAG + 8 * l);
+    bf8 w4[2][2];  // backward through the output layer: W4 rows n0 + 32 i + lane (its 32 outputs)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int s4 = 0; s4 < 2; s4++)
+            w4[i][s4] = *reinterpret_cast<const bf8*>(wbk + P.off_w[3] + ((n0 >> 5) + i) * T4 + s4 * FRAG + 8 * l);
+    train_epilogue<false>(acc, cst + 2 * HID, nullptr, n0, hs[2], T.ht[2], B, m0, l);
+    __syncthreads();
+
+    // output layer: out^T = W4^T H3^T, the k range split over the four waves, summed in wave order
+    f16v a3;
+#pragma unroll
+    for (int r = 0; r < 16; r++) a3[r] = 0.0f;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+        const bf8 b = *reinterpret_cast<const bf8*>(hs[2] + lr * PB_HP + 64 * wv + 16 * s4 + kq);
+        a3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a4[s4], b, a3, 0, 0, 0);
+    }
+    if (l < 32) {
+#pragma unroll
+        for (int q = 0; q < NOUT; q++) red[wv][q][l] = a3[q];  // features 0..3 of row l
+    }
+    __syncthreads();
+    if (wv == 0) {
+        // loss and output gradient of row m0 + l: squared error of the three distance heads
+        // (targets in metres, heads in tens of metres; skipped for oversize designs) and the
+        // logistic loss of the oversize head, each gradient over the batch mean
+        float lossv = 0.0f;
+        float d[NOUT] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (l < 32 && m0 + l < n) {
+            float o[NOUT];
+#pragma unroll
+            for (int q = 0; q < NOUT; q++)
+                o[q] = ((red[0][q][l] + red[1][q][l]) + red[2][q][l]) + red[3][q][l] + cst[PB_B4 + q];
+            const float over = tg[3];
+            if (over < 0.5f) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const float e = o[k] - tg[k] * 0.1f;
+                    lossv += e * e;
+                    d[k] = 2.0f * e * inv_n;
+                }
+            }
+            const float z = o[3];
+            const float p = 1.0f / (1.0f + expf(-z));
+            lossv += fmaxf(z, 0.0f) - z * over + log1pf(expf(-fabsf(z)));
+            d[3] = (p - over) * inv_n;
+        }
+        if (l < 32) {
+            bf8 row0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) row0[k] = to_bf(k < NOUT ? d[k] : 0.0f);
+            bf8 zero;
+#pragma unroll
+            for (int k = 0; k < 8; k++) zero[k] = to_bf(0.0f);
+            *reinterpret_cast<bf8*>(dos + l * TB_DP) = row0;
+#pragma unroll
+            for (int c = 1; c < NOP / 8; c++) *reinterpret_cast<bf8*>(dos + l * TB_DP + 8 * c) = zero;
+#pragma unroll
+            for (int k = 0; k < NOP; k++) T.doutt[(size_t)k * B + m0 + l] = k < 8 ? row0[k] : zero[0];
+        }
+        for (int o = 32; o > 0; o >>= 1) lossv += __shfl_xor(lossv, o);
+        if (l == 0) loss_part[blockIdx.x] = lossv;
+    }
+    __syncthreads();
+
+    // dH3 = (dY4 W4^T) masked by H3 > 0: K = the output layer's 32 features
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[i][0][r] = 0.0f;
+#pragma unroll
+    for (int s4 = 0; s4 < 2; s4++) {
+        const bf8 b = *reinterpret_cast<const bf8*>(dos + lr * TB_DP + 16 * s4 + kq);
+#pragma unroll
+        for (int i = 0; i < 2; i++) acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w4[i][s4], b, acc[i][0], 0, 0, 0);
+    }
+    train_epilogue<true>(acc, nullptr, hs[2], n0, ds, T.dht[2], B, m0, l);
+    __syncthreads();
+    // dH2 = (dH3 W3^T) masked by H2 > 0, written over dH3 once every wave has read it
+    fused_layer_k<HID, 2, 1, 2>(acc, ring, pb2, TH, pb1, TH, ds, PB_HP, 0, lr, kq);
+    __syncthreads();
+    train_epilogue<true>(acc, nullptr, hs[1], n0, ds, T.dht[1], B, m0, l);
+    __syncthreads();
+    // dH1 = (dH2 W2^T) masked by H1 > 0 (the last input gradient the weight gradients need)
+    fused_layer_k<HID, 2, 1, 0>(acc, ring, pb1, TH, nullptr, 0, ds, PB_HP, 0, lr, kq);
+    train_epilogue<true>(acc, nullptr, hs[0], n0, nullptr, T.dht[0], B, m0, l);
 }
 
 
@@ -636,22 +667,19 @@ struct pr_mlp {
     int nsplit = 1;          // splits the last backward pass wrote
     float* m = nullptr;   // Adam first moment
     float* v = nullptr;   // Adam second moment
-    __bf16* wb = nullptr; // BF16 copy of the weights (biases stay FP32)
-    __bf16* wbt = nullptr;  // BF16 copy of each layer's weights transposed ([n x k]), for gemm_nt
+    // BF16 copies of the weights in MFMA fragment order (biases stay FP32): the forward pass's
+    // A operands (rows = outputs) and the input gradients' (rows = inputs)
+    __bf16* wf = nullptr;
+    __bf16* wbk = nullptr;
     ParamLayout layout = {};
-    __bf16* X = nullptr;  // [B x 64]
-    __bf16* H[3] = {};    // [B x 256] post-ReLU activations
-    __bf16* dH[3] = {};   // [B x 256] gradients
-    // transposes (features x B) of the layer inputs and output gradients: the operands of the
-    // weight gradients
-    __bf16* Xt = nullptr;
-    __bf16* Ht[3] = {};
-    __bf16* dHt[3] = {};
+    // transposes (features x B) of the layer inputs and output gradients of a training batch:
+    // the operands of the weight gradients
+    __bf16* Xt = nullptr;     // [64 x B]
+    __bf16* Ht[3] = {};       // [256 x B] post-ReLU activations
+    __bf16* dHt[3] = {};      // [256 x B] their gradients
     __bf16* doutT = nullptr;  // [32 x B]
-    float* out = nullptr; // [B x 32]
-    __bf16* dout = nullptr;
     float* loss = nullptr;
-    float* loss_part = nullptr;  // [max_batch / 64] per-block loss sums
+    float* loss_part = nullptr;  // per-32-row-block loss sums
     float* lo = nullptr;
     float* inv_range = nullptr;
     int step = 0;
@@ -674,96 +702,50 @@ static float init_u(unsigned seed, size_t i) {
     return (float)(x >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
 }
 
-// C[M x N] = A[M x K] * (Bt[N x K])^T through gemm_nt, with the workgroup tile chosen for the
-// size: 128 x 128 for large batches (M >= 32768), 64 x 64 otherwise (a 4096-row batch then
-// spreads over 256 workgroups), 64 x 32 for the 32-wide output layer.
-template <int EPI>
-static void launch_nt(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* Bt, int ldb,
-                      const float* bias, const __bf16* mask, int ldm, void* C, int ldc, __bf16* Ct, int ldct) {
-    if (N % 128 == 0 && M >= 32768 && M % 128 == 0 && K % 64 == 0)
-        hipLaunchKernelGGL((gemm_lds<EPI, 2, 2, 2, 2>), dim3(M / 128, N / 128), dim3(256), 0, st, A, lda, Bt, ldb, K,
-                           bias, mask, ldm, C, ldc, Ct, ldct);
-    else if (N % 64 == 0 && M % 64 == 0 && K % 64 == 0)
-        hipLaunchKernelGGL((gemm_lds<EPI, 2, 2, 1, 1>), dim3(M / 64, N / 64), dim3(256), 0, st, A, lda, Bt, ldb, K,
-                           bias, mask, ldm, C, ldc, Ct, ldct);
-    else if (N % 64 == 0)
-        hipLaunchKernelGGL((gemm_nt<EPI, 2, 2, 1, 1>), dim3(M / 64, N / 64), dim3(256), 0, st, A, lda, Bt, ldb, K, bias,
-                           mask, ldm, C, ldc, Ct, ldct);
-    else
-        hipLaunchKernelGGL((gemm_nt<EPI, 2, 1, 1, 1>), dim3(M / 64, N / 32), dim3(128), 0, st, A, lda, Bt, ldb, K, bias,
-                           mask, ldm, C, ldc, Ct, ldct);
-}
-
-// One layer's forward GEMM: [B x K] x W[K x N] (its transposed copy) + bias (ReLU, BF16 for the
-// hidden layers, FP32 for the output layer); Ht: the activations' transpose as well (training)
-static void launch_layer(pr_mlp* q, int L, const __bf16* in, int B, void* out, __bf16* Ht) {
-    const __bf16* Wt = q->wbt + q->off_w[L];
-    const float* b = q->w + q->off_b[L];
-    if (L < NL - 1)
-        launch_nt<EPI_BIAS_RELU_BF16>(q->s, B, LN[L], LK[L], in, LK[L], Wt, LK[L], b, nullptr, 0, out, LN[L], Ht, B);
-    else
-        launch_nt<EPI_BIAS_F32>(q->s, B, LN[L], LK[L], in, LK[L], Wt, LK[L], b, nullptr, 0, out, NOP, nullptr, 0);
-}
-
-static int forward(pr_mlp* q, int B) {
-    // layer 1..3: H = ReLU(X W + b) (BF16, and its transpose); layer 4: out = H3 W4 + b4 (FP32)
-    const __bf16* in = q->X;
-    for (int L = 0; L < NL; L++) {
-        launch_layer(q, L, in, B, L < NL - 1 ? (void*)q->H[L] : (void*)q->out, L < NL - 1 ? q->Ht[L] : nullptr);
-        MCK(hipGetLastError());
-        if (L < NL - 1) in = q->H[L];
-    }
-    return 0;
-}
-
-// forward + loss + backward into q->gpart (FP32 gradients of the mean loss over n samples, one
-// copy per batch split of the weight gradients: q->nsplit);
-// want_loss: also reduce the mean loss into q->loss (the caller reads it)
+// One training batch: the fused forward / loss / input-gradient pass, the mean loss (when
+// wanted), and every layer's weight and bias gradients in one launch (nsplit batch-split copies
+// in gpart, which the Adam step or k_sum_splits add)
 static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, bool want_loss) {
     const int B = pad_rows(n);
-    hipLaunchKernelGGL(k_embed, dim3((B * K0 + 255) / 256), dim3(256), 0, q->s, d_genomes, n, B, q->lo, q->inv_range,
-                       q->X, q->Xt);
-    MCK(hipGetLastError());
-    int rc = forward(q, B);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_loss, dim3((B + 63) / 64), dim3(64), 0, q->s, q->out, d_targets, n, B, 1.0f / n, q->dout,
-                       q->doutT, q->loss_part);
+    TrainOut T;
+    T.xt = q->Xt;
+    for (int k = 0; k < 3; k++) { T.ht[k] = q->Ht[k]; T.dht[k] = q->dHt[k]; }
+    T.doutt = q->doutT;
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(B / TB_ROWS), dim3(256), 0, q->s, d_genomes, d_targets, n, B, 1.0f / n,
+                       q->wf, q->wbk, q->layout, q->w, q->lo, q->inv_range, T, q->loss_part);
     MCK(hipGetLastError());
     if (want_loss) {
-        hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, q->s, q->loss_part, (B + 63) / 64, 1.0f / n, q->loss);
+        hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(64), 0, q->s, q->loss_part, B / TB_ROWS, 1.0f / n, q->loss);
         MCK(hipGetLastError());
     }
-    // backward, layer 4 .. 1; every gradient entry is written (weights and biases by gemm_wgrad)
-    const __bf16* dY = q->dout;
-    const __bf16* dYt = q->doutT;
-    int ldy = NOP;
-    for (int L = NL - 1; L >= 0; L--) {
-        const __bf16* Xt = L == 0 ? q->Xt : q->Ht[L - 1];
-        const int K = LK[L], N = LN[L];
-        // dW [K x N] = Xin^T dY and db = column sums of dY, over the (zero-padded) batch
-        const int z = B % (128 * WG_SPLITS) == 0 ? WG_SPLITS : 1;
-        hipLaunchKernelGGL(gemm_wgrad<true>, dim3(K / 32, N / 32, z), dim3(512), 0, q->s, Xt, dYt, B, B,
-                           q->gpart + q->off_w[L], N, q->gpart + q->off_b[L], q->n_params);
-        q->nsplit = z;
-        MCK(hipGetLastError());
-        if (L > 0) {
-            // dXin [B x K] = dY [B x N] W^T, masked by ReLU'(Xin) (Xin > 0); W [K x N] row-major is
-            // the transposed operand gemm_nt takes; the transpose feeds the next weight gradient
-            const __bf16* W = q->wb + q->off_w[L];
-            launch_nt<EPI_MASK_BF16>(q->s, B, K, N, dY, ldy, W, N, nullptr, q->H[L - 1], K, (void*)q->dH[L - 1], K,
-                                     q->dHt[L - 1], B);
-            MCK(hipGetLastError());
-            dY = q->dH[L - 1];
-            dYt = q->dHt[L - 1];
-            ldy = K;
-        }
+    // dW [K x N] = Xin^T dY and db = column sums of dY over the (zero-padded) batch; the two
+    // 256 x 256 layers first
+    const int z = B % (128 * WG_SPLITS) == 0 ? WG_SPLITS : 1;
+    WgradAll W;
+    const int order[NL] = {1, 2, 0, 3};
+    int first = 0;
+    for (int u = 0; u < NL; u++) {
+        const int L = order[u];
+        WgradLayer& G = W.L[u];
+        G.A = L == 0 ? q->Xt : q->Ht[L - 1];
+        G.Bt = L == NL - 1 ? q->doutT : q->dHt[L];
+        G.C = q->gpart + q->off_w[L];
+        G.db = q->gpart + q->off_b[L];
+        G.mt = LK[L] / 32;
+        G.nt = LN[L] / 32;
+        W.first[u] = first;
+        first += G.mt * G.nt * z;
     }
+    W.first[NL] = first;
+    hipLaunchKernelGGL(k_wgrad_all, dim3(first), dim3(512), 0, q->s, W, B, z, q->n_params);
+    MCK(hipGetLastError());
+    q->nsplit = z;
     return 0;
 }
 
 static int refresh_bf16(pr_mlp* q) {
     const int n = (int)q->n_params;
-    hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, q->s, q->w, n, q->wb, q->wbt, q->layout);
+    hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, q->s, q->w, n, q->wf, q->wbk, q->layout);
     MCK(hipGetLastError());
     return 0;
 }
@@ -796,19 +778,13 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
     MCK(hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
     MALLOC(q->w, 4 * o); MALLOC(q->g, 4 * o); MALLOC(q->m, 4 * o); MALLOC(q->v, 4 * o);
     MALLOC(q->gpart, 4 * o * WG_SPLITS);
-    MALLOC(q->wb, 2 * o);
-    MALLOC(q->wbt, 2 * o);
-    MALLOC(q->X, 2 * (size_t)B * K0);
+    MALLOC(q->wf, 2 * o);
+    MALLOC(q->wbk, 2 * o);
     MALLOC(q->Xt, 2 * (size_t)B * K0);
-    for (int k = 0; k < 3; k++) {
-        MALLOC(q->H[k], 2 * (size_t)B * HID); MALLOC(q->dH[k], 2 * (size_t)B * HID);
-        MALLOC(q->Ht[k], 2 * (size_t)B * HID); MALLOC(q->dHt[k], 2 * (size_t)B * HID);
-    }
-    MALLOC(q->out, 4 * (size_t)B * NOP);
-    MALLOC(q->dout, 2 * (size_t)B * NOP);
+    for (int k = 0; k < 3; k++) { MALLOC(q->Ht[k], 2 * (size_t)B * HID); MALLOC(q->dHt[k], 2 * (size_t)B * HID); }
     MALLOC(q->doutT, 2 * (size_t)B * NOP);
     MALLOC(q->loss, 4);
-    MALLOC(q->loss_part, 4 * (size_t)(B / 64 + 1));  // B is a multiple of 128
+    MALLOC(q->loss_part, 4 * (size_t)(B / TB_ROWS));  // B is a multiple of 128
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
 #undef MALLOC
@@ -840,9 +816,9 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
 void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
-    void* bufs[] = {q->w, q->g, q->gpart, q->m, q->v, q->wb, q->wbt, q->X, q->Xt, q->out, q->dout, q->doutT, q->loss,
-                    q->loss_part, q->lo, q->inv_range, q->H[0], q->H[1], q->H[2], q->dH[0], q->dH[1], q->dH[2],
-                    q->Ht[0], q->Ht[1], q->Ht[2], q->dHt[0], q->dHt[1], q->dHt[2]};
+    void* bufs[] = {q->w,      q->g,      q->gpart,  q->m,      q->v,      q->wf,     q->wbk,
+                    q->Xt,     q->doutT,  q->loss,   q->loss_part, q->lo,  q->inv_range,
+                    q->Ht[0],  q->Ht[1],  q->Ht[2],  q->dHt[0], q->dHt[1], q->dHt[2]};
     for (void* b : bufs) if (b) (void)hipFree(b);
     if (q->s) (void)hipStreamDestroy(q->s);
     delete q;
@@ -870,7 +846,7 @@ int pr_mlp_set_params(pr_mlp* q, const float* in, int n) {
 // (metres), oversize probability, predicted robust score (-100 if predicted oversize).
 int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
     if (!q || n < 1) { g_mlp_err = "pr_mlp_predict: n >= 1"; return -1; }
-    hipLaunchKernelGGL(k_mlp_predict, dim3((n + PB_ROWS - 1) / PB_ROWS), dim3(256), 0, q->s, d_genomes, n, q->wbt,
+    hipLaunchKernelGGL(k_mlp_predict, dim3((n + PB_ROWS - 1) / PB_ROWS), dim3(256), 0, q->s, d_genomes, n, q->wf,
                        q->layout, q->w, q->lo, q->inv_range, d_pred);
     MCK(hipGetLastError());
     MCK(hipStreamSynchronize(q->s));
@@ -916,7 +892,7 @@ int pr_mlp_train(pr_mlp* q, const float* d_genomes, const float* d_targets, int 
             const float c2 = 1.0f / (1.0f - std::pow(b2, (float)q->step));
             hipLaunchKernelGGL(k_adam, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->w, q->gpart,
                                q->nsplit, q->m,
-                               q->v, q->wb, q->wbt, q->layout, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3],
+                               q->v, q->wf, q->wbk, q->layout, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3],
                                (int)q->off_b[3], lr, b1, b2, eps, c1, c2);
             MCK(hipGetLastError());
         }
