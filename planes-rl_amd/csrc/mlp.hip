@@ -6,19 +6,20 @@
 // layers.  Heads: 0 mean distance, 1 CVaR of the worst quarter, 2 worst distance (all in
 // units of 10 m, regression, masked for unprintable designs), 3 oversize logit (BCE).
 //
-// Four kernels do the work, every MFMA operand a 16-byte load per lane (K-contiguous):
+// Three kernels do the work, every MFMA operand a 16-byte load per lane:
 //   k_mlp_predict  one pass per 128 rows: embedding, four layers and heads, the activations
 //                  in LDS between layers;
 //   k_mlp_fwd_bwd  a training batch's forward pass, loss, output gradient and the input
 //                  gradients back through the hidden layers, 32 rows per workgroup, the
-//                  activations (the ReLU masks) in LDS; it writes each layer input and
-//                  output gradient transposed (features x batch), the operands of
-//   k_wgrad_all    dW = Xin^T dY for all four layers in one launch: eight waves per 32 x 32
-//                  tile on batch slices summed in wave order through LDS, the bias gradient
-//                  (column sums of dY) from an all-ones MFMA operand alongside, the batch split
-//                  four ways into copies that
-//   k_adam         adds in order before its step: training is deterministic.
-// A training step is three launches (four when the loss is read).
+//                  activations (the ReLU masks) in LDS; it writes each layer input and output
+//                  gradient transposed (features x batch, fragment order), the operands of
+//   k_wgrad_all    dW = Xin^T dY for all four layers in one launch, one 32 x 32 tile per
+//                  workgroup over the whole batch (16 waves on batch slices summed in wave
+//                  order through LDS, the bias gradient from an all-ones MFMA operand
+//                  alongside), each tile's parameters taking their Adam step in the same
+//                  workgroup: training is deterministic.
+// Weights are read from BF16 copies in MFMA fragment order (1 KB contiguous per wave load).
+// A training step is two launches (three when the loss is read).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -46,6 +47,15 @@ __device__ __forceinline__ float from_bf(__bf16 x) { return (float)x; }
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 
+// MFMA fragment order of a [rows x red] BF16 operand (reduction index red): a fragment is one
+// k-step of one 32-row tile, the 64 lanes' 16-byte operands back to back (lane l: row
+// 32 t + l % 32, reduction index 16 s + 8 (l / 32) .. + 7), so that a wave's operand load is
+// 1 KB contiguous; fragment (t, s) at (t * steps + s) * FRAG, steps = the reduction length / 16.
+constexpr int FRAG = 512;  // bf16 elements per fragment
+__device__ __forceinline__ int frag_offset(int row, int red, int steps) {
+    return ((row >> 5) * steps + (red >> 4)) * FRAG + ((row & 31) + 32 * ((red >> 3) & 1)) * 8 + (red & 7);
+}
+
 // k-loop of one wave of a weight-gradient tile over its batch slice (ns k-steps), operands D steps ahead
 template <bool DOB>
 __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, int ns, f16v& acc, f16v& accb) {
@@ -57,8 +67,8 @@ __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, 
 #pragma unroll
     for (int d = 0; d < D; d++)
         if (d < ns) {
-            ra[d] = *reinterpret_cast<const bf8*>(pa + 16 * d);
-            rb[d] = *reinterpret_cast<const bf8*>(pb + 16 * d);
+            ra[d] = *reinterpret_cast<const bf8*>(pa + FRAG * d);
+            rb[d] = *reinterpret_cast<const bf8*>(pb + FRAG * d);
         }
     for (int s0 = 0; s0 < ns; s0 += D) {
 #pragma unroll
@@ -67,8 +77,8 @@ __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, 
             if (st < ns) {
                 const bf8 a = ra[d], b = rb[d];
                 if (st + D < ns) {
-                    ra[d] = *reinterpret_cast<const bf8*>(pa + 16 * (st + D));
-                    rb[d] = *reinterpret_cast<const bf8*>(pb + 16 * (st + D));
+                    ra[d] = *reinterpret_cast<const bf8*>(pa + FRAG * (st + D));
+                    rb[d] = *reinterpret_cast<const bf8*>(pb + FRAG * (st + D));
                 }
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
                 if (DOB) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ones, b, accb, 0, 0, 0);
@@ -77,27 +87,23 @@ __device__ __forceinline__ void wgrad_slice(const __bf16* pa, const __bf16* pb, 
     }
 }
 
-// Weight (and bias) gradients: dW[M x N] (FP32) = A[M x Kd] * Bt[N x Kd]^T over the batch
-// Kd, with A = the layer input transposed (features x batch) and Bt = the output gradient
-// transposed -- both batch-contiguous, so the operands are 16-byte loads.  The batch is split
-// nz ways (so that the few 32 x 32 tiles of a layer fill the GPU); a workgroup of 8 waves
-// computes one tile of one split: wave w sums its slice, the eight partial tiles are added in
-// wave order through LDS, and split z writes C + z split_stride (the Adam step adds the splits
-// in order: deterministic, no atomics).  The workgroups of row tile 0 also form db[n] = sum
-// over the batch of dY[b][n]: an MFMA with an all-ones A operand on the same Bt operands gives
-// the column sums of the slice, added in wave order likewise.  Kd a multiple of 128 nz.
-__device__ __forceinline__ void wgrad_tile(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, int ld, int Kd,
-                                           float* __restrict__ C, int ldc, float* __restrict__ db,
-                                           size_t split_stride, int m0, int n0, int z, int nz, float (*part)[16][64],
-                                           float (*bpart)[32]) {
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // batch split z of nz, wave w its eighth; each split writes its own copy
-    const int ks = Kd / (8 * nz), kb = (z * 8 + w) * ks;
-    C += z * split_stride;
-    db += z * split_stride;
-    const int lr = l & 31, kq = 8 * (l >> 5);
-    const __bf16* pa = A + (size_t)(m0 + lr) * ld + kb + kq;
-    const __bf16* pb = Bt + (size_t)(n0 + lr) * ld + kb + kq;
+// Weight (and bias) gradients of one 32 x 32 tile: dW[m][n] = sum over the batch of
+// A[m][b] Bt[n][b], with A = the layer input transposed (features x batch) and Bt = the output
+// gradient transposed -- both in fragment order over the batch (steps = B / 16), so every
+// operand load of a wave is 1 KB contiguous.  A workgroup of WG_WAVES waves: wave w sums its
+// share of the batch, the partial tiles are added in wave order through LDS (deterministic, no
+// atomics).  The workgroups of row tile 0 also form db[n] = sum over the batch of dY[b][n]: an
+// MFMA with an all-ones A operand on the same Bt operands gives the column sums of the slice,
+// added in wave order likewise.  emit(m, n, value) receives every element (m = -1: the bias
+// of column n).  B a multiple of 16 WG_WAVES.
+constexpr int WG_WAVES = 16;
+template <class Emit>
+__device__ __forceinline__ void wgrad_tile(const __bf16* __restrict__ A, const __bf16* __restrict__ Bt, int B, int m0,
+                                           int n0, float (*part)[16][64], float (*bpart)[32], Emit emit) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6, lr = l & 31;
+    const int ks = B / WG_WAVES, kb = w * ks;
+    const __bf16* pa = A + ((size_t)(m0 >> 5) * (B >> 4) + (kb >> 4)) * FRAG + 8 * l;
+    const __bf16* pb = Bt + ((size_t)(n0 >> 5) * (B >> 4) + (kb >> 4)) * FRAG + 8 * l;
     const bool dob = m0 == 0;
     f16v acc, accb;
 #pragma unroll
@@ -108,53 +114,25 @@ __device__ __forceinline__ void wgrad_tile(const __bf16* __restrict__ A, const _
     for (int r = 0; r < 16; r++) part[w][r][l] = acc[r];
     if (dob && l < 32) bpart[w][l] = accb[0];  // row 0 of ones * B: the column sums
     __syncthreads();
-#pragma unroll
-    for (int rr = 2 * w; rr < 2 * w + 2; rr++) {
+    static_assert(WG_WAVES == 16, "one accumulator row per wave in the reduction");
+    {
+        const int rr = w;
         float v = part[0][rr][l];
 #pragma unroll
-        for (int u = 1; u < 8; u++) v += part[u][rr][l];
-        const int m = m0 + 8 * (rr >> 2) + 4 * (l >> 5) + (rr & 3), n = n0 + lr;
-        C[(size_t)m * ldc + n] = v;
+        for (int u = 1; u < WG_WAVES; u++) v += part[u][rr][l];
+        emit(m0 + 8 * (rr >> 2) + 4 * (l >> 5) + (rr & 3), n0 + lr, v);
     }
     if (dob && w == 0 && l < 32) {
         float v = bpart[0][l];
 #pragma unroll
-        for (int u = 1; u < 8; u++) v += bpart[u][l];
-        db[n0 + l] = v;
+        for (int u = 1; u < WG_WAVES; u++) v += bpart[u][l];
+        emit(-1, n0 + l, v);
     }
 }
 
-// the operands and outputs of one layer's weight gradient
-struct WgradLayer {
-    const __bf16* A;   // layer input, transposed [k x batch]
-    const __bf16* Bt;  // output gradient, transposed [n x batch]
-    float* C;          // dW [k x n] of split 0
-    float* db;         // db [n] of split 0
-    int mt, nt;        // 32 x 32 tiles along k and n
-};
-struct WgradAll {
-    WgradLayer L[NL];  // in launch order
-    int first[NL + 1]; // first workgroup of each layer (of the nz splits), and the total
-};
-
-// every layer's weight and bias gradients in one launch: workgroup b belongs to the layer whose
-// range holds it, then decodes (split, row tile, column tile)
-__global__ __launch_bounds__(512) void k_wgrad_all(WgradAll W, int ld, int nz, size_t split_stride) {
-    __shared__ float part[8][16][64];
-    __shared__ float bpart[8][32];
-    int L = 0;
-#pragma unroll
-    for (int u = 1; u < NL; u++)
-        if ((int)blockIdx.x >= W.first[u]) L = u;
-    const WgradLayer& G = W.L[L];
-    const int b = blockIdx.x - W.first[L];
-    const int z = b % nz, tile = b / nz, mi = tile / G.nt, ni = tile - mi * G.nt;
-    wgrad_tile(G.A, G.Bt, ld, ld, G.C, 32 * G.nt, G.db, split_stride, 32 * mi, 32 * ni, z, nz, part, bpart);
-}
-
-// rows of a training batch are padded to this multiple (four training blocks: the weight
-// gradients split the batch into four 32-row-aligned parts)
-constexpr int ROW_PAD = 128;
+// rows of a training batch are padded to this multiple: the weight gradients give each of
+// their 16 waves a batch slice of whole 16-row k-steps
+constexpr int ROW_PAD = 256;
 
 // mean loss = the partials summed in a fixed order (lane-strided, then a butterfly), times 1/n
 __global__ void k_loss_fin(const float* __restrict__ loss_part, int nblk, float inv_n, float* __restrict__ loss) {
@@ -162,15 +140,6 @@ __global__ void k_loss_fin(const float* __restrict__ loss_part, int nblk, float 
     for (int b = threadIdx.x; b < nblk; b += 64) l += loss_part[b];
     for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
     if (threadIdx.x == 0) *loss = l * inv_n;
-}
-
-// out[i] = the nsplit copies of part (count apart) added in split order
-__global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int count, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    float s = part[i];
-    for (int z = 1; z < nsplit; z++) s += part[(size_t)z * count + i];
-    out[i] = s;
 }
 
 // The parameter block's geometry: the weights of layer L at off_w[L], [k x n] row-major.
@@ -184,10 +153,6 @@ __global__ void k_sum_splits(const float* __restrict__ part, int nsplit, int cou
 struct ParamLayout {
     int off_w[4], k[4], n[4];
 };
-constexpr int FRAG = 512;  // bf16 elements per fragment
-__device__ __forceinline__ int frag_offset(int row, int red, int steps) {
-    return ((row >> 5) * steps + (red >> 4)) * FRAG + ((row & 31) + 32 * ((red >> 3) & 1)) * 8 + (red & 7);
-}
 // positions of parameter i in the forward and backward copies (-1: a bias)
 __device__ __forceinline__ void frag_index(const ParamLayout& P, int i, int& fwd, int& bwd) {
     fwd = bwd = -1;
@@ -202,29 +167,71 @@ __device__ __forceinline__ void frag_index(const ParamLayout& P, int i, int& fwd
     }
 }
 
-// Adam step on the FP32 master parameters, writing their BF16 fragment copies too. Padding
-// parameters (W1 rows >= 35, W4 / b4 columns >= 4) take no step, so they stay zero.
-// g: nsplit gradient copies n apart, added in split order
-__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, int nsplit, float* __restrict__ m,
-                       float* __restrict__ v, __bf16* __restrict__ wf, __bf16* __restrict__ wbk, ParamLayout P, int n,
-                       int off_w1, int off_w4, int off_b4, float lr, float b1, float b2, float eps, float c1, float c2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Adam step of parameter i with gradient gi on the FP32 master copy and moments, writing its
+// BF16 fragment copies too.  Padding parameters (W1 rows >= 35, W4 / b4 columns >= 4) take no
+// step, so they stay zero.
+struct AdamArgs {
+    float lr, b1, b2, eps, c1, c2;  // c1, c2: the bias corrections 1 / (1 - b^step)
+};
+__device__ __forceinline__ void adam_param(int i, float gi, float* __restrict__ w, float* __restrict__ m,
+                                           float* __restrict__ v, __bf16* __restrict__ wf, __bf16* __restrict__ wbk,
+                                           const ParamLayout& P, const AdamArgs& A) {
+    const int off_w1 = P.off_w[0], off_w4 = P.off_w[3], off_b4 = P.off_w[3] + P.k[3] * P.n[3];
     const bool pad = (i >= off_w1 + NIN * HID && i < off_w1 + K0 * HID) ||
                      (i >= off_w4 && i < off_w4 + HID * NOP && (i - off_w4) % NOP >= NOUT) ||
                      (i >= off_b4 && i < off_b4 + NOP && i - off_b4 >= NOUT);
     if (pad) return;
-    float gi = g[i];
-    for (int z = 1; z < nsplit; z++) gi += g[(size_t)z * n + i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    const float mi = A.b1 * m[i] + (1.0f - A.b1) * gi;
+    const float vi = A.b2 * v[i] + (1.0f - A.b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    const float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    const float wi = w[i] - A.lr * (mi * A.c1) / (sqrtf(vi * A.c2) + A.eps);
     w[i] = wi;
     int f, b;
     frag_index(P, i, f, b);
     if (f >= 0) { wf[f] = to_bf(wi); wbk[b] = to_bf(wi); }
+}
+
+// the operands and parameter block of one layer's weight gradient
+struct WgradLayer {
+    const __bf16* A;   // layer input, transposed [k x batch] (fragment order)
+    const __bf16* Bt;  // output gradient, transposed [n x batch] (fragment order)
+    int off_w, off_b;  // the layer's weights [k x n] and biases [n] in the parameter block
+    int mt, nt;        // 32 x 32 tiles along k and n
+};
+struct WgradAll {
+    WgradLayer L[NL];  // in launch order
+    int first[NL + 1]; // first workgroup of each layer (multiples of 8), and the total
+    int gm[NL], gn[NL];  // the gm x gn block of tiles each XCD takes in each layer
+};
+
+// every layer's weight and bias gradients in one launch: workgroup b belongs to the layer whose
+// range holds it and computes one 32 x 32 tile over the whole batch; ADAM: the tile's
+// parameters take their Adam step right there, else the gradient goes to g
+template <bool ADAM>
+__global__ __launch_bounds__(64 * WG_WAVES) void k_wgrad_all(WgradAll W, int B, float* __restrict__ g, float* __restrict__ w,
+                                                   float* __restrict__ m, float* __restrict__ v,
+                                                   __bf16* __restrict__ wf, __bf16* __restrict__ wbk, ParamLayout P,
+                                                   AdamArgs A) {
+    __shared__ float part[WG_WAVES][16][64];
+    __shared__ float bpart[WG_WAVES][32];
+    int L = 0;
+#pragma unroll
+    for (int u = 1; u < NL; u++)
+        if ((int)blockIdx.x >= W.first[u]) L = u;
+    const WgradLayer& G = W.L[L];
+    // workgroups are dispatched to the eight XCDs round-robin: workgroup b of the layer runs on
+    // XCD b % 8, which takes a gm x gn block of tiles, so that the operand rows the tiles share
+    // are read into that XCD's L2 once
+    const int b = blockIdx.x - W.first[L], x = b & 7, j = b >> 3;
+    const int gm = W.gm[L], gn = W.gn[L], gx = x % (G.mt / gm), gy = x / (G.mt / gm);
+    const int mi = gx * gm + j % gm, ni = gy * gn + j / gm;
+    const int ldc = 32 * G.nt;
+    wgrad_tile(G.A, G.Bt, B, 32 * mi, 32 * ni, part, bpart, [&](int r, int c, float gv) {
+        const int i = r < 0 ? G.off_b + c : G.off_w + r * ldc + c;
+        if (ADAM) adam_param(i, gv, w, m, v, wf, wbk, P, A);
+        else g[i] = gv;
+    });
 }
 
 // the BF16 fragment copies of the weights
@@ -380,6 +387,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict_
         const int e = t + 256 * i;
         gv[i] = e < live ? gb[e] : 0.0f;
     }
+#pragma unroll
     for (int i = t; i < PB_CONST; i += 256) {
         float v;
         if (i < PB_B4) v = w[P.off_w[i >> 8] + P.k[i >> 8] * P.n[i >> 8] + (i & (HID - 1))];
@@ -445,7 +453,8 @@ __global__ __launch_bounds__(256, 2) void k_mlp_predict(const float* __restrict_
 constexpr int TB_ROWS = 32;      // rows per workgroup
 constexpr int TB_DP = NOP + 8;   // LDS row pitch of the output gradient
 
-// the transposed global copies a block writes (each [features x B], B the padded batch)
+// the transposed global copies a block writes (each [features x B], B the padded batch), in
+// fragment order over the batch: element (f, m) at frag_offset(f, m, B / 16)
 struct TrainOut {
     __bf16* xt;      // [64 x B]
     __bf16* ht[3];   // [256 x B] hidden activations
@@ -455,12 +464,11 @@ struct TrainOut {
 
 // epilogue of a 32-row block's transposed tiles (features n0 .. n0 + 63): MASK false:
 // ReLU(acc + bias); MASK true: acc where the activation in `mask` (LDS rows) is positive, else
-// 0.  The BF16 result goes to the LDS rows of `out` (when given) and, transposed, to the global
-// copy gt [features x ldt] at column m0.
+// 0; the BF16 result to the LDS rows of `out`
 template <bool MASK>
 __device__ __forceinline__ void train_epilogue(const f16v (&acc)[2][1], const float* __restrict__ bias,
                                                const __bf16* __restrict__ mask, int n0, __bf16* __restrict__ out,
-                                               __bf16* __restrict__ gt, int ldt, int m0, int l) {
+                                               int l) {
     const int m = l & 31;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -480,10 +488,25 @@ __device__ __forceinline__ void train_epilogue(const f16v (&acc)[2][1], const fl
 #pragma unroll
                 for (int q = 0; q < 4; q++) o[q] = to_bf(fmaxf(acc[i][0][4 * g + q] + bv[q], 0.0f));
             }
-            if (out) *reinterpret_cast<bf4*>(out + m * PB_HP + n) = o;
-#pragma unroll
-            for (int q = 0; q < 4; q++) gt[(size_t)(n + q) * ldt + m0 + m] = o[q];
+            *reinterpret_cast<bf4*>(out + m * PB_HP + n) = o;
         }
+}
+
+// a block's 32 LDS rows src (features 0 .. F - 1, row pitch `pitch`) -> the global copy g
+// [F x B] in fragment order over the batch, rows m0 .. m0 + 31: each thread moves 16-byte
+// chunks (8 batch rows of one feature); consecutive threads take consecutive features, so
+// the LDS reads are conflict-free and a wave's stores are two 512-byte runs
+template <int F>
+__device__ __forceinline__ void copy_out(const __bf16* __restrict__ src, int pitch, __bf16* __restrict__ g, int B,
+                                         int m0, int t) {
+#pragma unroll
+    for (int c = t; c < 4 * F; c += 256) {
+        const int f = c % F, r8 = c / F;
+        bf8 v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = src[(8 * r8 + j) * pitch + f];
+        *reinterpret_cast<bf8*>(g + frag_offset(f, m0 + 8 * r8, B >> 4)) = v;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ genomes,
@@ -528,6 +551,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ g
 #pragma unroll
         for (int q = 0; q < NOUT; q++) tg[q] = tgt[(size_t)(m0 + t) * NOUT + q];
     }
+#pragma unroll
     for (int i = t; i < PB_CONST; i += 256) {
         float v;
         if (i < PB_B4) v = w[P.off_w[i >> 8] + P.k[i >> 8] * P.n[i >> 8] + (i & (HID - 1))];
@@ -538,23 +562,24 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ g
     }
     __syncthreads();
     // embedding (genes scaled to [0, 1] by the bounds, zero beyond gene 35 and row n) -> LDS
-    // rows and X^T
+    // rows, and X^T
 #pragma unroll
     for (int i = 0; i < NE; i++) {
         const int e = t + 256 * i, m = e & 31, k = e >> 5;
-        const __bf16 x = to_bf((k < NIN && m0 + m < n) ? (gv[i] - cst[PB_LO + k]) * cst[PB_IR + k] : 0.0f);
-        xs[m * PB_XP + k] = x;
-        T.xt[(size_t)k * B + m0 + m] = x;
+        xs[m * PB_XP + k] = to_bf((k < NIN && m0 + m < n) ? (gv[i] - cst[PB_LO + k]) * cst[PB_IR + k] : 0.0f);
     }
     __syncthreads();
+    copy_out<K0>(xs, PB_XP, T.xt, B, m0, t);
 
     f16v acc[2][1];
     fused_layer_k<K0, 2, 1, 2>(acc, ring, pf0, T0, pf1, TH, xs, PB_XP, 0, lr, kq);
-    train_epilogue<false>(acc, cst, nullptr, n0, hs[0], T.ht[0], B, m0, l);
+    train_epilogue<false>(acc, cst, nullptr, n0, hs[0], l);
     __syncthreads();
+    copy_out<HID>(hs[0], PB_HP, T.ht[0], B, m0, t);
     fused_layer_k<HID, 2, 1, 2>(acc, ring, pf1, TH, pf2, TH, hs[0], PB_HP, 0, lr, kq);
-    train_epilogue<false>(acc, cst + HID, nullptr, n0, hs[1], T.ht[1], B, m0, l);
+    train_epilogue<false>(acc, cst + HID, nullptr, n0, hs[1], l);
     __syncthreads();
+    copy_out<HID>(hs[1], PB_HP, T.ht[1], B, m0, t);
     // the ring continues with the backward pass through layer 3 (W3 rows); layer 4 forward and
     // backward load their few operands directly
     fused_layer_k<HID, 2, 1, 2>(acc, ring, pf2, TH, pb2, TH, hs[1], PB_HP, 0, lr, kq);
@@ -568,8 +593,9 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ g
 #pragma unroll
         for (int s4 = 0; s4 < 2; s4++)
             w4[i][s4] = *reinterpret_cast<const bf8*>(wbk + P.off_w[3] + ((n0 >> 5) + i) * T4 + s4 * FRAG + 8 * l);
-    train_epilogue<false>(acc, cst + 2 * HID, nullptr, n0, hs[2], T.ht[2], B, m0, l);
+    train_epilogue<false>(acc, cst + 2 * HID, nullptr, n0, hs[2], l);
     __syncthreads();
+    copy_out<HID>(hs[2], PB_HP, T.ht[2], B, m0, t);
 
     // output layer: out^T = W4^T H3^T, the k range split over the four waves, summed in wave order
     f16v a3;
@@ -620,13 +646,12 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ g
             *reinterpret_cast<bf8*>(dos + l * TB_DP) = row0;
 #pragma unroll
             for (int c = 1; c < NOP / 8; c++) *reinterpret_cast<bf8*>(dos + l * TB_DP + 8 * c) = zero;
-#pragma unroll
-            for (int k = 0; k < NOP; k++) T.doutt[(size_t)k * B + m0 + l] = k < 8 ? row0[k] : zero[0];
         }
         for (int o = 32; o > 0; o >>= 1) lossv += __shfl_xor(lossv, o);
         if (l == 0) loss_part[blockIdx.x] = lossv;
     }
     __syncthreads();
+    copy_out<NOP>(dos, TB_DP, T.doutt, B, m0, t);
 
     // dH3 = (dY4 W4^T) masked by H3 > 0: K = the output layer's 32 features
 #pragma unroll
@@ -639,16 +664,21 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float* __restrict__ g
 #pragma unroll
         for (int i = 0; i < 2; i++) acc[i][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w4[i][s4], b, acc[i][0], 0, 0, 0);
     }
-    train_epilogue<true>(acc, nullptr, hs[2], n0, ds, T.dht[2], B, m0, l);
+    train_epilogue<true>(acc, nullptr, hs[2], n0, ds, l);
     __syncthreads();
+    copy_out<HID>(ds, PB_HP, T.dht[2], B, m0, t);
     // dH2 = (dH3 W3^T) masked by H2 > 0, written over dH3 once every wave has read it
     fused_layer_k<HID, 2, 1, 2>(acc, ring, pb2, TH, pb1, TH, ds, PB_HP, 0, lr, kq);
     __syncthreads();
-    train_epilogue<true>(acc, nullptr, hs[1], n0, ds, T.dht[1], B, m0, l);
+    train_epilogue<true>(acc, nullptr, hs[1], n0, ds, l);
     __syncthreads();
-    // dH1 = (dH2 W2^T) masked by H1 > 0 (the last input gradient the weight gradients need)
+    copy_out<HID>(ds, PB_HP, T.dht[1], B, m0, t);
+    // dH1 = (dH2 W2^T) masked by H1 > 0 (the last input gradient the weight gradients need),
+    // staged in the rows of H3, which no step reads any more
     fused_layer_k<HID, 2, 1, 0>(acc, ring, pb1, TH, nullptr, 0, ds, PB_HP, 0, lr, kq);
-    train_epilogue<true>(acc, nullptr, hs[0], n0, nullptr, T.dht[0], B, m0, l);
+    train_epilogue<true>(acc, nullptr, hs[0], n0, hs[2], l);
+    __syncthreads();
+    copy_out<HID>(hs[2], PB_HP, T.dht[0], B, m0, t);
 }
 
 
@@ -662,9 +692,7 @@ struct pr_mlp {
     int max_batch = 0;
     size_t off_w[NL], off_b[NL], n_params = 0;
     float* w = nullptr;   // master FP32 parameters
-    float* g = nullptr;   // gradients (the splits summed; for pr_mlp_loss_grad)
-    float* gpart = nullptr;  // [WG_SPLITS x n_params] gradient copies of the batch splits
-    int nsplit = 1;          // splits the last backward pass wrote
+    float* g = nullptr;   // gradients (pr_mlp_loss_grad)
     float* m = nullptr;   // Adam first moment
     float* v = nullptr;   // Adam second moment
     // BF16 copies of the weights in MFMA fragment order (biases stay FP32): the forward pass's
@@ -693,7 +721,6 @@ struct pr_mlp {
     } while (0)
 
 static int pad_rows(int n) { return (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD; }
-constexpr int WG_SPLITS = 4;  // batch splits of the weight gradients (their copies summed by the Adam step)
 
 // deterministic He-uniform initialisation from a 32-bit hash of (seed, index)
 static float init_u(unsigned seed, size_t i) {
@@ -703,9 +730,10 @@ static float init_u(unsigned seed, size_t i) {
 }
 
 // One training batch: the fused forward / loss / input-gradient pass, the mean loss (when
-// wanted), and every layer's weight and bias gradients in one launch (nsplit batch-split copies
-// in gpart, which the Adam step or k_sum_splits add)
-static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, bool want_loss) {
+// wanted), and every layer's weight and bias gradients in one launch, which either takes the
+// Adam step (adam given) or leaves the gradient in q->g
+static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, bool want_loss,
+                     const AdamArgs* adam) {
     const int B = pad_rows(n);
     TrainOut T;
     T.xt = q->Xt;
@@ -720,7 +748,6 @@ static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, 
     }
     // dW [K x N] = Xin^T dY and db = column sums of dY over the (zero-padded) batch; the two
     // 256 x 256 layers first
-    const int z = B % (128 * WG_SPLITS) == 0 ? WG_SPLITS : 1;
     WgradAll W;
     const int order[NL] = {1, 2, 0, 3};
     int first = 0;
@@ -729,17 +756,24 @@ static int loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, 
         WgradLayer& G = W.L[u];
         G.A = L == 0 ? q->Xt : q->Ht[L - 1];
         G.Bt = L == NL - 1 ? q->doutT : q->dHt[L];
-        G.C = q->gpart + q->off_w[L];
-        G.db = q->gpart + q->off_b[L];
+        G.off_w = (int)q->off_w[L];
+        G.off_b = (int)q->off_b[L];
         G.mt = LK[L] / 32;
         G.nt = LN[L] / 32;
+        // per-XCD blocks: 8 x 8 tiles -> 2 x 4, 2 x 8 -> 1 x 2, 8 x 1 -> 1 x 1
+        W.gm[u] = G.mt * G.nt == 64 ? 2 : 1;
+        W.gn[u] = G.mt * G.nt / 8 / W.gm[u];
         W.first[u] = first;
-        first += G.mt * G.nt * z;
+        first += G.mt * G.nt;
     }
     W.first[NL] = first;
-    hipLaunchKernelGGL(k_wgrad_all, dim3(first), dim3(512), 0, q->s, W, B, z, q->n_params);
+    if (adam)
+        hipLaunchKernelGGL(k_wgrad_all<true>, dim3(first), dim3(64 * WG_WAVES), 0, q->s, W, B, q->g, q->w, q->m, q->v, q->wf,
+                           q->wbk, q->layout, *adam);
+    else
+        hipLaunchKernelGGL(k_wgrad_all<false>, dim3(first), dim3(64 * WG_WAVES), 0, q->s, W, B, q->g, q->w, q->m, q->v, q->wf,
+                           q->wbk, q->layout, AdamArgs{});
     MCK(hipGetLastError());
-    q->nsplit = z;
     return 0;
 }
 
@@ -777,14 +811,13 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
 #define MALLOC(p, bytes) MCK(hipMalloc((void**)&(p), (bytes)))
     MCK(hipStreamCreateWithFlags(&q->s, hipStreamNonBlocking));
     MALLOC(q->w, 4 * o); MALLOC(q->g, 4 * o); MALLOC(q->m, 4 * o); MALLOC(q->v, 4 * o);
-    MALLOC(q->gpart, 4 * o * WG_SPLITS);
     MALLOC(q->wf, 2 * o);
     MALLOC(q->wbk, 2 * o);
     MALLOC(q->Xt, 2 * (size_t)B * K0);
     for (int k = 0; k < 3; k++) { MALLOC(q->Ht[k], 2 * (size_t)B * HID); MALLOC(q->dHt[k], 2 * (size_t)B * HID); }
     MALLOC(q->doutT, 2 * (size_t)B * NOP);
     MALLOC(q->loss, 4);
-    MALLOC(q->loss_part, 4 * (size_t)(B / TB_ROWS));  // B is a multiple of 128
+    MALLOC(q->loss_part, 4 * (size_t)(B / TB_ROWS));
     MALLOC(q->lo, 4 * NIN);
     MALLOC(q->inv_range, 4 * NIN);
 #undef MALLOC
@@ -816,7 +849,7 @@ int pr_mlp_create(pr_mlp** out, int max_batch, unsigned seed) {
 void pr_mlp_destroy(pr_mlp* q) {
     if (!q) return;
     if (q->s) (void)hipStreamSynchronize(q->s);
-    void* bufs[] = {q->w,      q->g,      q->gpart,  q->m,      q->v,      q->wf,     q->wbk,
+    void* bufs[] = {q->w,      q->g,      q->m,      q->v,      q->wf,     q->wbk,
                     q->Xt,     q->doutT,  q->loss,   q->loss_part, q->lo,  q->inv_range,
                     q->Ht[0],  q->Ht[1],  q->Ht[2],  q->dHt[0], q->dHt[1], q->dHt[2]};
     for (void* b : bufs) if (b) (void)hipFree(b);
@@ -857,16 +890,13 @@ int pr_mlp_predict(pr_mlp* q, const float* d_genomes, int n, float* d_pred) {
 // parameter layout of pr_mlp_get_params.
 int pr_mlp_loss_grad(pr_mlp* q, const float* d_genomes, const float* d_targets, int n, float* loss, float* grads) {
     if (!q || n < 1 || n > q->max_batch) { g_mlp_err = "pr_mlp_loss_grad: 1 <= n <= max_batch"; return -1; }
-    int rc = loss_grad(q, d_genomes, d_targets, n, true);
+    int rc = loss_grad(q, d_genomes, d_targets, n, true, nullptr);
     if (rc) return rc;
     if (loss) {
         MCK(hipMemcpyAsync(loss, q->loss, sizeof(float), hipMemcpyDeviceToHost, q->s));
         MCK(hipStreamSynchronize(q->s));
     }
     if (grads) {
-        hipLaunchKernelGGL(k_sum_splits, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->gpart,
-                           q->nsplit, (int)q->n_params, q->g);
-        MCK(hipGetLastError());
         MCK(hipStreamSynchronize(q->s));
         MCK(hipMemcpy(grads, q->g, 4 * q->n_params, hipMemcpyDeviceToHost));
     }
@@ -885,16 +915,11 @@ int pr_mlp_train(pr_mlp* q, const float* d_genomes, const float* d_targets, int 
         for (int s0 = 0; s0 + 32 <= n; s0 += batch) {
             const int nb = std::min(batch, n - s0) / 32 * 32;
             const bool last = e == epochs - 1 && s0 + batch + 32 > n;  // the loss the caller reads
-            int rc = loss_grad(q, d_genomes + (size_t)s0 * NIN, d_targets + (size_t)s0 * NOUT, nb, last);
-            if (rc) return rc;
             q->step++;
-            const float c1 = 1.0f / (1.0f - std::pow(b1, (float)q->step));
-            const float c2 = 1.0f / (1.0f - std::pow(b2, (float)q->step));
-            hipLaunchKernelGGL(k_adam, dim3((int)((q->n_params + 255) / 256)), dim3(256), 0, q->s, q->w, q->gpart,
-                               q->nsplit, q->m,
-                               q->v, q->wf, q->wbk, q->layout, (int)q->n_params, (int)q->off_w[0], (int)q->off_w[3],
-                               (int)q->off_b[3], lr, b1, b2, eps, c1, c2);
-            MCK(hipGetLastError());
+            const AdamArgs A{lr, b1, b2, eps, 1.0f / (1.0f - std::pow(b1, (float)q->step)),
+                             1.0f / (1.0f - std::pow(b2, (float)q->step))};
+            int rc = loss_grad(q, d_genomes + (size_t)s0 * NIN, d_targets + (size_t)s0 * NOUT, nb, last, &A);
+            if (rc) return rc;
         }
     }
     if (last_loss) {
