@@ -74,9 +74,10 @@ def test_mlp_forward_matches_cpu_model():
 
 @pytest.mark.gpu
 def test_mlp_prediction_in_chunks_matches_cpu_model():
-    """Prediction runs in passes of 65536 rows on its own buffers: rows at the start, across
-    the first pass boundary and at the end of a 70000-row call equal the CPU model's (BF16
-    mimic), and a row's prediction does not depend on the call it is part of."""
+    """Prediction runs in one fused pass of 128-row blocks: rows at the start, across block
+    boundaries and in the last, partial block (70000 = 546 x 128 + 112) of a 70000-row call
+    equal the CPU model's (BF16 mimic), and a row's prediction does not depend on the call it
+    is part of."""
     import planes_amd as PA
     g, t, lo, hi = _data(70000, 8)
     p = _params(9)
@@ -92,9 +93,13 @@ def test_mlp_prediction_in_chunks_matches_cpu_model():
 
 
 @pytest.mark.gpu
-def test_mlp_gradients_vs_cpu_model():
+@pytest.mark.parametrize("n", [512, 300])
+def test_mlp_gradients_vs_cpu_model(n):
+    """Gradients of the fused training pass (forward, loss, input gradients in 32-row blocks;
+    weight gradients over the batch) against the CPU model; n = 300 leaves 212 zero rows in
+    the 512-row padded batch, which must contribute nothing."""
     import planes_amd as PA
-    g, t, lo, hi = _data(512, 6)
+    g, t, lo, hi = _data(n, 6)
     p = _params(7)
     S = PA.Surrogate(max_batch=1024)
     S.set_params(p)
@@ -111,6 +116,45 @@ def test_mlp_gradients_vs_cpu_model():
             print(f"layer {L} [{lo_}:{hi_}] rel err vs BF16 model {rel_b:.2e}, vs FP64 {rel_64:.2e}, cos {cos:.5f}")
             assert rel_b < 5e-3 and rel_64 < 5e-2 and cos > 0.998
     assert abs(loss - lb) <= 1e-3 * abs(lb) and abs(loss - l64) <= 2e-2 * abs(l64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1024, 480])
+def test_mlp_adam_step_matches_gradient(n):
+    """One training batch (pr_mlp_train: the weight-gradient kernel takes each parameter's
+    Adam step itself) equals an FP64 Adam step applied to the gradient pr_mlp_loss_grad
+    returns for the same batch; the padding parameters (W1 rows >= 35, W4 and b4 columns
+    >= 4) stay zero, and predictions afterwards use the stepped weights."""
+    import planes_amd as PA
+    g, t, lo, hi = _data(n, 12)
+    p = _params(13)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    S = PA.Surrogate(max_batch=1024)
+    S.set_params(p)
+    _, grad = S.loss_grad(g, t)
+    S.train(g, t, epochs=1, batch=n, lr=lr, shuffle_seed=None)
+    after = S.params()
+    gd = grad.astype(np.float64)
+    m = (1 - b1) * gd
+    v = (1 - b2) * gd * gd
+    want = p.astype(np.float64) - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    sl, _ = M.param_slices()
+    pad = np.zeros(len(p), bool)
+    (a, b), _ = sl[0]
+    pad[a:b] = (np.arange(b - a) // M.HID) >= M.NIN
+    (a, b), (c, d) = sl[3]
+    pad[a:b] = (np.arange(b - a) % M.NOP) >= M.NOUT
+    pad[c:d] = np.arange(d - c) >= M.NOUT
+    assert np.all(after[pad] == 0.0)
+    live = ~pad
+    # the first Adam step moves each parameter by ~lr sign(g); FP32 against FP64 rounding
+    step_gpu = after[live].astype(np.float64) - p[live]
+    step_ref = want[live] - p[live]
+    assert np.max(np.abs(step_gpu - step_ref)) <= 1e-6 + 1e-3 * lr, np.max(np.abs(step_gpu - step_ref))
+    assert np.mean(np.abs(step_ref) > 0.5 * lr) > 0.7  # most parameters moved (dead ReLU units: zero gradient)
+    S2 = PA.Surrogate(max_batch=1024)
+    S2.set_params(after)
+    assert np.array_equal(S.predict(g[:64]), S2.predict(g[:64]))
 
 
 @pytest.mark.gpu
