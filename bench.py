@@ -128,27 +128,54 @@ def fp64_check(genomes, res, w, samples, seeds):
             "fp64_wall_s": round(wall, 1)}
 
 
-def pretrain_surrogate(PA, S, n_pairs, params, batch=250000, seeds_per=8, epochs=4, lr=2e-3):
-    """Setup (untimed, reported): sim pairs of n_pairs designs from the family generator,
-    each flown under 8 seeds with the workload's scenario preset, in batches; the network
-    trains on their [mean, CVaR25, worst, oversize] targets."""
+def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000, seeds_per=8, epochs=4, lr=2e-3):
+    """Setup (untimed, reported): about n_pairs (genome, fitness) pairs, on whose
+    [mean, CVaR25, worst, oversize] targets the network trains.
+    source "c2" (the configuration's "generated by config 2 runs"): `runs` independent runs
+    of the reference GA at config 2's settings -- pop 10 000 x 8 seeds (gen g flies seeds
+    8 g + 1 .. 8 g + 8), config preset with gusts <= 2 m/s RMS, 15 s cap, reference mesh --
+    each for n_pairs / (runs x 10 000) generations; every genome a generation evaluates is
+    one pair with its 8-seed statistics.
+    source "family": n_pairs designs of the family generator, each flown under seeds_per
+    seeds with the workload's scenario preset (params), in batches."""
     t0 = time.perf_counter()
     gs, tg = [], []
-    seeds = seeds_for(900, seeds_per)
-    for b0 in range(0, n_pairs, batch):
-        n = min(batch, n_pairs - b0)
-        g, r, _ = PA.family_population(n, seed=31 + b0 // batch)
-        pl = PA.Planes(n, seeds_per, n * seeds_per, int(r[:, 0].max()), int(r[:, 1].max()))
-        fit, _ = pl.evaluate(g, seeds, params, res=r)
-        pl.close()
-        gs.append(g)
-        tg.append(robust_stats(np.asarray(fit).reshape(n, seeds_per))[0])
+    if source == "c2":
+        w = WORKLOADS["c2"]
+        pop, ns = w["pop"], w["nseeds"]
+        gens = max(1, n_pairs // (runs * pop))
+        p2 = PA.SimParams.config(gust_rms_max=w["gust"], warp_max_deg=w["warp_deg"], wind_sd=w["wind_sd"],
+                                 max_steps=w["max_steps"])
+        E = PA.Generation(pop, ns)
+        for run in range(runs):
+            ga = PA.GA(seed=101 + run)
+            g = ga.random(pop)
+            for gen in range(gens):
+                E.upload(g)
+                r = E.evaluate(pop, seeds_for(gen, ns), p2, holdout_k=0, want_samples=True)
+                gs.append(g)
+                tg.append(robust_stats(r["samples"])[0])
+                g = ga.breed(g, r["acc"], r["idx"], gen, gens)
+        desc = (f"config 2 runs: {runs} GA runs x {gens} generations, pop {pop} x {ns} seeds, config preset "
+                f"(gusts <= {w['gust']} m/s RMS), {w['max_steps'] // 600} s cap")
+        per = ns
+    else:
+        seeds = seeds_for(900, seeds_per)
+        for b0 in range(0, n_pairs, batch):
+            n = min(batch, n_pairs - b0)
+            g, r, _ = PA.family_population(n, seed=31 + b0 // batch)
+            pl = PA.Planes(n, seeds_per, n * seeds_per, int(r[:, 0].max()), int(r[:, 1].max()))
+            fit, _ = pl.evaluate(g, seeds, params, res=r)
+            pl.close()
+            gs.append(g)
+            tg.append(robust_stats(np.asarray(fit).reshape(n, seeds_per))[0])
+        desc, per = "family generator, config preset", seeds_per
     sim_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    loss = S.train(np.concatenate(gs), np.concatenate(tg), epochs=epochs, batch=4096, lr=lr)
-    return {"pairs": n_pairs, "seeds_per_design": seeds_per, "source": "family generator, config preset",
-            "sim_s": round(sim_s, 2), "train_s": round(time.perf_counter() - t0, 2), "epochs": epochs,
-            "final_loss": round(float(loss), 4)}
+    G, T = np.concatenate(gs), np.concatenate(tg)
+    loss = S.train(G, T, epochs=epochs, batch=4096, lr=lr)
+    return {"pairs": int(len(G)), "seeds_per_design": per, "source": desc, "sim_s": round(sim_s, 2),
+            "train_s": round(time.perf_counter() - t0, 2), "epochs": epochs, "final_loss": round(float(loss), 4)}
 
 
 ARMS = {
@@ -164,16 +191,17 @@ ARMS = {
 
 def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, gens=4, sur_gens=None, pretrain=2_000_000,
                  explore=0.3, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
-                 arms=("plain", "surrogate", "strict", "uniform", "parent"), delta_mode=True, pretrained_state=None):
+                 arms=("plain", "surrogate", "strict", "uniform", "parent"), delta_mode=True, pretrained_state=None,
+                 pretrain_source="c2"):
     """GA generations/hour with surrogate pre-screening against the plain batched sim, and
     against the same fly budget chosen without the network.  One generation: a pool of n_pool
     Gaussian mutants (sigma 5 % of each gene's range) of an n_elite elite; every arm flies its
     fly set under 12 environments (gusts <= 4 m/s RMS, 12 s cap), keeps the best simulated
     genomes as the next elite and re-flies its top 8 on the 128-scenario holdout suite.  The
     arms (ARMS): "plain" flies all n_pool candidates; the others fly n_fly.  The network is
-    pretrained once on `pretrain` sim pairs of the family generator (8 seeds each; setup,
-    untimed, its cost reported and also added to the network arms' time to the plain
-    champion), and each network run starts from those weights and fine-tunes on its own
+    pretrained once on about `pretrain` sim pairs (pretrain_surrogate: config 2 GA runs by
+    default; setup, untimed, its cost reported and also added to the network arms' time to
+    the plain champion), and each network run starts from those weights and fine-tunes on its own
     flights.  With pretrain 0 a network arm's first generation is a plain one whose flights
     bootstrap the network (timed).  Every arm runs `reps` GA seeds from the same initial
     elite; the plain arm `gens` generations, the others `sur_gens` (default 3 x gens).
@@ -193,7 +221,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
             pre, p0 = pretrained_state
         else:
             S = PA.Surrogate(max_batch=mlp_batch, seed=3)
-            pre = pretrain_surrogate(PA, S, pretrain, params)
+            pre = pretrain_surrogate(PA, S, pretrain, params, source=pretrain_source)
             p0 = S.params()
             log(f"[surrogate] pretraining: {pre}")
         pre_s = pre["sim_s"] + pre["train_s"]
@@ -299,7 +327,8 @@ def pool_ab(PA, args):
     """C4 as its own bench line: the surrogate A/B (surrogate_ab) with the command-line sizes;
     value = the surrogate arm's generations/hour."""
     ab = surrogate_ab(PA, n_pool=args.pool, n_fly=args.fly, n_elite=args.elite, reps=args.reps, gens=args.gens,
-                      sur_gens=args.sur_gens, pretrain=args.pretrain, explore=args.explore, buffer=args.buffer,
+                      sur_gens=args.sur_gens, pretrain=args.pretrain, pretrain_source=args.pretrain_source,
+                      explore=args.explore, buffer=args.buffer,
                       mlp_batch=args.mlp_batch, train_epochs=args.train_epochs, lr=args.lr,
                       boot_epochs=args.boot_epochs)
     ab.pop("_pretrained_state")
@@ -456,6 +485,9 @@ def main():
                     help="c4: surrogate learning rate (Adam; default 1e-4 fine-tuning after pretraining, else 2e-3)")
     ap.add_argument("--reps", type=int, default=5, help="c4: GA seeds per arm")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
+    ap.add_argument("--pretrain-source", choices=["c2", "family"], default="c2",
+                    help="surrogate pretraining pairs: config 2 GA runs (the configuration's source) or the family "
+                         "generator")
     ap.add_argument("--pretrain", type=int, default=2_000_000,
                     help="c4: sim pairs the surrogate is pretrained on (setup; 0: bootstrap generation instead)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget (0 = skip)")
@@ -653,7 +685,8 @@ def main():
     ab = None
     if args.surrogate_ab if args.surrogate_ab is not None else args.workload == "c3":
         t0 = time.perf_counter()
-        ab = surrogate_ab(PA, reps=args.ab_reps, gens=args.ab_gens, sur_gens=args.ab_sur_gens)
+        ab = surrogate_ab(PA, reps=args.ab_reps, gens=args.ab_gens, sur_gens=args.ab_sur_gens,
+                          pretrain_source=args.pretrain_source)
         ab.pop("_pretrained_state")
         ab["wall_s"] = round(time.perf_counter() - t0, 1)
         log(f"surrogate A/B: {json.dumps(ab['arms'].get('surrogate', {}))[:400]}")
