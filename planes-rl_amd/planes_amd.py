@@ -491,6 +491,7 @@ class Generation:
         self.pop = pop
         self.d_genomes = DeviceBuffer(4 * N_GENES * pop)
         self.d_res = DeviceBuffer(8 * pop)
+        self.has_res = False
 
     def upload(self, genomes, res=None):
         self.d_genomes.upload(np.ascontiguousarray(genomes, np.float32))

@@ -363,6 +363,89 @@ def test_capacity_errors_are_reported(PA):
         pl.fly(0, 2, 4)
 
 
+def test_empty_inputs_are_refused(PA):
+    """Zero genomes, scenarios, flights or population: each entry point refuses the call
+    with an error before anything is launched, and the context stays usable."""
+    pl = PA.Planes(4, 4, 16)
+    with pytest.raises(PA.PlanesError, match="1..max_genomes"):
+        PA._ck(PA.LIB.pr_prepare(pl.h, pl.d_genomes.ptr, None, 0), "pr_prepare")
+    pl.prepare(O.random_genomes(4, 1))
+    with pytest.raises(PA.PlanesError, match="1..max_scenarios"):
+        pl.scenarios(n=0)
+    pl.scenarios([1, 2, 3])
+    with pytest.raises(PA.PlanesError, match="n_genomes"):
+        pl.fly(0, 0, 3)
+    with pytest.raises(PA.PlanesError, match="n_scen"):
+        pl.fly(0, 4, 0)
+    with pytest.raises(PA.PlanesError, match="pop > 0"):
+        PA._ck(PA.LIB.pr_scores(pl.h, pl.d_fit.ptr, 0, 3, pl.d_scores.ptr), "pr_scores")
+    gen = PA.Generation(8, 4)
+    with pytest.raises(PA.PlanesError):
+        gen.evaluate(0, [1, 2], PA.SimParams.reference())
+    fit, steps = pl.fly(0, 4, 3)  # the refused calls left the prepared batch intact
+    ref, _ = PA.Planes(4, 3, 12).evaluate(O.random_genomes(4, 1), [1, 2, 3])
+    assert np.array_equal(fit.reshape(4, 3), ref)
+
+
+def _ending(pos, fit, steps, cap):
+    """Which termination rule ended a flight (sim.cu:199-204): unprintable (never flown),
+    NaN state, ground, lateral bound |y| > 300, behind the launch x < -50, step cap."""
+    if fit == -100.0 and steps == 0:
+        return "unprintable"
+    if fit == -10.0:
+        return "nan"
+    if pos[2] <= 0.0:
+        return "ground"
+    if abs(pos[1]) > 300.0:
+        return "lateral"
+    if pos[0] < -50.0:
+        return "behind"
+    return "cap" if steps == cap else "none"
+
+
+def test_termination_branches_vs_fp64(PA):
+    """Every way a flight ends, against the FP64 oracle: 128 printable and 16 unprintable
+    random genomes x 4 seeds under 30 m/s-SD winds (config preset, gusts <= 4 m/s RMS,
+    warp 1 deg, 3600-step cap), which blow flights past the lateral and rear bounds as well
+    as to the ground or the cap.  Checked: each rule fires on the GPU; a flown flight's
+    fitness is its final x (sim.cu:201-204) and its final state meets the rule that ended it;
+    the ending and the distance agree with FP64 (distance within 1 % for >= 97 %)."""
+    rng_gs = O.random_genomes(512, 5)
+    ok = np.array([bool(O.mass(g)[1]) for g in rng_gs])
+    gs = np.concatenate([rng_gs[ok][:128], rng_gs[~ok][:16]])
+    assert len(gs) == 144
+    seeds = np.arange(1, 5, dtype=np.int32)
+    cap = 3600
+    params = PA.SimParams.config(gust_rms_max=4.0, warp_max_deg=1.0, wind_sd=30.0, max_steps=cap)
+    kw = dict(preset=1, gust=4.0, warp=float(params.warp_max), wsd=30.0)
+    pl, fit, steps, f64, s64 = _fly_both(PA, gs, seeds, params, 64, kw)
+    st = pl.read_states(0, len(fit))
+    ends = [_ending(st["state"][f, :3], fit[f], steps[f], cap) for f in range(len(fit))]
+    flown = np.array([e != "unprintable" for e in ends])
+    assert not flown[-64:].any() and flown[:-64].all()
+    # the recorded fitness of a flown flight is its final x, and the rule that stopped it holds
+    assert np.array_equal(fit[flown], st["state"][flown, 0])
+    assert "none" not in ends and "nan" not in ends
+    counts = {e: ends.count(e) for e in ("unprintable", "ground", "lateral", "behind", "cap")}
+    print("GPU endings:", counts, flush=True)
+    assert all(c >= 4 for c in counts.values()), counts
+    # the same endings in FP64 (its final states from the oracle's trajectories)
+    ends64 = []
+    for f in range(len(fit)):
+        g, s = divmod(f, len(seeds))
+        if f64[f] == -100.0:
+            ends64.append("unprintable")
+            continue
+        tr = O.trajectory(gs[g], int(seeds[s]), int(s64[f]), 64, **kw, max_steps=cap)
+        ends64.append(_ending(tr[-1, :3], f64[f], s64[f], cap))
+    same = np.mean([a == b for a, b in zip(ends, ends64)])
+    rel = np.abs(fit - f64) / np.maximum(np.abs(f64), 1.0)
+    within = np.mean(rel <= 0.01)
+    print(f"ending equal to FP64 {same * 100:.1f}%, distance within 1% {within * 100:.2f}%, "
+          f"equal step counts {np.mean(steps == s64) * 100:.1f}%", flush=True)
+    assert same >= 0.95 and within >= 0.97
+
+
 def test_full_size_generation_properties(PA):
     """pop 10,000 x 8 seeds (one GA generation at the config-2 size): no NaN, results in the
     ranges the model allows, statistics consistent, scores ranking consistent."""
