@@ -128,7 +128,18 @@ def fp64_check(genomes, res, w, samples, seeds):
             "fp64_wall_s": round(wall, 1)}
 
 
-def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000, seeds_per=8, epochs=4, lr=2e-3):
+def relative_targets(tg):
+    """The regression targets (mean, CVaR, worst) of one batch flown under one scenario set,
+    relative to the batch's medians over its flyable designs (pr_pool_set_target_norm)."""
+    tg = tg.copy()
+    ok = tg[:, 3] == 0
+    if ok.any():  # the upper median (element n // 2 in order), as the library takes it
+        tg[:, :3] -= np.sort(tg[ok, :3], axis=0)[ok.sum() // 2]
+    return tg
+
+
+def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000, seeds_per=8, epochs=4, lr=2e-3,
+                       norm=False):
     """Setup (untimed, reported): about n_pairs (genome, fitness) pairs, on whose
     [mean, CVaR25, worst, oversize] targets the network trains.
     source "c2" (the configuration's "generated by config 2 runs"): `runs` independent runs
@@ -137,7 +148,8 @@ def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000
     each for n_pairs / (runs x 10 000) generations; every genome a generation evaluates is
     one pair with its 8-seed statistics.
     source "family": n_pairs designs of the family generator, each flown under seeds_per
-    seeds with the workload's scenario preset (params), in batches."""
+    seeds with the workload's scenario preset (params), in batches.
+    norm: each batch's targets relative to its medians (relative_targets)."""
     t0 = time.perf_counter()
     gs, tg = [], []
     if source == "c2":
@@ -154,7 +166,8 @@ def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000
                 E.upload(g)
                 r = E.evaluate(pop, seeds_for(gen, ns), p2, holdout_k=0, want_samples=True)
                 gs.append(g)
-                tg.append(robust_stats(r["samples"])[0])
+                t = robust_stats(r["samples"])[0]
+                tg.append(relative_targets(t) if norm else t)
                 g = ga.breed(g, r["acc"], r["idx"], gen, gens)
         desc = (f"config 2 runs: {runs} GA runs x {gens} generations, pop {pop} x {ns} seeds, config preset "
                 f"(gusts <= {w['gust']} m/s RMS), {w['max_steps'] // 600} s cap")
@@ -168,13 +181,15 @@ def pretrain_surrogate(PA, S, n_pairs, params, source="c2", runs=8, batch=250000
             fit, _ = pl.evaluate(g, seeds, params, res=r)
             pl.close()
             gs.append(g)
-            tg.append(robust_stats(np.asarray(fit).reshape(n, seeds_per))[0])
+            t = robust_stats(np.asarray(fit).reshape(n, seeds_per))[0]
+            tg.append(relative_targets(t) if norm else t)
         desc, per = "family generator, config preset", seeds_per
     sim_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     G, T = np.concatenate(gs), np.concatenate(tg)
     loss = S.train(G, T, epochs=epochs, batch=4096, lr=lr)
-    return {"pairs": int(len(G)), "seeds_per_design": per, "source": desc, "sim_s": round(sim_s, 2),
+    return {"pairs": int(len(G)), "seeds_per_design": per, "source": desc, "relative_targets": bool(norm),
+            "sim_s": round(sim_s, 2),
             "train_s": round(time.perf_counter() - t0, 2), "epochs": epochs, "final_loss": round(float(loss), 4)}
 
 
@@ -190,9 +205,9 @@ ARMS = {
 
 
 def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, gens=4, sur_gens=None, pretrain=2_000_000,
-                 explore=0.3, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
+                 explore=0.5, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
                  arms=("plain", "surrogate", "strict", "uniform", "parent"), delta_mode=True, pretrained_state=None,
-                 pretrain_source="c2"):
+                 pretrain_source="c2", target_norm=True):
     """GA generations/hour with surrogate pre-screening against the plain batched sim, and
     against the same fly budget chosen without the network.  One generation: a pool of n_pool
     Gaussian mutants (sigma 5 % of each gene's range) of an n_elite elite; every arm flies its
@@ -221,7 +236,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
             pre, p0 = pretrained_state
         else:
             S = PA.Surrogate(max_batch=mlp_batch, seed=3)
-            pre = pretrain_surrogate(PA, S, pretrain, params, source=pretrain_source)
+            pre = pretrain_surrogate(PA, S, pretrain, params, source=pretrain_source, norm=target_norm)
             p0 = S.params()
             log(f"[surrogate] pretraining: {pre}")
         pre_s = pre["sim_s"] + pre["train_s"]
@@ -238,6 +253,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
                 S = PA.Surrogate(max_batch=mlp_batch, seed=3)
                 ga.set_explore(explore if arm == "surrogate" else 0.0)
                 ga.set_parent_delta(delta_mode if arm == "surrogate" else False)
+                ga.set_target_norm(target_norm)
                 if pretrained:
                     S.set_params(p0)
                 else:
@@ -318,6 +334,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
             summary[arm]["mean_key_fit"] = [round(float(v), 3) for v in np.mean(np.array(w), 0)]
     return {"config": {"pool": n_pool, "fly": n_fly, "elite": n_elite, "environments": 12, "ga_seeds": reps,
                        "plain_gens": gens, "other_gens": max(gens, sur_gens), "explore": explore,
+                       "relative_targets": bool(target_norm),
                        "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": buffer},
                        "scenarios": "config preset, gusts <= 4 m/s RMS, warp +-1 deg, 12 s cap"},
             "pretraining": pre, "arms": summary, "network_gain": gain, "_pretrained_state": (pre, p0)}
@@ -328,7 +345,7 @@ def pool_ab(PA, args):
     value = the surrogate arm's generations/hour."""
     ab = surrogate_ab(PA, n_pool=args.pool, n_fly=args.fly, n_elite=args.elite, reps=args.reps, gens=args.gens,
                       sur_gens=args.sur_gens, pretrain=args.pretrain, pretrain_source=args.pretrain_source,
-                      explore=args.explore, buffer=args.buffer,
+                      explore=args.explore, buffer=args.buffer, target_norm=args.target_norm,
                       mlp_batch=args.mlp_batch, train_epochs=args.train_epochs, lr=args.lr,
                       boot_epochs=args.boot_epochs)
     ab.pop("_pretrained_state")
@@ -477,7 +494,10 @@ def main():
                     help="c4: generations of the surrogate arm (default 3 x --gens), for the time it takes to "
                          "reach the plain arm's champion")
     ap.add_argument("--buffer", type=int, default=300_000, help="c4: surrogate replay buffer")
-    ap.add_argument("--explore", type=float, default=0.3, help="c4: fraction of the fly set drawn at random")
+    ap.add_argument("--explore", type=float, default=0.5, help="c4: fraction of the fly set drawn at random")
+    ap.add_argument("--target-norm", action=argparse.BooleanOptionalAction, default=True,
+                    help="surrogate targets of each batch relative to its medians (the scenario set's common "
+                         "effect removed); --no-target-norm: absolute targets")
     ap.add_argument("--boot-epochs", type=int, default=16, help="c4: surrogate epochs on the bootstrap generation")
     ap.add_argument("--train-epochs", type=int, default=None,
                     help="c4: surrogate epochs per generation (default 2 after pretraining, else 4)")
@@ -686,7 +706,7 @@ def main():
     if args.surrogate_ab if args.surrogate_ab is not None else args.workload == "c3":
         t0 = time.perf_counter()
         ab = surrogate_ab(PA, reps=args.ab_reps, gens=args.ab_gens, sur_gens=args.ab_sur_gens,
-                          pretrain_source=args.pretrain_source)
+                          pretrain_source=args.pretrain_source, target_norm=args.target_norm)
         ab.pop("_pretrained_state")
         ab["wall_s"] = round(time.perf_counter() - t0, 1)
         log(f"surrogate A/B: {json.dumps(ab['arms'].get('surrogate', {}))[:400]}")

@@ -296,6 +296,12 @@ int pr_pool_set_parent_delta(pr_pool* p, int on);
  * network: the baselines against which the surrogate's selection is measured.  No reference
  * counterpart (the reference has no candidate pool). */
 int pr_pool_set_selection(pr_pool* p, int mode);
+/* 1: the regression targets (mean, CVaR, worst) of each collected batch -- a generation's
+ * fly set, or one pr_pool_add_training call -- are stored relative to the batch's medians
+ * over its flyable designs, which removes the scenario set's common effect on every
+ * distance of that batch (default); 0: absolute targets.  pr_pool_last returns the stored
+ * targets.  No reference counterpart. */
+int pr_pool_set_target_norm(pr_pool* p, int on);
 int pr_pool_train(pr_pool* p, pr_mlp* model, int epochs, float lr, float* loss);
 int pr_pool_champion(pr_pool* p, float* genome35, float* holdout);
 /* The last generation's pool [n_pool x 35], fly set [n_fly x 35] in flight order, its

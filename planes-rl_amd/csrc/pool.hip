@@ -171,6 +171,22 @@ __global__ void k_targets(const float* __restrict__ samples, int n, int k, float
     tgt[(size_t)i * 4 + 3] = over ? 1.0f : 0.0f;
 }
 
+// sort keys of one target column: the flyable rows' values, +inf for the others (sorted last)
+__global__ void k_median_keys(const float* __restrict__ tgt, int n, int col, float* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = tgt[(size_t)i * 4 + 3] == 0.0f ? tgt[(size_t)i * 4 + col] : __builtin_inff();
+}
+
+// regression targets (heads 0..2) relative to their batch's offsets (target normalisation)
+__global__ void k_shift_targets(float* __restrict__ tgt, int n, float o0, float o1, float o2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tgt[(size_t)i * 4 + 0] -= o0;
+    tgt[(size_t)i * 4 + 1] -= o1;
+    tgt[(size_t)i * 4 + 2] -= o2;
+}
+
 }  // namespace
 
 static const float kDeltaWeights[3] = {1.0f, 1.0f, -1.0f};
@@ -184,6 +200,7 @@ struct pr_pool {
     float *d_pred = nullptr, *d_keys = nullptr, *d_keys2 = nullptr, *d_samples = nullptr, *d_tgt = nullptr;
     float *d_buf_g = nullptr, *d_buf_t = nullptr;
     float* d_escore = nullptr;  // [n_elite] simulated scores of the current elite
+    float* d_med = nullptr;     // [2 n_pool] sort keys of the target medians
     int has_escore = 0;         // set by a generation, cleared by pr_pool_set_elite
     int parent_delta = 1;       // mutants' keys from the parent's simulated score (when known); 2: calibrated
     float cw[3] = {1.0f, 1.0f, -1.0f};  // key weights of (parent score, prediction, parent's prediction)
@@ -196,6 +213,8 @@ struct pr_pool {
     int collect = 0;  // plain generations also feed the replay buffer
     int select_mode = 0;  // without a model: 0 fly the whole pool, 1 uniform subset, 2 parent-score subset
     float explore = 0.0f;  // fraction of the surrogate fly set drawn at random from the pool
+    int norm_targets = 1;  // 1: each batch's regression targets relative to the batch's medians
+    float tgt_off[3] = {0.0f, 0.0f, 0.0f};  // the last collected batch's offsets
     float best_holdout = -1e9f;
     std::vector<float> acc, samples, champion;
     std::vector<int> idx;
@@ -248,8 +267,12 @@ int pr_pool_create(pr_pool** out, int n_elite, int n_pool, int n_fly, int nseeds
     PCK(hipMalloc(&p->d_tgt, sizeof(float) * 4 * (size_t)n_pool));
     PCK(hipMalloc(&p->d_buf_g, G * p->buf_cap));
     PCK(hipMalloc(&p->d_buf_t, sizeof(float) * 4 * (size_t)p->buf_cap));
+    PCK(hipMalloc(&p->d_med, sizeof(float) * 2 * (size_t)n_pool));
     PCK(hipcub::DeviceRadixSort::SortPairs(nullptr, p->sort_tmp_bytes, p->d_keys, p->d_keys2, p->d_vals, p->d_vals2,
                                            n_pool));
+    size_t key_bytes = 0;
+    PCK(hipcub::DeviceRadixSort::SortKeys(nullptr, key_bytes, p->d_med, p->d_med + n_pool, n_pool));
+    p->sort_tmp_bytes = std::max(p->sort_tmp_bytes, key_bytes);
     PCK(hipMalloc(&p->d_sort_tmp, p->sort_tmp_bytes + 16));
     float lo[pr::NG], hi[pr::NG];
     pr_gene_bounds(lo, hi);
@@ -265,7 +288,7 @@ void pr_pool_destroy(pr_pool* p) {
     pr_gen_destroy(p->gen_eval);
     void* bufs[] = {p->d_elite, p->d_pool, p->d_fly, p->d_lo, p->d_hi, p->d_pred, p->d_keys, p->d_keys2,
                     p->d_vals, p->d_vals2, p->d_idx, p->d_samples, p->d_tgt, p->d_buf_g, p->d_buf_t, p->d_sort_tmp,
-                    p->d_escore};
+                    p->d_escore, p->d_med};
     for (void* b : bufs) if (b) (void)hipFree(b);
     delete p;
 }
@@ -300,6 +323,37 @@ static int buffer_add(pr_pool* p, const float* d_g, const float* d_t, int n) {
     }
     // the network trains on its own (non-blocking) stream: the buffer must be complete first
     PCK(hipStreamSynchronize(nullptr));
+    return 0;
+}
+
+// Target normalisation: every design of a batch flew the same scenarios, whose difficulty
+// moves all of the batch's distances together (the same designs' robust scores differ by
+// metres between scenario sets), so the regression targets (mean, CVaR, worst) are taken
+// relative to the batch's medians over its flyable designs: the buffer then holds one
+// frame across generations.  The selection keys use differences of predictions and rank
+// within one generation, so they are unaffected by the offsets.
+static int normalize_targets(pr_pool* p, float* d_t, int n, const float* samples) {
+    p->tgt_off[0] = p->tgt_off[1] = p->tgt_off[2] = 0.0f;
+    if (!p->norm_targets || n < 1) return 0;
+    // flyable rows: not every sample -100 (the targets' oversize flag is 0 exactly there)
+    int n_ok = 0;
+    for (int i = 0; i < n; i++) {
+        const float* r = samples + (size_t)i * p->nseeds;
+        for (int k = 0; k < p->nseeds; k++)
+            if (r[k] != -100.0f) { n_ok++; break; }
+    }
+    if (n_ok == 0) return 0;
+    // each head's upper median over the flyable rows: element n_ok / 2 of its sorted values
+    for (int k = 0; k < 3; k++) {
+        hipLaunchKernelGGL(k_median_keys, dim3((n + 255) / 256), dim3(256), 0, 0, d_t, n, k, p->d_med);
+        PCK(hipGetLastError());
+        size_t tb = p->sort_tmp_bytes;
+        PCK(hipcub::DeviceRadixSort::SortKeys(p->d_sort_tmp, tb, p->d_med, p->d_med + p->n_pool, n));
+        PCK(hipMemcpy(&p->tgt_off[k], p->d_med + p->n_pool + n_ok / 2, sizeof(float), hipMemcpyDeviceToHost));
+    }
+    hipLaunchKernelGGL(k_shift_targets, dim3((n + 255) / 256), dim3(256), 0, 0, d_t, n, p->tgt_off[0], p->tgt_off[1],
+                       p->tgt_off[2]);
+    PCK(hipGetLastError());
     return 0;
 }
 
@@ -391,6 +445,19 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     PCK(hipMemcpy(p->champion.data(), fly + (size_t)champ * pr::NG, sizeof(float) * pr::NG, hipMemcpyDeviceToHost));
     p->champion_score = holdout;
     p->best_holdout = std::max(p->best_holdout, holdout);
+    // training targets of the flown pairs (before the screening figures: the mean absolute
+    // error compares predictions in the targets' frame)
+    if (collect) {
+        auto m0 = std::chrono::steady_clock::now();
+        PCK(hipMemcpy(p->d_samples, p->samples.data(), sizeof(float) * p->samples.size(), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_targets, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_samples, nf, p->nseeds, p->d_tgt);
+        PCK(hipGetLastError());
+        int rc = normalize_targets(p, p->d_tgt, nf, p->samples.data());
+        if (rc) return rc;
+        mlp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - m0).count();
+    }
+    // the predicted robust score's offset in that frame
+    const float score_off = 0.65f * p->tgt_off[0] + 0.25f * p->tgt_off[1] + 0.10f * p->tgt_off[2];
     // screening quality: rank correlation of predicted and simulated scores over the
     // exploration sample of the fly set (uniform draws from the pool, so unbiased by the
     // selection; the surrogate-selected part over-represents over-predicted candidates)
@@ -464,7 +531,7 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
             for (size_t i = 0; i < a.size(); i++) {
                 const bool pa = a[i] == -100.0f, pb = b[i] == -100.0f;
                 agree += pa == pb;
-                if (!pa && !pb) { e += std::fabs((double)a[i] - (double)b[i]); n_fly++; }
+                if (!pa && !pb) { e += std::fabs((double)a[i] + (double)score_off - (double)b[i]); n_fly++; }
             }
             if (n_fly) mae = e / n_fly;
             acc_over = (double)agree / (double)a.size();
@@ -473,9 +540,6 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
     float loss = 0;
     if (collect) {
         auto m0 = std::chrono::steady_clock::now();
-        PCK(hipMemcpy(p->d_samples, p->samples.data(), sizeof(float) * p->samples.size(), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_targets, dim3((nf + 255) / 256), dim3(256), 0, 0, p->d_samples, nf, p->nseeds, p->d_tgt);
-        PCK(hipGetLastError());
         int rc = buffer_add(p, fly, p->d_tgt, nf);
         if (rc) return rc;
         if (model && train_epochs > 0)
@@ -513,6 +577,8 @@ int pr_pool_add_training(pr_pool* p, const float* d_genomes, const float* sample
     PCK(hipMemcpy(p->d_samples, samples, sizeof(float) * (size_t)n * p->nseeds, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, 0, p->d_samples, n, p->nseeds, p->d_tgt);
     PCK(hipGetLastError());
+    int rc = normalize_targets(p, p->d_tgt, n, samples);
+    if (rc) return rc;
     return buffer_add(p, d_genomes, p->d_tgt, n);
 }
 
@@ -524,8 +590,6 @@ int pr_pool_set_explore(pr_pool* p, float frac) {
     return 0;
 }
 
-// 1: plain generations (model = null) also add their flown pairs to the replay buffer,
-// e.g. to bootstrap the surrogate before pre-screening starts.
 int pr_pool_set_parent_delta(pr_pool* p, int on) {
     if (!p || on < 0 || on > 2) { g_pool_err = "pr_pool_set_parent_delta: null pool or mode outside 0..2"; return -1; }
     p->parent_delta = on;
@@ -538,9 +602,19 @@ int pr_pool_set_selection(pr_pool* p, int mode) {
     return 0;
 }
 
+// 1: plain generations (model = null) also add their flown pairs to the replay buffer,
+// e.g. to bootstrap the surrogate before pre-screening starts.
 int pr_pool_set_collect(pr_pool* p, int on) {
     if (!p) { g_pool_err = "pr_pool_set_collect: null pool"; return -1; }
     p->collect = on ? 1 : 0;
+    return 0;
+}
+
+// 1 (default): each collected batch's regression targets relative to the batch's medians
+// (see normalize_targets); 0: absolute targets.
+int pr_pool_set_target_norm(pr_pool* p, int on) {
+    if (!p) { g_pool_err = "pr_pool_set_target_norm: null pool"; return -1; }
+    p->norm_targets = on ? 1 : 0;
     return 0;
 }
 

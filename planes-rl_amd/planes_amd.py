@@ -137,6 +137,7 @@ def _load():
     lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
     lib.pr_pool_set_parent_delta.argtypes = [VP, C.c_int]
     lib.pr_pool_set_selection.argtypes = [VP, C.c_int]
+    lib.pr_pool_set_target_norm.argtypes = [VP, C.c_int]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_pool_last.argtypes = [VP, VP, VP, VP, VP, C.POINTER(C.c_int)]
     lib.pr_mlp_param_count.restype = C.c_int
@@ -879,6 +880,11 @@ class PoolGA:
         (elite + the mutants of the best-scoring parents) -- the no-network baselines."""
         m = {"all": 0, "uniform": 1, "parent": 2}[mode]
         self._ck(LIB.pr_pool_set_selection(self.h, m), "pr_pool_set_selection")
+
+    def set_target_norm(self, on=True):
+        """Regression targets of each collected batch relative to the batch's medians over
+        its flyable designs (removes the scenario set's common effect on the distances)."""
+        self._ck(LIB.pr_pool_set_target_norm(self.h, int(bool(on))), "pr_pool_set_target_norm")
 
     def set_collect(self, on=True):
         """Plain generations also feed the surrogate's replay buffer (bootstrap)."""

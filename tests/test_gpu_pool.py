@@ -12,7 +12,9 @@ operators are pinned in test_gpu_parity.py and test_host.py).  Pinned here:
     per candidate), then the best-predicted candidates, in the order of a stable sort of
     those keys; the reported rank correlation, mean absolute error (members both call
     flyable) and oversize-flag accuracy are those over the exploration sample;
-  - the replay-buffer targets equal the CPU model's fitness statistics of the flights;
+  - the replay-buffer targets equal the CPU model's fitness statistics of the flights, taken
+    relative to the batch's upper medians over its flyable designs (and absolute with the
+    normalisation off);
   - reproducibility: the same seed gives the same generations bit for bit.
 """
 import os
@@ -170,7 +172,13 @@ def test_surrogate_prescreening_selection_and_targets(PA):
     # simulated scores and targets of the fly set vs an independent evaluation
     fit, want_sc = independent_scores(PA, fly, gen, k, params)
     assert np.array_equal(sc, want_sc)
-    assert np.array_equal(tg, M.fitness_targets(fit))
+    T = M.fitness_targets(fit)
+    ok = T[:, 3] == 0
+    med = np.sort(T[ok, :3], axis=0)[ok.sum() // 2]  # the upper median of each head
+    want_tg = T.copy()
+    want_tg[:, :3] = T[:, :3] - med  # float32, as the library subtracts
+    assert np.array_equal(tg, want_tg)
+    off = 0.65 * float(med[0]) + 0.25 * float(med[1]) + 0.10 * float(med[2])  # the predicted score's offset
     # Spearman over the exploration sample (positions n_elite .. n_elite + n_exp of the fly set)
     ex = slice(n_elite, n_elite + n_exp)
     a, b = pred[order[:n_fly]][ex], sc[ex]
@@ -180,12 +188,20 @@ def test_surrogate_prescreening_selection_and_targets(PA):
     assert r["rank_corr"] == pytest.approx(rho, abs=1e-9)
     pa, pb = a == -100.0, b == -100.0
     both = ~pa & ~pb
-    assert r["mae"] == pytest.approx(float(np.mean(np.abs(a[both].astype(np.float64) - b[both]))), rel=1e-9)
+    assert r["mae"] == pytest.approx(float(np.mean(np.abs(a[both].astype(np.float64) + off - b[both]))), rel=1e-6)
     assert r["oversize_acc"] == pytest.approx(float(np.mean(pa == pb)), abs=1e-12)
     # the best-predicted part really is the best predicted outside the elite and the sample
     rest = ~is_exp & (i >= n_elite)
     chosen = order[n_elite + n_exp:n_fly]
     assert pred[chosen].min() >= np.sort(pred[rest])[::-1][len(chosen) - 1]
+    # with the normalisation off the stored targets are the absolute statistics
+    ga2 = PA.PoolGA(n_elite, n_pool, n_fly, k, buf_cap=4 * n_fly, sigma_frac=0.05, seed=seed)
+    ga2.set_elite(g0)
+    ga2.set_explore(explore)
+    ga2.set_target_norm(False)
+    ga2.generation(gen, params, S, train_epochs=0)
+    last2 = ga2.last(targets=True)
+    assert np.array_equal(last2["fly"], fly) and np.array_equal(last2["targets"], T)
 
 
 def test_surrogate_parent_delta_keys(PA):
