@@ -138,6 +138,7 @@ def _load():
     lib.pr_pool_set_parent_delta.argtypes = [VP, C.c_int]
     lib.pr_pool_set_selection.argtypes = [VP, C.c_int]
     lib.pr_pool_set_target_norm.argtypes = [VP, C.c_int]
+    lib.pr_pool_add_training.argtypes = [VP, VP, F32P, C.c_int]
     lib.pr_pool_champion.argtypes = [VP, F32P, C.POINTER(C.c_float)]
     lib.pr_pool_last.argtypes = [VP, VP, VP, VP, VP, C.POINTER(C.c_int)]
     lib.pr_mlp_param_count.restype = C.c_int
@@ -880,6 +881,13 @@ class PoolGA:
         (elite + the mutants of the best-scoring parents) -- the no-network baselines."""
         m = {"all": 0, "uniform": 1, "parent": 2}[mode]
         self._ck(LIB.pr_pool_set_selection(self.h, m), "pr_pool_set_selection")
+
+    def add_training(self, d_genomes, samples):
+        """(genome, fitness) pairs for the replay buffer: device genomes [n x 35] and their
+        host flight distances [n x nseeds] (targets computed, and normalised, as for flown
+        generations)."""
+        s = np.ascontiguousarray(samples, np.float32)
+        self._ck(LIB.pr_pool_add_training(self.h, d_genomes, s.reshape(-1), s.shape[0]), "pr_pool_add_training")
 
     def set_target_norm(self, on=True):
         """Regression targets of each collected batch relative to the batch's medians over

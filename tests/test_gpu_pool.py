@@ -204,6 +204,34 @@ def test_surrogate_prescreening_selection_and_targets(PA):
     assert np.array_equal(last2["fly"], fly) and np.array_equal(last2["targets"], T)
 
 
+def test_add_training_buffer_trains_like_the_relative_targets(PA):
+    """pr_pool_add_training turns distances into the batch's targets relative to its upper
+    medians over the flyable designs (absolute with the normalisation off): training on the
+    pool's replay buffer gives the network bit for bit the weights that training on those
+    targets directly gives (same initial weights, batches in buffer order)."""
+    n, k = 2048, 6
+    g, _, _ = PA.family_population(n, seed=8)
+    rng = np.random.default_rng(4)
+    samples = rng.normal(6.0, 3.0, (n, k)).astype(np.float32)
+    samples[rng.random(n) < 0.07] = -100.0  # oversize designs: every sample -100
+    T = M.fitness_targets(samples)
+    ok = T[:, 3] == 0
+    rel = T.copy()
+    rel[:, :3] = T[:, :3] - np.sort(T[ok, :3], axis=0)[ok.sum() // 2]
+    d_g = PA.DeviceBuffer(g.nbytes)
+    d_g.upload(np.ascontiguousarray(g, np.float32))
+    for norm, want in ((True, rel), (False, T)):
+        ga = PA.PoolGA(8, n, n, k, buf_cap=n, sigma_frac=0.05, seed=1)
+        ga.set_target_norm(norm)
+        ga.add_training(d_g.ptr, samples)
+        S1 = PA.Surrogate(max_batch=1024, seed=3)
+        ga.train(S1, epochs=1, lr=1e-3)
+        S2 = PA.Surrogate(max_batch=1024, seed=3)
+        S2.train(g, want, epochs=1, batch=1024, lr=1e-3, shuffle_seed=None)
+        assert np.array_equal(S1.params(), S2.params()), norm
+        del ga
+
+
 def test_surrogate_parent_delta_keys(PA):
     """After a generation has simulated the elite, a mutant's selection key is its parent's
     simulated score plus the network's predicted change from parent to mutant: the second
