@@ -380,6 +380,11 @@ def roofline_of(stats_list):
 
 
 def audit_bench(PA, args):
+    """C5 as its own bench line."""
+    print(json.dumps(audit_run(PA, args.steps, args.warmup, args.cpu_seconds)), flush=True)
+
+
+def audit_run(PA, steps, warmup, cpu_seconds):
     """C5: robustness audit of a champion + 1 024 designs on 4 096 scenarios each (seed 9001 +
     s*7919 by scenario), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap; one step = one batched
     audit (prepare, scenarios, every flight, per-design statistics on the host)."""
@@ -388,12 +393,12 @@ def audit_bench(PA, args):
     params = PA.SimParams.config(gust_rms_max=5.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=12000)
     pl = PA.Planes(n_gen, n_scen, n_gen * n_scen, max_ch=int(res[:, 0].max()), max_sp=int(res[:, 1].max()))
     pl.upload_genomes(genomes, res)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         pl.audit(None, n_scen, 9001, params, upload=False)
     PA.LIB.pr_device_sync()
     fsteps, stats = 0, []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         st, dist = pl.audit(None, n_scen, 9001, params, upload=False)
         s = pl.stats()
         stats.append(s)
@@ -401,26 +406,28 @@ def audit_bench(PA, args):
     wall = time.perf_counter() - t0
     mean_col = PA.AUDIT_FIELDS.index("mean")
     best = int(np.argmax(st[:, mean_col]))
-    log(f"{args.steps} audits of {n_gen} x {n_scen} in {wall:.3f} s; best mean {st[best, mean_col]:.2f} m "
+    log(f"{steps} audits of {n_gen} x {n_scen} in {wall:.3f} s; best mean {st[best, mean_col]:.2f} m "
         f"(design {best}), failures {int(st[best, PA.AUDIT_FIELDS.index('failures')])}")
     value = fsteps / 600.0 / wall
     rank = surrogate_ranking(PA, genomes, res, dist, params)
     log(f"surrogate ranking of the audited designs: {rank}")
-    cpu, f64 = None, None
-    if args.cpu_seconds > 0:
+    pl.close()
+    cpu = None
+    f64 = audit_fp64_check(genomes, res, best, dist)
+    log(f"FP64 check {f64}")
+    if cpu_seconds > 0:
         wc5 = dict(gust=5.0, warp_deg=1.0, wind_sd=2.0, max_steps=12000, nseeds=n_scen)
-        cpu = cpu_baseline(genomes, res, wc5, args.cpu_seconds)
-        f64 = audit_fp64_check(genomes, res, best, dist)
-        log(f"cpu baseline {cpu['value']:.1f} pfs/s; FP64 check {f64}")
+        cpu = cpu_baseline(genomes, res, wc5, cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.1f} pfs/s")
     out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * wall / args.steps, 3),
+           "steps": steps, "warmup": warmup, "ms_per_step": round(1000 * wall / steps, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "C5: robustness audit, 1025 designs (family mix) x 4096 scenarios (seed 9001 + "
                                   "s*7919), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap",
                       "designs": n_gen, "scenarios": n_scen, "max_steps": 12000, "parallelism": "single GPU"},
-           "audits_per_hour": round(3600.0 * args.steps / wall, 1), "roofline": roofline_of(stats),
+           "audits_per_hour": round(3600.0 * steps / wall, 1), "roofline": roofline_of(stats),
            "cpu_baseline": cpu, "fp64_check": f64, "surrogate_ranking": rank}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def robust_stats(samples):
@@ -477,6 +484,200 @@ def audit_fp64_check(genomes, res, best, dist, n_scen=256):
     return {"design": int(best), "scenarios": n_scen, "within_1pct": round(float(np.mean(rel < 0.01)), 4),
             "median_rel_err": float(f"{np.median(rel):.3g}"), "p99_rel_err": float(f"{np.quantile(rel, 0.99):.3g}"),
             "fp64_wall_s": round(wall, 1)}
+
+
+def generation_run(PA, wname, args, steps, warmup, cpu_seconds):
+    """C1-C3: `steps` timed GA-generation evaluations (prepare, scenarios, every flight, the
+    robust scores, the top-8 x 128 holdout) of one population after `warmup` untimed ones;
+    returns the bench line (without the surrogate A/B)."""
+    w = WORKLOADS[wname]
+    res = None
+    if w.get("families"):
+        genomes, res, _ = PA.family_population(w["pop"], seed=0)
+    else:
+        lo, hi = PA.gene_bounds()
+        rng = np.random.default_rng(0)
+        genomes = (lo + rng.random((w["pop"], 35), dtype=np.float32) * (hi - lo)).astype(np.float32)
+    params = PA.SimParams.config(gust_rms_max=w["gust"], warp_max_deg=w["warp_deg"], wind_sd=w["wind_sd"],
+                                 max_steps=w["max_steps"])
+    max_ch = int(res[:, 0].max()) if res is not None else 0
+    max_sp = int(res[:, 1].max()) if res is not None else 0
+    E = PA.Generation(w["pop"], w["nseeds"], max_ch, max_sp)
+    E.upload(genomes, res)
+    if args.unfused:
+        E.set_fused(False)
+    if args.list_order != "size":
+        E.set_list_order(False)
+    if args.no_split:
+        E.set_split(False)
+    elif args.split_holdout:
+        E.set_split(True, holdout=True)
+    if args.graph is None:
+        args.graph = w.get("graph", False)
+    if args.graph:
+        E.set_graph(True)
+    if args.tail_mode is not None:
+        E.set_tail_mapping(args.tail_mode)
+    tuned = None
+    if args.buckets:
+        spec = args.buckets if args.buckets.lstrip().startswith("{") else open(args.buckets).read()
+        b = json.loads(spec)
+        tuned = (list(b["nt_max"]), list(b["L"]), list(b["P"]))
+        E.set_buckets(*tuned)
+    elif args.autotune if args.autotune is not None else w.get("autotune", False):
+        # one-time setup outside the timed region: the mapping per triangle-count bucket from
+        # timing every compiled (L, P) on a sample of this population's flights
+        t0 = time.perf_counter()
+        tuned = PA.autotune(genomes, res, seeds_for(20000, w["nseeds"]), params)
+        E.set_buckets(*tuned)
+        log(f"autotuned buckets {list(zip(*tuned))} in {time.perf_counter() - t0:.1f} s")
+
+    for k in range(warmup):
+        E.evaluate(w["pop"], seeds_for(10000 + k, w["nseeds"]), params)
+    PA.LIB.pr_device_sync()
+    fsteps = 0
+    bucket = {}
+    fused = {"flight_steps": 0, "panel_steps": 0, "ms": 0.0, "launches": 0}
+    fused_graph = {"flight_steps": 0, "panel_steps": 0, "ms": 0.0, "launches": 0}
+    t0 = time.perf_counter()
+    pipelined = not args.no_pipeline
+    for k in range(steps):
+        if pipelined:
+            # generation k's holdout flies on its own stream while generation k+1's
+            # population flies; the last holdout is collected inside the timed region
+            E.evaluate_async(w["pop"], seeds_for(k, w["nseeds"]), params)
+            st_list = [E.stats()[0]]
+            if k > 0:
+                r = E.collect()
+                st_list.append(E.stats()[1])
+            if k == steps - 1:
+                r = E.collect()
+                st_list.append(E.stats()[1])
+        else:
+            r = E.evaluate(w["pop"], seeds_for(k, w["nseeds"]), params)
+            st_list = E.stats()
+        for i, st in enumerate(st_list):
+            fsteps += st["flight_steps"]
+            for b in st["buckets"]:
+                key = (b["L"], b["P"], b["tail"])
+                acc = bucket.setdefault(key, {"flight_steps": 0, "panel_steps": 0, "fly_ms": 0.0, "launches": 0,
+                                              "fused": b["fused"]})
+                for f in ("flight_steps", "panel_steps", "fly_ms", "launches"):
+                    acc[f] += b[f]
+            if i > 0 and pipelined:
+                continue  # a holdout batch overlapping other work: its launch times include sharing
+            fused["ms"] += st["fused_ms"]
+            fused["launches"] += st["fused_launches"]
+            fused_graph["ms"] += st["fly_ms"]
+            fused_graph["launches"] += 1
+            fused_graph["flight_steps"] += st["flight_steps"]
+            fused_graph["panel_steps"] += st["panel_steps"]
+            for b in st["buckets"]:
+                if b["fused"]:
+                    fused["flight_steps"] += b["flight_steps"]
+                    fused["panel_steps"] += b["panel_steps"]
+            # small chunks: k_fly_split, timed inside the fused kernel's event bracket
+            fused["flight_steps"] += st["split_flight_steps"]
+            fused["panel_steps"] += st["split_panel_steps"]
+            fused["split_steps"] = fused.get("split_steps", 0) + st["split_flight_steps"]
+    PA.LIB.pr_device_sync()
+    wall = time.perf_counter() - t0
+    # the last timed generation's distances for its first genomes, for the FP64 check below
+    # (still on the device; read after the timed region)
+    timed_samples = E.read_samples(0, min(256, w["pop"]), w["nseeds"])
+    timed_seeds = seeds_for(steps - 1, w["nseeds"])
+    pfs = fsteps / 600.0
+    value = pfs / wall
+    ms_per_step = 1000.0 * wall / steps
+    log(f"{steps} generations in {wall:.3f} s, {pfs:.0f} plane-flight-s, last holdout {r['holdout']:.3f}")
+
+    # dominant kernel: the fused flight kernel (one launch per 1-s chunk) unless every
+    # bucket was flown by its own launches; algorithmic FLOPs per launch / its HIP-event
+    # timed average duration on the context stream. Pipelined runs time the population
+    # launches only (the holdout batches run concurrently on their own stream).
+    if args.graph:
+        # replayed graphs carry no per-launch events: the graph's own events time the whole
+        # flight sequence of each population evaluation (its k_init and list compactions
+        # are < 0.2 % of it)
+        name, fl, ms, nl = "k_fly_all (graph replays)", fused_graph, fused_graph["ms"], fused_graph["launches"]
+    elif fused["launches"]:
+        name, fl, ms, nl = "k_fly_all", fused, fused["ms"], fused["launches"]
+        if fused.get("split_steps"):
+            name += " + k_fly_split (small chunks)"
+    else:
+        key, fl = max(bucket.items(), key=lambda kv: kv[1]["fly_ms"])
+        name, ms, nl = f"k_fly<{key[0]},{key[1]}>" + (" (tail mapping)" if key[2] else ""), fl["fly_ms"], fl["launches"]
+    flops = FLOP_PANEL * fl["panel_steps"] + FLOP_PLANE * fl["flight_steps"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    if pipelined and fused["launches"]:
+        name += " (population flights)"
+    roof = {"bound": "valu", "kernel": name, "achieved": round(achieved, 3),
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "traffic": None, "avg_launch_ms": round(ms / max(1, nl), 4), "launches": nl,
+            "flops_per_launch": round(flops / max(1, nl))}
+    if name.startswith("k_fly_all") and wname in TRAFFIC:
+        roof["traffic"], roof["traffic_source"] = TRAFFIC[wname]
+    log(f"  k_fly_all: {fused['launches']} launches, {fused['ms']:.2f} ms")
+    last = E.stats()[0]
+    for k, ch in enumerate(last["chunks"]):
+        fl = FLOP_PANEL * ch["panel_steps"] + FLOP_PLANE * ch["flight_steps"]
+        rate = f"{fl / ch['ms'] / 1e9:.1f} TFLOP/s" if ch["ms"] > 0 else "untimed (graph replay)"
+        log(f"  last generation chunk {k}: {ch['flights']} flights, {ch['flight_steps']} flight-steps, "
+            f"{ch['ms']:.2f} ms, {rate}")
+    for kk, bb in sorted(bucket.items()):
+        log(f"  {'(fused) ' if bb['fused'] else ''}k_fly<{kk[0]},{kk[1]}>{' tail' if kk[2] else ''}: "
+            f"{bb['launches']} own launches, {bb['fly_ms']:.2f} ms, "
+            f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
+
+    cpu = None
+    f64 = fp64_check(genomes, res, w, timed_samples, timed_seeds)
+    log(f"FP64 check: {f64}")
+    if cpu_seconds > 0:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(genomes, res, w, cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.1f} pfs/s on {cpu['cores']} threads")
+
+    E.close()
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{wname.upper()}: {w['desc']}", "pop": w["pop"], "seeds": w["nseeds"],
+                      "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU",
+                      "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table",
+                      "launch": "hipGraph replay per evaluation" if args.graph else "direct launches"},
+           "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
+           "roofline": roof, "cpu_baseline": cpu, "fp64_check": f64}
+    return out
+
+
+def generation_options(**kw):
+    """The c1-c3 run options of the command line, at their defaults unless given."""
+    o = dict(unfused=False, list_order="size", no_split=False, split_holdout=False, graph=None, tail_mode=None,
+             buckets=None, autotune=None, no_pipeline=False)
+    o.update(kw)
+    return argparse.Namespace(**o)
+
+
+def other_workloads(PA):
+    """C2 and C5 measured in the same process after the C3 line's own work, at their default
+    settings (C2: 10 timed generations after 2 warm-up ones, graph replays; C5: 2 timed audits
+    after 1), each with its dominant-kernel roofline and an FP64 check of its own timed results."""
+    out = {}
+    t0 = time.perf_counter()
+    c2 = generation_run(PA, "c2", generation_options(), 10, 2, 0)
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "gens_per_hour_eval", "roofline", "fp64_check")
+    out["c2"] = {k: c2[k] for k in keep}
+    out["c2"]["workload"] = c2["config"]["workload"]
+    out["c2"]["wall_s"] = round(time.perf_counter() - t0, 1)
+    t0 = time.perf_counter()
+    c5 = audit_run(PA, 2, 1, 0)
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "audits_per_hour", "roofline", "fp64_check",
+            "surrogate_ranking")
+    out["c5"] = {k: c5[k] for k in keep}
+    out["c5"]["workload"] = c5["config"]["workload"]
+    out["c5"]["wall_s"] = round(time.perf_counter() - t0, 1)
+    log(f"other workloads: C2 {out['c2']['value']:.0f} pfs/s, C5 {out['c5']['value']:.0f} pfs/s")
+    return out
 
 
 def main():
@@ -539,6 +740,9 @@ def main():
     ap.add_argument("--unfused", action="store_true", help="one flight launch per bucket instead of the fused kernel")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="c1-c3: finish each generation's holdout before the next generation starts")
+    ap.add_argument("--other-workloads", action=argparse.BooleanOptionalAction, default=None,
+                    help="c3: after the surrogate A/B, C2 (10 timed generations) and C5 (2 timed audits) in the "
+                         "same line, each with its roofline and FP64 check (default on for c3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout "
@@ -553,155 +757,8 @@ def main():
     if args.workload == "c5":
         log("device:", PA.device_name())
         return audit_bench(PA, args)
-    w = WORKLOADS[args.workload]
     log("device:", PA.device_name())
-    res = None
-    if w.get("families"):
-        genomes, res, _ = PA.family_population(w["pop"], seed=0)
-    else:
-        lo, hi = PA.gene_bounds()
-        rng = np.random.default_rng(0)
-        genomes = (lo + rng.random((w["pop"], 35), dtype=np.float32) * (hi - lo)).astype(np.float32)
-    params = PA.SimParams.config(gust_rms_max=w["gust"], warp_max_deg=w["warp_deg"], wind_sd=w["wind_sd"],
-                                 max_steps=w["max_steps"])
-    max_ch = int(res[:, 0].max()) if res is not None else 0
-    max_sp = int(res[:, 1].max()) if res is not None else 0
-    E = PA.Generation(w["pop"], w["nseeds"], max_ch, max_sp)
-    E.upload(genomes, res)
-    if args.unfused:
-        E.set_fused(False)
-    if args.list_order != "size":
-        E.set_list_order(False)
-    if args.no_split:
-        E.set_split(False)
-    elif args.split_holdout:
-        E.set_split(True, holdout=True)
-    if args.graph is None:
-        args.graph = w.get("graph", False)
-    if args.graph:
-        E.set_graph(True)
-    if args.tail_mode is not None:
-        E.set_tail_mapping(args.tail_mode)
-    tuned = None
-    if args.buckets:
-        spec = args.buckets if args.buckets.lstrip().startswith("{") else open(args.buckets).read()
-        b = json.loads(spec)
-        tuned = (list(b["nt_max"]), list(b["L"]), list(b["P"]))
-        E.set_buckets(*tuned)
-    elif args.autotune if args.autotune is not None else w.get("autotune", False):
-        # one-time setup outside the timed region: the mapping per triangle-count bucket from
-        # timing every compiled (L, P) on a sample of this population's flights
-        t0 = time.perf_counter()
-        tuned = PA.autotune(genomes, res, seeds_for(20000, w["nseeds"]), params)
-        E.set_buckets(*tuned)
-        log(f"autotuned buckets {list(zip(*tuned))} in {time.perf_counter() - t0:.1f} s")
-
-    for k in range(args.warmup):
-        E.evaluate(w["pop"], seeds_for(10000 + k, w["nseeds"]), params)
-    PA.LIB.pr_device_sync()
-    fsteps = 0
-    bucket = {}
-    fused = {"flight_steps": 0, "panel_steps": 0, "ms": 0.0, "launches": 0}
-    fused_graph = {"flight_steps": 0, "panel_steps": 0, "ms": 0.0, "launches": 0}
-    t0 = time.perf_counter()
-    pipelined = not args.no_pipeline
-    for k in range(args.steps):
-        if pipelined:
-            # generation k's holdout flies on its own stream while generation k+1's
-            # population flies; the last holdout is collected inside the timed region
-            E.evaluate_async(w["pop"], seeds_for(k, w["nseeds"]), params)
-            st_list = [E.stats()[0]]
-            if k > 0:
-                r = E.collect()
-                st_list.append(E.stats()[1])
-            if k == args.steps - 1:
-                r = E.collect()
-                st_list.append(E.stats()[1])
-        else:
-            r = E.evaluate(w["pop"], seeds_for(k, w["nseeds"]), params)
-            st_list = E.stats()
-        for i, st in enumerate(st_list):
-            fsteps += st["flight_steps"]
-            for b in st["buckets"]:
-                key = (b["L"], b["P"], b["tail"])
-                acc = bucket.setdefault(key, {"flight_steps": 0, "panel_steps": 0, "fly_ms": 0.0, "launches": 0,
-                                              "fused": b["fused"]})
-                for f in ("flight_steps", "panel_steps", "fly_ms", "launches"):
-                    acc[f] += b[f]
-            if i > 0 and pipelined:
-                continue  # a holdout batch overlapping other work: its launch times include sharing
-            fused["ms"] += st["fused_ms"]
-            fused["launches"] += st["fused_launches"]
-            fused_graph["ms"] += st["fly_ms"]
-            fused_graph["launches"] += 1
-            fused_graph["flight_steps"] += st["flight_steps"]
-            fused_graph["panel_steps"] += st["panel_steps"]
-            for b in st["buckets"]:
-                if b["fused"]:
-                    fused["flight_steps"] += b["flight_steps"]
-                    fused["panel_steps"] += b["panel_steps"]
-            # small chunks: k_fly_split, timed inside the fused kernel's event bracket
-            fused["flight_steps"] += st["split_flight_steps"]
-            fused["panel_steps"] += st["split_panel_steps"]
-            fused["split_steps"] = fused.get("split_steps", 0) + st["split_flight_steps"]
-    PA.LIB.pr_device_sync()
-    wall = time.perf_counter() - t0
-    # the last timed generation's distances for its first genomes, for the FP64 check below
-    # (still on the device; read after the timed region)
-    timed_samples = E.read_samples(0, min(256, w["pop"]), w["nseeds"])
-    timed_seeds = seeds_for(args.steps - 1, w["nseeds"])
-    pfs = fsteps / 600.0
-    value = pfs / wall
-    ms_per_step = 1000.0 * wall / args.steps
-    log(f"{args.steps} generations in {wall:.3f} s, {pfs:.0f} plane-flight-s, last holdout {r['holdout']:.3f}")
-
-    # dominant kernel: the fused flight kernel (one launch per 1-s chunk) unless every
-    # bucket was flown by its own launches; algorithmic FLOPs per launch / its HIP-event
-    # timed average duration on the context stream. Pipelined runs time the population
-    # launches only (the holdout batches run concurrently on their own stream).
-    if args.graph:
-        # replayed graphs carry no per-launch events: the graph's own events time the whole
-        # flight sequence of each population evaluation (its k_init and list compactions
-        # are < 0.2 % of it)
-        name, fl, ms, nl = "k_fly_all (graph replays)", fused_graph, fused_graph["ms"], fused_graph["launches"]
-    elif fused["launches"]:
-        name, fl, ms, nl = "k_fly_all", fused, fused["ms"], fused["launches"]
-        if fused.get("split_steps"):
-            name += " + k_fly_split (small chunks)"
-    else:
-        key, fl = max(bucket.items(), key=lambda kv: kv[1]["fly_ms"])
-        name, ms, nl = f"k_fly<{key[0]},{key[1]}>" + (" (tail mapping)" if key[2] else ""), fl["fly_ms"], fl["launches"]
-    flops = FLOP_PANEL * fl["panel_steps"] + FLOP_PLANE * fl["flight_steps"]
-    achieved = flops / (ms * 1e-3) / 1e12
-    if pipelined and fused["launches"]:
-        name += " (population flights)"
-    roof = {"bound": "valu", "kernel": name, "achieved": round(achieved, 3),
-            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None, "avg_launch_ms": round(ms / max(1, nl), 4), "launches": nl,
-            "flops_per_launch": round(flops / max(1, nl))}
-    if name.startswith("k_fly_all") and args.workload in TRAFFIC:
-        roof["traffic"], roof["traffic_source"] = TRAFFIC[args.workload]
-    log(f"  k_fly_all: {fused['launches']} launches, {fused['ms']:.2f} ms")
-    last = E.stats()[0]
-    for k, ch in enumerate(last["chunks"]):
-        fl = FLOP_PANEL * ch["panel_steps"] + FLOP_PLANE * ch["flight_steps"]
-        rate = f"{fl / ch['ms'] / 1e9:.1f} TFLOP/s" if ch["ms"] > 0 else "untimed (graph replay)"
-        log(f"  last generation chunk {k}: {ch['flights']} flights, {ch['flight_steps']} flight-steps, "
-            f"{ch['ms']:.2f} ms, {rate}")
-    for kk, bb in sorted(bucket.items()):
-        log(f"  {'(fused) ' if bb['fused'] else ''}k_fly<{kk[0]},{kk[1]}>{' tail' if kk[2] else ''}: "
-            f"{bb['launches']} own launches, {bb['fly_ms']:.2f} ms, "
-            f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
-
-    cpu = None
-    f64 = None
-    if args.cpu_seconds > 0:
-        log("cpu baseline ...")
-        cpu = cpu_baseline(genomes, res, w, args.cpu_seconds)
-        log(f"cpu baseline {cpu['value']:.1f} pfs/s on {cpu['cores']} threads")
-        f64 = fp64_check(genomes, res, w, timed_samples, timed_seeds)
-        log(f"FP64 check: {f64}")
-
+    out = generation_run(PA, args.workload, args, args.steps, args.warmup, args.cpu_seconds)
     ab = None
     if args.surrogate_ab if args.surrogate_ab is not None else args.workload == "c3":
         t0 = time.perf_counter()
@@ -710,18 +767,10 @@ def main():
         ab.pop("_pretrained_state")
         ab["wall_s"] = round(time.perf_counter() - t0, 1)
         log(f"surrogate A/B: {json.dumps(ab['arms'].get('surrogate', {}))[:400]}")
-
-    out = {"metric": METRIC, "value": round(value, 2), "unit": "plane-flight-s/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"{args.workload.upper()}: {w['desc']}", "pop": w["pop"], "seeds": w["nseeds"],
-                      "max_steps": w["max_steps"], "holdout": "top-8 x 128 scenarios", "parallelism": "single GPU",
-                      "buckets": [list(b) for b in zip(*tuned)] if tuned else "default table",
-                      "launch": "hipGraph replay per evaluation" if args.graph else "direct launches"},
-           "gens_per_hour_eval": round(3600.0 / (ms_per_step / 1000.0), 1),
-           "roofline": roof, "cpu_baseline": cpu, "fp64_check": f64}
     if ab is not None:
         out["surrogate_ab"] = ab
+    if args.other_workloads if args.other_workloads is not None else args.workload == "c3":
+        out["other_workloads"] = other_workloads(PA)
     print(json.dumps(out), flush=True)
 
 

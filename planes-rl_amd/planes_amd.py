@@ -582,9 +582,15 @@ class Generation:
         _ck(LIB.pr_gen_collect(self.h, C.byref(hold), C.byref(champ)), "pr_gen_collect")
         return {"holdout": hold.value, "champion": champ.value}
 
+    def close(self):
+        """Frees the generation's device buffers now (the destructor does it otherwise)."""
+        if getattr(self, "h", None):
+            LIB.pr_gen_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            LIB.pr_gen_destroy(self.h)
+            self.close()
         except Exception:
             pass
 
