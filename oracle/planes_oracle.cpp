@@ -27,8 +27,12 @@
 //     conservation without torque, flat-plate special cases, torque = r x F, normal-flip
 //     invariance) and FP32 <-> FP64 convergence.
 //   robust_scores, ga_rank, ga2_breed: hand vectors and the reference's score weights.
-//   scenario_cfg, lane_normal: PARITY UNPINNED against the reference (no counterpart
-//     there); checked only by definition (Box-Muller moments) and CPU <-> GPU bit-exact.
+//   scenario_cfg, lane_normal: no counterpart in the reference (the configurations'
+//     scenario preset).  The fields drawn as in the reference's scenario_desc (speed, skin
+//     friction, grip offsets, print biases, infill) bit-exact against the reference's
+//     scenario vectors; the rest by the distributions the configurations state (KS and
+//     moment tests per field, pairwise independence of all 26, the gust-field RMS constant);
+//     planted slips in both fail those tests.
 #include <algorithm>
 #include <atomic>
 #include <chrono>

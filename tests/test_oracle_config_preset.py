@@ -1,8 +1,10 @@
-"""The parametric scenario preset (scenario_cfg, lane_normal) has no counterpart in the
-reference, so nothing pins it bit-for-bit against the reference: parity unpinned.  These
-tests pin it by its definition instead -- the distributions each field is documented to
-follow, and the gust-field RMS constant that scales the gust amplitude -- chosen so that a
-wrong lane, scale, offset or dropped sinusoid fails one of them."""
+"""The parametric scenario preset (scenario_cfg, lane_normal) of the configurations.  The
+reference holds no such preset, so it is pinned in two ways: the fields it shares with the
+reference's scenario_desc (common.h:389-421: the same hash lanes and the same formulas for
+speed, skin friction, grip offsets, print biases and infill) against the reference's own
+scenario vectors, bit for bit; the fields the configurations redefine by the distributions
+they state, and the gust-field RMS constant that scales the gust amplitude -- chosen so that
+a wrong lane, scale, offset or dropped sinusoid fails one of them."""
 import numpy as np
 import pytest
 from scipy import stats
@@ -13,6 +15,8 @@ S_SPEED, S_PITCH, S_YAW, S_ROLL, S_HEIGHT, S_RR, S_PR, S_YR, S_RHO, S_CF, S_GUST
 S_BASE_X, S_BASE_Y, S_FPITCH, S_FYAW = 11, 12, 13, 14
 S_MATERIAL, S_INFILL, S_WARP_P, S_WARP_R, S_BASE_Z = 18, 19, 23, 24, 25
 DEG = 0.0174532925
+# reference ScenarioDesc field order (common.h:377-387)
+R_SPEED, R_CF, R_GRIP_X, R_GRIP_Y, R_GRIP_Z, R_INFILL, R_BIAS_X, R_BIAS_Y, R_BIAS_Z = 0, 9, 15, 16, 17, 19, 20, 21, 22
 
 
 def test_lane_normal_is_standard_normal():
@@ -59,8 +63,9 @@ def test_config_preset_field_distributions():
     uniform_rms = d[:, S_GUST] * 0.69191763
     assert uniform_rms.max() <= gust + 1e-5
     assert stats.kstest(uniform_rms / gust, "uniform").pvalue > 1e-3
-    # independent lanes: no two fields drawn from the same hash lane
-    c = np.corrcoef(d[:, [S_SPEED, S_HEIGHT, S_RHO, S_PITCH, S_YAW, S_BASE_X, S_BASE_Y, S_BASE_Z, S_GUST]].T)
+    # independent lanes: no two of the 26 fields drawn from the same hash lane (40 000 seeds:
+    # a correlation of independent fields has a standard deviation of 0.005)
+    c = np.corrcoef(d.T)
     assert np.abs(c - np.eye(len(c))).max() < 0.03
 
 
@@ -83,3 +88,19 @@ def test_gust_field_rms_constant():
     w2 = np.array([.55, .35, .25, .50, .30, .30, .20]) ** 2
     assert abs(np.sqrt(w2.sum() / 2) - 0.69191763) < 1e-7
     assert rms == pytest.approx(0.69191763, rel=0.015)
+
+
+def test_fields_shared_with_the_reference_preset_are_bit_exact():
+    """Speed, skin friction (two lanes), grip offsets, print biases and infill: drawn from the
+    reference's hash lanes by the reference's formulas (common.h:391,400,408-417), so equal to
+    the reference's scenario_desc fields bit for bit, at every gust / warp / wind setting."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_scenario.npz"), allow_pickle=False)
+    seeds = ref["seeds"].astype(np.int32)
+    pairs = [(S_SPEED, R_SPEED), (S_CF, R_CF), (S_INFILL, R_INFILL)]
+    pairs += [(15 + k, R_GRIP_X + k) for k in range(3)]         # grip x, y, z
+    pairs += [(20 + k, R_BIAS_X + k) for k in range(3)]         # print bias x, y, z
+    for gust, warp, wsd in ((4.0, 1.0 * DEG, 2.0), (5.0, 0.5 * DEG, 0.0)):
+        d = O.scenarios(seeds, preset=1, gust_rms_max=gust, warp_max=warp, wind_sd=wsd)
+        for cfg_col, ref_col in pairs:
+            assert np.array_equal(d[:, cfg_col], ref["desc"][:, ref_col]), (cfg_col, ref_col)

@@ -1,7 +1,8 @@
 """Planted mistakes in the oracle (on the CPU): each mutant below is the oracle's source with
 one plausible slip -- a sign, an operand order, a transposed rotation, a rate, a scale, a
-range, an index -- compiled as its own library; the pins of tests/test_oracle_pins.py that
-should catch it are run against that build and must fail.  An unmodified build compiled
+range, an index, a shared lane -- compiled as its own library; the pins that should catch it
+(tests/test_oracle_pins.py, or tests/test_oracle_config_preset.py for the configurations'
+scenario preset) are run against that build and must fail.  An unmodified build compiled
 the same way passes them (the control)."""
 import concurrent.futures as cf
 import os
@@ -14,6 +15,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "planes_oracle.cpp")
 PINS = os.path.join(ROOT, "tests", "test_oracle_pins.py")
+CFG = os.path.join(ROOT, "tests", "test_oracle_config_preset.py")
+CFG_FN = "static void scenario_cfg("  # config-preset mistakes are planted in its body
+BM_FN = "static float lane_normal("
 
 GA2 = "void or_ga2_breed"  # breeding mistakes are planted in the persistent-distribution GA only
 
@@ -49,6 +53,21 @@ MUTANTS = {
                             "int a = base + i * (rs.ch + 1) + j, b = a + rs.ch;", None, "topology"),
     "mirror_sign": ("return mk<R>(x, (u < 0 ? -y : y), zc + zoff);", "return mk<R>(x, (u < 0 ? y : y), zc + zoff);",
                     None, "mirror"),
+    # the configurations' scenario preset: tests/test_oracle_config_preset.py
+    "cfg_speed_scale": ("9.2f + 2.0f * lane_pm1(seed, 0)", "9.2f + 2.5f * lane_pm1(seed, 0)", CFG_FN, "", CFG),
+    "cfg_pitch_nominal": ("0.10f + 5.0f * deg * lane_normal(seed, 32)", "5.0f * deg * lane_normal(seed, 32)", CFG_FN,
+                          "", CFG),
+    "cfg_rates_in_degrees": ("30.0f * deg * lane_normal(seed, 40)", "30.0f * lane_normal(seed, 40)", CFG_FN, "", CFG),
+    "cfg_gust_unscaled": ("lane_u01(seed, 23) * (1.0f / 0.69191763f)", "lane_u01(seed, 23)", CFG_FN, "", CFG),
+    "cfg_wind_sd": ("cf.wind_sd * lane_normal(seed, 46)", "0.5f * cf.wind_sd * lane_normal(seed, 46)", CFG_FN, "",
+                    CFG),
+    "cfg_grip_lane": ("0.022f * lane_pm1(seed, 13)", "0.022f * lane_pm1(seed, 12)", CFG_FN, "", CFG),
+    "cfg_cf_factor_dropped": ("d[S_CF] *= 0.90f + 0.25f * lane_u01(seed, 20);", "", CFG_FN, "", CFG),
+    "cfg_warp_one_sided": ("cf.warp_max * lane_pm1(seed, 21)", "cf.warp_max * lane_u01(seed, 21)", CFG_FN, "", CFG),
+    "cfg_shared_lane": ("d[S_FYAW] = 5.0f * deg * lane_normal(seed, 50);", "d[S_FYAW] = 5.0f * deg * lane_normal(seed, 34);",
+                        CFG_FN, "", CFG),
+    "box_muller_radius": ("std::sqrt(-2.0 * std::log(u1))", "std::sqrt(-std::log(u1))", BM_FN, "", CFG),
+    "box_muller_angle_lane": ("std::cos(6.283185307179586 * u2)", "std::cos(6.283185307179586 * u1)", BM_FN, "", CFG),
 }
 
 
@@ -68,10 +87,10 @@ def build(src_text, out):
     return out
 
 
-def run_pins(so, kexpr):
+def run_pins(so, kexpr, path=PINS):
     env = dict(os.environ, PLANES_ORACLE_SO=so)
-    r = subprocess.run([sys.executable, "-m", "pytest", PINS, "-q", "-x", "-p", "no:cacheprovider", "-k", kexpr],
-                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
+    cmd = [sys.executable, "-m", "pytest", path, "-q", "-x", "-p", "no:cacheprovider"] + (["-k", kexpr] if kexpr else [])
+    r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
     return r.returncode, r.stdout[-2000:]
 
 
@@ -79,16 +98,18 @@ def run_pins(so, kexpr):
 def test_planted_mistakes_are_caught(tmp_path):
     src = open(SRC).read()
     jobs = {"control": (src, "attitude or grip or clamp or grand or elite or identical_parents or tournament or "
-                             "refinement or topology or mirror")}
-    for name, (old, new, marker, kexpr) in MUTANTS.items():
+                             "refinement or topology or mirror", PINS),
+            "control_cfg": (src, "", CFG)}
+    for name, (old, new, marker, kexpr, *path) in MUTANTS.items():
         m = mutate(src, old, new, marker)
         assert m != src, name
-        jobs[name] = (m, kexpr)
+        jobs[name] = (m, kexpr, path[0] if path else PINS)
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        builds = {n: ex.submit(build, t, str(tmp_path / f"{n}.so")) for n, (t, _) in jobs.items()}
-        runs = {n: ex.submit(lambda n=n: run_pins(builds[n].result(), jobs[n][1])) for n in jobs}
+        builds = {n: ex.submit(build, t, str(tmp_path / f"{n}.so")) for n, (t, _, _) in jobs.items()}
+        runs = {n: ex.submit(lambda n=n: run_pins(builds[n].result(), jobs[n][1], jobs[n][2])) for n in jobs}
         res = {n: f.result() for n, f in runs.items()}
-    rc, out = res.pop("control")
-    assert rc == 0, out
+    for control in ("control", "control_cfg"):
+        rc, out = res.pop(control)
+        assert rc == 0, out
     survivors = [n for n, (rc, _) in res.items() if rc != 1]
     assert not survivors, {n: res[n][1] for n in survivors}
