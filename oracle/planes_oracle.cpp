@@ -1012,19 +1012,10 @@ void or_robust_scores(const float* samples, int pop, int nseeds, float* scores) 
     }
 }
 
-// GA state: mt19937 serialized as text (libstdc++ operator<<)
-struct OrGA { std::mt19937 rng; };
-void* or_ga_new(unsigned seed) { OrGA* g = new OrGA; g->rng.seed(seed); return g; }
-void or_ga_free(void* h) { delete (OrGA*)h; }
-
 static void clampset(float* g, int i, float v) { g[i] = std::fmin(HI[i], std::fmax(LO[i], v)); }
 static void random_genome(float* g, std::mt19937& rng) {  // sim.cu:318-321
     std::uniform_real_distribution<float> u(0, 1);
     for (int i = 0; i < N_GENES; i++) clampset(g, i, LO[i] + u(rng) * (HI[i] - LO[i]));
-}
-void or_ga_random(void* h, float* genomes, int n) {
-    OrGA* g = (OrGA*)h;
-    for (int i = 0; i < n; i++) random_genome(genomes + (size_t)N_GENES * i, g->rng);
 }
 void or_ga_sanitize(float* genomes, int n) {
     for (int i = 0; i < n; i++) for (int j = 0; j < N_GENES; j++) clampset(genomes + (size_t)N_GENES * i, j, genomes[(size_t)N_GENES * i + j]);
@@ -1034,38 +1025,10 @@ void or_ga_rank(const float* acc, int pop, int* idx) {
     for (int i = 0; i < pop; i++) idx[i] = i;
     std::sort(idx, idx + pop, [&](int a, int b) { return acc[a] > acc[b]; });
 }
-// breeding (sim.cu:472-490): gs sorted by idx rank -> next
-void or_ga_breed(void* h, const float* gs, const float* acc, const int* idx, int pop, int gen, int gens, float* next) {
-    OrGA* G = (OrGA*)h;
-    std::mt19937& rng = G->rng;
-    std::uniform_real_distribution<float> u01(0, 1);
-    std::normal_distribution<float> nrm(0, 1);
-    int elite = pop / 64;
-    float anneal = 1.0f - 0.6f * gen / gens;
-    for (int i = 0; i < elite; i++) std::memcpy(next + (size_t)N_GENES * i, gs + (size_t)N_GENES * idx[i], 4 * N_GENES);
-    for (int i = elite; i < pop; i++) {
-        float* out = next + (size_t)N_GENES * i;
-        if (u01(rng) < 0.05f) { random_genome(out, rng); continue; }
-        auto tourn = [&]() {
-            int b = idx[(int)(u01(rng) * pop)];
-            for (int k = 0; k < 3; k++) { int c = (int)(u01(rng) * pop); if (acc[c] > acc[b]) b = c; }
-            return b;
-        };
-        int pa = tourn();
-        int pc = tourn();
-        for (int j = 0; j < N_GENES; j++) {
-            float a = -0.1f + 1.2f * u01(rng);
-            float v = gs[(size_t)N_GENES * pa + j] * a + gs[(size_t)N_GENES * pc + j] * (1 - a);
-            if (u01(rng) < 0.25f) v += nrm(rng) * 0.08f * (HI[j] - LO[j]) * anneal;
-            clampset(out, j, v);
-        }
-    }
-}
-// Note: std::normal_distribution<float> caches its second Marsaglia-polar draw inside the
-// distribution object; the reference keeps one distribution object for the whole run
-// (sim.cu:430).  or_ga_breed creates it per generation, so a cached value would be lost
-// at the generation boundary; the product keeps it across generations like the reference.
-// The oracle exposes the persistent variant below for exact GA-sequence parity.
+// GA state and breeding (sim.cu:398, 429-430, 472-490): gs sorted by idx rank -> next.  The
+// engine, the uniform and the normal distribution objects persist across generations as in
+// the reference (sim.cu:398, 429-430): std::normal_distribution<float> caches its second
+// Marsaglia-polar draw inside the object, so a per-generation object would lose it.
 struct OrGA2 { std::mt19937 rng; std::uniform_real_distribution<float> u01{0, 1}; std::normal_distribution<float> nrm{0, 1}; };
 void* or_ga2_new(unsigned seed) { OrGA2* g = new OrGA2; g->rng.seed(seed); return g; }
 void or_ga2_free(void* h) { delete (OrGA2*)h; }

@@ -192,7 +192,7 @@ def mt19937(seed, n):
 
 def ga_breed(gs, acc, idx, gen, gens, seed=42, state=None):
     """One breeding pass (sim.cu:472-490) with the reference's persistent distributions; state:
-    a handle from ga_new() to continue a stream, else a fresh mt19937(seed)."""
+    a handle from lib().or_ga2_new(seed) to continue a stream, else a fresh mt19937(seed)."""
     h = state if state is not None else lib().or_ga2_new(seed)
     gs = np.ascontiguousarray(gs, np.float32)
     pop = len(gs)
