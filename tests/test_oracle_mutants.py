@@ -1,8 +1,9 @@
 """Planted mistakes in the oracle (on the CPU): each mutant below is the oracle's source with
 one plausible slip -- a sign, an operand order, a transposed rotation, a rate, a scale, a
 range, an index, a shared lane -- compiled as its own library; the pins that should catch it
-(tests/test_oracle_pins.py, or tests/test_oracle_config_preset.py for the configurations'
-scenario preset) are run against that build and must fail.  An unmodified build compiled
+(tests/test_oracle_pins.py; tests/test_oracle_config_preset.py for the configurations'
+scenario preset; tests/test_oracle_physics.py for the pose recorder) are run against that
+build and must fail.  An unmodified build compiled
 the same way passes them (the control)."""
 import concurrent.futures as cf
 import os
@@ -18,6 +19,7 @@ PINS = os.path.join(ROOT, "tests", "test_oracle_pins.py")
 CFG = os.path.join(ROOT, "tests", "test_oracle_config_preset.py")
 CFG_FN = "static void scenario_cfg("  # config-preset mistakes are planted in its body
 BM_FN = "static float lane_normal("
+PHYS = os.path.join(ROOT, "tests", "test_oracle_physics.py")
 
 GA2 = "void or_ga2_breed"  # breeding mistakes are planted in the persistent-distribution GA only
 
@@ -68,6 +70,12 @@ MUTANTS = {
                         CFG_FN, "", CFG),
     "box_muller_radius": ("std::sqrt(-2.0 * std::log(u1))", "std::sqrt(-std::log(u1))", BM_FN, "", CFG),
     "box_muller_angle_lane": ("std::cos(6.283185307179586 * u2)", "std::cos(6.283185307179586 * u1)", BM_FN, "", CFG),
+    # pose recording (sim.cu:117-128): tests/test_oracle_physics.py
+    "rec_index_ceil": ("int rec_i = step0 / rec_every;", "int rec_i = (step0 + rec_every - 1) / rec_every;", None,
+                       "pose_recording", PHYS),
+    "rec_alive_inverted": ("r[7] = s.done ? 0.0f : 1.0f;", "r[7] = s.done ? 1.0f : 0.0f;", None, "pose_recording", PHYS),
+    "rec_one_step_late": ("if (rec && st % rec_every == 0 && rec_i < rec_cap)",
+                          "if (rec && (st + 1) % rec_every == 0 && rec_i < rec_cap)", None, "pose_recording", PHYS),
 }
 
 
@@ -99,7 +107,8 @@ def test_planted_mistakes_are_caught(tmp_path):
     src = open(SRC).read()
     jobs = {"control": (src, "attitude or grip or clamp or grand or elite or identical_parents or tournament or "
                              "refinement or topology or mirror", PINS),
-            "control_cfg": (src, "", CFG)}
+            "control_cfg": (src, "", CFG),
+            "control_phys": (src, "pose_recording", PHYS)}
     for name, (old, new, marker, kexpr, *path) in MUTANTS.items():
         m = mutate(src, old, new, marker)
         assert m != src, name
@@ -108,7 +117,7 @@ def test_planted_mistakes_are_caught(tmp_path):
         builds = {n: ex.submit(build, t, str(tmp_path / f"{n}.so")) for n, (t, _, _) in jobs.items()}
         runs = {n: ex.submit(lambda n=n: run_pins(builds[n].result(), jobs[n][1], jobs[n][2])) for n in jobs}
         res = {n: f.result() for n, f in runs.items()}
-    for control in ("control", "control_cfg"):
+    for control in ("control", "control_cfg", "control_phys"):
         rc, out = res.pop(control)
         assert rc == 0, out
     survivors = [n for n, (rc, _) in res.items() if rc != 1]

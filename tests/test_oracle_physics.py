@@ -179,3 +179,34 @@ def test_champion_stl_topology():
             edges[(min(a, b), max(a, b))] += 1
     boundary = sum(1 for c in edges.values() if c == 1)
     assert 2 * len(T) + 2 * boundary == 500
+
+
+def test_pose_recording_selects_the_reference_steps():
+    """sim.cu:117-128: before step s a frame {pos, q, alive} is written when s % rec_every == 0,
+    at index rec_i, which restarts at step0 / rec_every at every 600-step chunk (sim.cu:117; so
+    with rec_every = 7 chunk 1's first frame, step 602, overwrites chunk 0's last, step 595) and
+    grows by one per frame; after the flight ends the frames keep its final pose with alive = 0.
+    The recorder's frames are compared with the oracle's separate trajectory loop (the state
+    after every step) at the steps that rule selects."""
+    steps, cap = 1800, 400
+    for g, seed in ((PRESETS[1], 42), (PRESETS[0], 3)):
+        tr = O.trajectory(g, seed, steps, 32, max_steps=steps)
+        st0 = O.launch(g, seed, prec=32)[0]
+        n = len(tr)
+        for r in (6, 7):
+            _, rec = O.record(g, seed, max_steps=steps, rec_every=r, rec_cap=cap)
+            frame_step = {}
+            for step0 in range(0, steps, 600):
+                i = step0 // r
+                for s in range(step0, min(step0 + 600, steps)):
+                    if s % r == 0 and i < cap:
+                        frame_step[i] = s
+                        i += 1
+            for i, s in frame_step.items():
+                state = st0 if s == 0 else tr[min(s, n) - 1]
+                assert np.array_equal(rec[i, :7], state[:7].astype(np.float32)), (seed, r, i, s)
+                assert rec[i, 7] == (1.0 if s < n else 0.0), (seed, r, i, s)
+            assert np.all(rec[max(frame_step) + 1:] == 0)  # slots no step selected stay untouched
+    # the cases cover both flags: one flight ends inside the window, and frames past it are dead
+    ends = [len(O.trajectory(g, s, steps, 32, max_steps=steps)) for g, s in ((PRESETS[1], 42), (PRESETS[0], 3))]
+    assert min(ends) < steps
