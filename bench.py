@@ -384,10 +384,12 @@ def audit_bench(PA, args):
     print(json.dumps(audit_run(PA, args.steps, args.warmup, args.cpu_seconds)), flush=True)
 
 
-def audit_run(PA, steps, warmup, cpu_seconds):
+def audit_run(PA, steps, warmup, cpu_seconds, oracle_check=None):
     """C5: robustness audit of a champion + 1 024 designs on 4 096 scenarios each (seed 9001 +
     s*7919 by scenario), gusts <= 5 m/s RMS, warp +-1 deg, 20 s cap; one step = one batched
-    audit (prepare, scenarios, every flight, per-design statistics on the host)."""
+    audit (prepare, scenarios, every flight, per-design statistics on the host).  The CPU leg
+    (oracle_check, default: when a CPU baseline is asked for) runs the oracle: the FP64 check
+    of the audit champion and the CPU baseline (cpu_seconds > 0)."""
     n_gen, n_scen = 1025, 4096
     genomes, res, _ = PA.family_population(n_gen, seed=9)
     params = PA.SimParams.config(gust_rms_max=5.0, warp_max_deg=1.0, wind_sd=2.0, max_steps=12000)
@@ -412,9 +414,10 @@ def audit_run(PA, steps, warmup, cpu_seconds):
     rank = surrogate_ranking(PA, genomes, res, dist, params)
     log(f"surrogate ranking of the audited designs: {rank}")
     pl.close()
-    cpu = None
-    f64 = audit_fp64_check(genomes, res, best, dist)
-    log(f"FP64 check {f64}")
+    cpu, f64 = None, None
+    if oracle_check if oracle_check is not None else cpu_seconds > 0:
+        f64 = audit_fp64_check(genomes, res, best, dist)
+        log(f"FP64 check {f64}")
     if cpu_seconds > 0:
         wc5 = dict(gust=5.0, warp_deg=1.0, wind_sd=2.0, max_steps=12000, nseeds=n_scen)
         cpu = cpu_baseline(genomes, res, wc5, cpu_seconds)
@@ -486,10 +489,12 @@ def audit_fp64_check(genomes, res, best, dist, n_scen=256):
             "fp64_wall_s": round(wall, 1)}
 
 
-def generation_run(PA, wname, args, steps, warmup, cpu_seconds):
+def generation_run(PA, wname, args, steps, warmup, cpu_seconds, oracle_check=None):
     """C1-C3: `steps` timed GA-generation evaluations (prepare, scenarios, every flight, the
     robust scores, the top-8 x 128 holdout) of one population after `warmup` untimed ones;
-    returns the bench line (without the surrogate A/B)."""
+    returns the bench line (without the surrogate A/B).  The CPU leg after the timed region
+    (oracle_check, default: when a CPU baseline is asked for) runs the oracle: the FP64 check of
+    the timed results and the CPU baseline (cpu_seconds > 0)."""
     w = WORKLOADS[wname]
     res = None
     if w.get("families"):
@@ -629,9 +634,10 @@ def generation_run(PA, wname, args, steps, warmup, cpu_seconds):
             f"{bb['launches']} own launches, {bb['fly_ms']:.2f} ms, "
             f"{bb['flight_steps']} plane-steps, {bb['panel_steps']} panel-steps")
 
-    cpu = None
-    f64 = fp64_check(genomes, res, w, timed_samples, timed_seeds)
-    log(f"FP64 check: {f64}")
+    cpu, f64 = None, None
+    if oracle_check if oracle_check is not None else cpu_seconds > 0:
+        f64 = fp64_check(genomes, res, w, timed_samples, timed_seeds)
+        log(f"FP64 check: {f64}")
     if cpu_seconds > 0:
         log("cpu baseline ...")
         cpu = cpu_baseline(genomes, res, w, cpu_seconds)
@@ -658,19 +664,20 @@ def generation_options(**kw):
     return argparse.Namespace(**o)
 
 
-def other_workloads(PA):
+def other_workloads(PA, oracle_check):
     """C2 and C5 measured in the same process after the C3 line's own work, at their default
     settings (C2: 10 timed generations after 2 warm-up ones, graph replays; C5: 2 timed audits
-    after 1), each with its dominant-kernel roofline and an FP64 check of its own timed results."""
+    after 1), each with its dominant-kernel roofline and, in the CPU leg (oracle_check), an FP64
+    check of its own timed results."""
     out = {}
     t0 = time.perf_counter()
-    c2 = generation_run(PA, "c2", generation_options(), 10, 2, 0)
+    c2 = generation_run(PA, "c2", generation_options(), 10, 2, 0, oracle_check)
     keep = ("value", "unit", "steps", "warmup", "ms_per_step", "gens_per_hour_eval", "roofline", "fp64_check")
     out["c2"] = {k: c2[k] for k in keep}
     out["c2"]["workload"] = c2["config"]["workload"]
     out["c2"]["wall_s"] = round(time.perf_counter() - t0, 1)
     t0 = time.perf_counter()
-    c5 = audit_run(PA, 2, 1, 0)
+    c5 = audit_run(PA, 2, 1, 0, oracle_check)
     keep = ("value", "unit", "steps", "warmup", "ms_per_step", "audits_per_hour", "roofline", "fp64_check",
             "surrogate_ranking")
     out["c5"] = {k: c5[k] for k in keep}
@@ -770,7 +777,7 @@ def main():
     if ab is not None:
         out["surrogate_ab"] = ab
     if args.other_workloads if args.other_workloads is not None else args.workload == "c3":
-        out["other_workloads"] = other_workloads(PA)
+        out["other_workloads"] = other_workloads(PA, args.cpu_seconds > 0)
     print(json.dumps(out), flush=True)
 
 
