@@ -27,7 +27,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
 FLOP_PANEL, FLOP_PLANE = 68, 222  # algorithmic FLOPs per panel-step and per plane-step
 # HBM bytes per k_fly_all launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled
 # by a 1 GiB copy's readings) on this workload; the counters cannot be read inside a timed run
-TRAFFIC = {"c3": (3.12e8, "profiles/k_fly_all_c3_hbm_traffic.md")}  # per population launch
+TRAFFIC = {"c3": (3.11e8, "profiles/k_fly_all_c3_hbm_traffic.md")}  # per population launch
 
 WORKLOADS = {
     # name: pop, seeds per genome, flight cap [steps], gust RMS max, warp max [deg], steady wind sd
