@@ -130,10 +130,16 @@ void pr_ga_sanitize(float* genomes, int n) {  // sim.cu:323-325
         for (int j = 0; j < NG; j++) gene_set(genomes + (size_t)NG * i, j, genomes[(size_t)NG * i + j]);
 }
 
-// ranking (sim.cu:440-441)
+// ranking (sim.cu:440-441): std::sort of the indices by acc[a] > acc[b].  Sorted here as
+// (score, index) pairs under the same comparison of scores: std::sort's moves depend only on
+// the comparator's answers, which are the same element for element, so the permutation is
+// the same (ties included) without an indirect load per comparison.
 void pr_ga_rank(const float* acc, int pop, int* idx) {
-    for (int i = 0; i < pop; i++) idx[i] = i;
-    std::sort(idx, idx + pop, [&](int a, int b) { return acc[a] > acc[b]; });
+    std::vector<std::pair<float, int>> v((size_t)pop);
+    for (int i = 0; i < pop; i++) v[i] = {acc[i], i};
+    std::sort(v.begin(), v.end(),
+              [](const std::pair<float, int>& a, const std::pair<float, int>& b) { return a.first > b.first; });
+    for (int i = 0; i < pop; i++) idx[i] = v[i].second;
 }
 
 // breeding (sim.cu:472-490): elite copy, 5% immigrants, 2x tournament-of-4, blend, annealed

@@ -52,7 +52,7 @@ def test_gene_bounds_match_reference_table():
 
 def test_rank_is_std_sort_order_with_ties():
     rng = np.random.default_rng(0)
-    for pop in (2, 17, 1000, 4096):
+    for pop in (2, 17, 1000, 4096, 100_000):
         acc = rng.integers(-3, 4, pop).astype(np.float32)  # heavy ties
         assert np.array_equal(PA.ga_rank(acc), O.ga_rank(acc))
         acc = rng.normal(size=pop).astype(np.float32)
