@@ -16,6 +16,7 @@
 #include <filesystem>
 #include <random>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -241,7 +242,6 @@ static int gen_main(pr_gen* e, const float* d_genomes, const int* d_res, int pop
         for (int i = 0; i < pop; i++) acc[i] = robust_row(e->samples.data() + (size_t)i * nseeds, nseeds, tmp);
         if (samples) std::memcpy(samples, e->samples.data(), sizeof(float) * pop * nseeds);
     }
-    pr_ga_rank(acc, pop, idx);
     e->last_pop = pop;
     e->last_nseeds = nseeds;
     return 0;
@@ -267,6 +267,55 @@ static int hold_launch(pr_gen* e, int s, const float* d_genomes, const int* idx,
     h.k = k;
     for (int c = 0; c < k; c++) h.idx[c] = idx[c];
     return 0;
+}
+
+// The first k (<= 8) places of the ranking without the full sort, when they are forced: if
+// the k + 1 largest scores are strictly decreasing (no tie among them) and no score is NaN,
+// every sort by acc[a] > acc[b] puts exactly these genomes, in this order, first.  false
+// otherwise (ties, NaN, pop <= k): then only the full ranking decides.
+static bool forced_top(const float* acc, int pop, int k, int* top) {
+    if (k < 1 || k > 8 || pop <= k) return false;
+    float v[9];
+    int ix[9];
+    int m = 0;
+    for (int i = 0; i < pop; i++) {
+        const float x = acc[i];
+        if (x != x) return false;
+        if (m == k + 1 && !(x > v[k])) continue;
+        int j = m < k + 1 ? m++ : k;  // a new slot, or the smallest kept one when full
+        while (j > 0 && x > v[j - 1]) {
+            v[j] = v[j - 1];
+            ix[j] = ix[j - 1];
+            j--;
+        }
+        v[j] = x;
+        ix[j] = i;
+    }
+    for (int j = 0; j < k; j++)
+        if (!(v[j] > v[j + 1])) return false;
+    for (int j = 0; j < k; j++) top[j] = ix[j];
+    return true;
+}
+
+// Ranking (sim.cu:440-441) and the holdout launch of its top k.  When forced_top fixes the
+// top k, the full std::sort ranking runs on a worker thread (host memory only) while this
+// thread launches the holdout, so the host's launch work is off the ranking's path.
+static int rank_and_hold(pr_gen* e, int s, const float* d_genomes, const float* acc, int pop, int* idx, int k,
+                         const pr_sim_params* p) {
+    int top[8];
+    if (!forced_top(acc, pop, k, top)) {
+        pr_ga_rank(acc, pop, idx);
+        return k > 0 ? hold_launch(e, s, d_genomes, idx, k, p) : 0;
+    }
+    std::thread w;
+    try {
+        w = std::thread(pr_ga_rank, acc, pop, idx);
+    } catch (const std::system_error&) {
+        pr_ga_rank(acc, pop, idx);
+    }
+    const int rc = hold_launch(e, s, d_genomes, top, k, p);
+    if (w.joinable()) w.join();
+    return rc;
 }
 
 // Waits for slot s's holdout batch; champion = the first strict maximum of the holdout
@@ -296,13 +345,11 @@ int pr_gen_evaluate(pr_gen* e, const float* d_genomes, const int* d_res, int pop
     if (e->oldest >= 0) return -2;  // an asynchronous holdout batch is still uncollected
     int rc = gen_main(e, d_genomes, d_res, pop, seeds, nseeds, p, acc, idx, samples);
     if (rc) return rc;
+    const int k = std::min(std::min(holdout_k, pop), 8);
+    if ((rc = rank_and_hold(e, 0, d_genomes, acc, pop, idx, k, p))) return rc;
     float holdout = -1e9f;
     int champion = idx[0];
-    const int k = std::min(std::min(holdout_k, pop), 8);
-    if (k > 0) {
-        if ((rc = hold_launch(e, 0, d_genomes, idx, k, p))) return rc;
-        if ((rc = hold_collect(e, 0, &holdout, &champion))) return rc;
-    }
+    if (k > 0 && (rc = hold_collect(e, 0, &holdout, &champion))) return rc;
     if (holdout_out) *holdout_out = holdout;
     if (champion_out) *champion_out = champion;
     return 0;
@@ -321,9 +368,11 @@ int pr_gen_evaluate_async(pr_gen* e, const float* d_genomes, const int* d_res, i
     if (rc) return rc;
     const int k = std::min(std::min(holdout_k, pop), 8);
     e->hold[s].k = 0;
-    e->hold[s].idx[0] = idx[0];
-    if (k > 0 && (rc = hold_launch(e, s, d_genomes, idx, k, p))) return rc;
-    if (k <= 0) e->hold[s].pending = 2;  // nothing to fly: collect returns -1e9 and idx[0]
+    if ((rc = rank_and_hold(e, s, d_genomes, acc, pop, idx, k, p))) return rc;
+    if (k <= 0) {  // nothing to fly: collect returns -1e9 and idx[0]
+        e->hold[s].idx[0] = idx[0];
+        e->hold[s].pending = 2;
+    }
     if (e->oldest < 0) e->oldest = s;
     e->next_slot = s ^ 1;
     return 0;

@@ -305,16 +305,31 @@ def test_scores_bit_exact(PA):
         assert np.array_equal(got, O.robust_scores(s, pop, nseeds)), nseeds
 
 
-def test_generation_evaluate_matches_host_logic(PA):
+@pytest.mark.parametrize("tied", [False, True])
+def test_generation_evaluate_matches_host_logic(PA, tied):
+    """Robust scores, std::sort ranking and the top-8 holdout champion of pr_gen_evaluate, and
+    the same from the pipelined entry point.  Without ties at the top the holdout's top 8 come
+    from a scan while the full ranking runs beside the holdout launch; tied: two copies of the
+    best design put three equal scores first, and the top 8 come from the full ranking."""
     pop, nseeds = 96, 8
     gs = O.random_genomes(pop, 11)
-    E = PA.Generation(pop, nseeds)
-    E.upload(gs)
     params = PA.SimParams.reference()
     seeds = np.arange(1, nseeds + 1, dtype=np.int32)
+    if tied:
+        E0 = PA.Generation(pop, nseeds)
+        E0.upload(gs)
+        i0 = E0.evaluate(pop, seeds, params)["idx"]
+        E0.close()
+        gs = gs.copy()
+        gs[i0[3]] = gs[i0[0]]
+        gs[i0[40]] = gs[i0[0]]
+    E = PA.Generation(pop, nseeds)
+    E.upload(gs)
     r = E.evaluate(pop, seeds, params, want_samples=True)
     assert np.array_equal(r["acc"], O.robust_scores(r["samples"], pop, nseeds))
     assert np.array_equal(r["idx"], O.ga_rank(r["acc"]))
+    a = r["acc"][r["idx"]]
+    assert (a[0] == a[1] == a[2]) if tied else np.all(a[:8] > a[1:9])
     # holdout: top-8 re-flown on seeds 100003 + p*7919
     top = r["idx"][:8]
     pl = PA.Planes(8, 128, 1024)
@@ -323,6 +338,10 @@ def test_generation_evaluate_matches_host_logic(PA):
     hsc = O.robust_scores(fit, 8, 128)
     best = int(np.argmax(hsc))  # first strict maximum
     assert r["champion"] == top[best] and np.float32(r["holdout"]) == hsc[best]
+    r2 = E.evaluate_async(pop, seeds, params)
+    c2 = E.collect()
+    assert np.array_equal(r2["idx"], r["idx"]) and c2["champion"] == r["champion"] and c2["holdout"] == r["holdout"]
+    E.close()
 
 
 def test_pose_recording_frames(PA):
