@@ -66,6 +66,7 @@ struct pr_ctx {
     int* d_counts = nullptr;  // PR_CNT per bucket: 2 alternating list counts, 2 claim counters
     int* d_tcount = nullptr;  // [NB_MAX][max_tiles] survivors per compaction tile
     int* d_perm = nullptr;    // [max_genomes] genomes in chunk 0's list order (sorted by mesh size)
+    int* d_sortwork = nullptr;  // [sort_work_ints(max_genomes)] the size sort's per-tile key counts
     int max_tiles = 0;
     int* d_bucket_nt = nullptr;
     unsigned long long* d_stats = nullptr;
@@ -233,6 +234,7 @@ int pr_create(pr_ctx** out, int max_genomes, long long max_panels, int max_scena
     c->max_tiles = (int)((max_flights + PR_COMPACT_TILE - 1) / PR_COMPACT_TILE);
     ALLOC(c->d_tcount, sizeof(int) * NB_MAX * (size_t)c->max_tiles);
     ALLOC(c->d_perm, sizeof(int) * max_genomes);
+    ALLOC(c->d_sortwork, sizeof(int) * sort_work_ints(max_genomes));
     ALLOC(c->d_bucket_nt, sizeof(int) * NB_MAX);
     ALLOC(c->d_stats, sizeof(unsigned long long) * PR_STATS_N);
     ALLOC(c->d_cstats, sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);
@@ -255,7 +257,7 @@ void pr_destroy(pr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* bufs[] = {c->d_ntraw, c->d_off, c->d_nt, c->d_nv, c->d_ext, c->d_mass, c->d_seeds, c->d_scen, c->d_state,
                     c->d_fc, c->d_lists[0], c->d_lists[1], c->d_counts, c->d_bucket_nt, c->d_stats, c->d_scan_tmp,
-                    c->d_cstats, c->d_chunk_n, c->d_steps_own, c->d_tcount, c->d_perm};
+                    c->d_cstats, c->d_chunk_n, c->d_steps_own, c->d_tcount, c->d_perm, c->d_sortwork};
     for (void* b : bufs) if (b) (void)hipFree(b);
     for (int k = 0; k < 7; k++) if (c->d_panel[k]) (void)hipFree(c->d_panel[k]);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -401,7 +403,7 @@ static int init_flights(pr_ctx* c, int layout, int n_genomes, int n_scen, const 
     CompactArgs ca = compact_args(c, NB);
     ca.src = c->d_lists[1]; ca.src_stride = 0; ca.by_bucket = 1; ca.n_fixed = (int)nf; ca.c_dst = 0;
     if (c->sort_lists) {
-        HCK(launch_sort_genomes(c->d_nt, n_genomes, c->d_perm, c->stream));
+        HCK(launch_sort_genomes(c->d_nt, n_genomes, c->d_perm, c->d_sortwork, c->stream));
         ca.perm = c->d_perm;
         ca.perm_div = layout == 0 ? n_scen : 1;
     }
@@ -852,6 +854,7 @@ static int noalloc_setup(int dev, pr_ctx* c) {
     c->d_counts = (int*)take(sizeof(int) * PR_CNT * NB_MAX);
     c->d_tcount = (int*)take(sizeof(int) * NB_MAX * (size_t)c->max_tiles);
     c->d_perm = (int*)take(sizeof(int) * G);
+    c->d_sortwork = (int*)take(sizeof(int) * sort_work_ints(G));
     c->d_bucket_nt = (int*)take(sizeof(int) * NB_MAX);
     c->d_stats = (unsigned long long*)take(sizeof(unsigned long long) * PR_STATS_N);
     c->d_cstats = (unsigned long long*)take(sizeof(unsigned long long) * 2 * PR_MAX_CHUNKS);

@@ -987,68 +987,93 @@ hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ k_sort_genomes
-// Stable counting sort of genomes by panel-slot key min(nt, 2047) / 8 (256 keys) in one
-// 1024-thread block: a histogram and its scan give each key's start; then 1024 genomes at
-// a time, each wave ranks its lanes among equal keys (a ballot per distinct key, in lane
-// order), the waves' counts are scanned per key, and every genome lands after all earlier
-// genomes of its key.  The result depends on the data alone.  Built into chunk 0's lists
-// (through CompactArgs::perm), it lets the groups of a wave -- which take neighbouring
-// list positions -- fly meshes of nearly equal size, so a wave's panel-slot count (its
-// fullest group's) wastes little on the others.
+// ------------------------------------------------------------------ k_sort_* (chunk 0's list order)
+// Stable counting sort of genomes by panel-slot key min(nt, 2047) / 8 (256 keys), in tiles of
+// SORT_THREADS genomes: k_sort_count takes each tile's histogram; k_sort_offsets turns the
+// histograms into each (tile, key)'s first output position (all genomes of smaller keys,
+// then those of the same key in earlier tiles); k_sort_scatter ranks each tile's genomes
+// among equal keys -- within a wave by a ballot per distinct key, in lane order, then
+// across the tile's waves in wave order -- and places every genome after all earlier genomes
+// of its key.  The result depends on the data alone.  Built into chunk 0's lists (through
+// CompactArgs::perm), it lets the groups of a wave -- which take neighbouring list positions
+// -- fly meshes of nearly equal size, so a wave's panel-slot count (its fullest group's)
+// wastes little on the others.
 constexpr int SORT_THREADS = 1024, SORT_KEYS = 256;
 
 __device__ __forceinline__ int sort_key(int nt) { return min(nt, 2047) >> 3; }
 
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_genomes(const int* __restrict__ nt, int n, int* __restrict__ perm) {
+size_t sort_work_ints(int n) { return (size_t)((n + SORT_THREADS - 1) / SORT_THREADS) * SORT_KEYS; }
+
+// cnt[tile][key]: genomes of the tile with that key
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_count(const int* __restrict__ nt, int n, int* __restrict__ cnt) {
+    __shared__ int h[SORT_KEYS];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) h[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * SORT_THREADS + tid;
+    if (i < n) atomicAdd(&h[sort_key(nt[i])], 1);
+    __syncthreads();
+    for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) cnt[(size_t)blockIdx.x * SORT_KEYS + k] = h[k];
+}
+
+// One block, one thread per key: cnt[tile][key] is replaced by the first output position of
+// that tile's genomes of that key.
+__global__ __launch_bounds__(SORT_KEYS) void k_sort_offsets(int* __restrict__ cnt, int tiles) {
     __shared__ int start[SORT_KEYS];
-    __shared__ int tot[SORT_KEYS];
-    __shared__ int wcnt[SORT_THREADS / 64][SORT_KEYS];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) start[k] = 0;
+    const int k = threadIdx.x;
+    int tot = 0;
+    for (int t = 0; t < tiles; t++) tot += cnt[(size_t)t * SORT_KEYS + k];
+    start[k] = tot;
     __syncthreads();
-    for (int i = tid; i < n; i += SORT_THREADS) atomicAdd(&start[sort_key(nt[i])], 1);
-    __syncthreads();
-    if (tid == 0) {
+    if (k == 0) {
         int s = 0;
-        for (int k = 0; k < SORT_KEYS; k++) { const int c = start[k]; start[k] = s; s += c; }
+        for (int j = 0; j < SORT_KEYS; j++) { const int c = start[j]; start[j] = s; s += c; }
     }
     __syncthreads();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int c0 = 0; c0 < n; c0 += SORT_THREADS) {
-        const int i = c0 + tid;
-        const bool valid = i < n;
-        const int key = valid ? sort_key(nt[i]) : -1;
-        for (int k = lane; k < SORT_KEYS; k += 64) wcnt[w][k] = 0;
-        __syncthreads();
-        // rank among this wave's lanes of equal key, in lane order (every lane loops until
-        // the wave has placed all its valid lanes)
-        int rank = 0;
-        bool left = valid;
-        for (unsigned long long m = __ballot(left); m; m = __ballot(left)) {
-            const int first = __builtin_ctzll(m);
-            const int kk = __shfl(key, first);
-            const unsigned long long same = __ballot(left && key == kk);
-            if (left && key == kk) { rank = __popcll(same & lt); left = false; }
-            if (lane == first) wcnt[w][kk] = __popcll(same);
-        }
-        __syncthreads();
-        for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) {  // waves' offsets per key
-            int s = 0;
-            for (int ww = 0; ww < SORT_THREADS / 64; ww++) { const int c = wcnt[ww][k]; wcnt[ww][k] = s; s += c; }
-            tot[k] = s;
-        }
-        __syncthreads();
-        if (valid) perm[start[key] + wcnt[w][key] + rank] = i;
-        __syncthreads();
-        for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) start[k] += tot[k];
-        __syncthreads();
+    int base = start[k];
+    for (int t = 0; t < tiles; t++) {
+        const int c = cnt[(size_t)t * SORT_KEYS + k];
+        cnt[(size_t)t * SORT_KEYS + k] = base;
+        base += c;
     }
 }
 
-hipError_t launch_sort_genomes(const int* nt, int n, int* perm, hipStream_t s) {
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const int* __restrict__ nt, int n,
+                                                               const int* __restrict__ off, int* __restrict__ perm) {
+    __shared__ int wcnt[SORT_THREADS / 64][SORT_KEYS];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int i = blockIdx.x * SORT_THREADS + tid;
+    const bool valid = i < n;
+    const int key = valid ? sort_key(nt[i]) : -1;
+    for (int k = lane; k < SORT_KEYS; k += 64) wcnt[w][k] = 0;
+    __syncthreads();
+    // rank among this wave's lanes of equal key, in lane order (every lane loops until the
+    // wave has placed all its valid lanes)
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int rank = 0;
+    bool left = valid;
+    for (unsigned long long m = __ballot(left); m; m = __ballot(left)) {
+        const int first = __builtin_ctzll(m);
+        const int kk = __shfl(key, first);
+        const unsigned long long same = __ballot(left && key == kk);
+        if (left && key == kk) { rank = __popcll(same & lt); left = false; }
+        if (lane == first) wcnt[w][kk] = __popcll(same);
+    }
+    __syncthreads();
+    for (int k = tid; k < SORT_KEYS; k += SORT_THREADS) {  // waves' offsets per key
+        int s = 0;
+        for (int ww = 0; ww < SORT_THREADS / 64; ww++) { const int c = wcnt[ww][k]; wcnt[ww][k] = s; s += c; }
+    }
+    __syncthreads();
+    if (valid) perm[off[(size_t)blockIdx.x * SORT_KEYS + key] + wcnt[w][key] + rank] = i;
+}
+
+hipError_t launch_sort_genomes(const int* nt, int n, int* perm, int* work, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_genomes, dim3(1), dim3(SORT_THREADS), 0, s, nt, n, perm);
+    const int tiles = (n + SORT_THREADS - 1) / SORT_THREADS;
+    hipLaunchKernelGGL(k_sort_count, dim3(tiles), dim3(SORT_THREADS), 0, s, nt, n, work);
+    hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(SORT_KEYS), 0, s, work, tiles);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(SORT_THREADS), 0, s, nt, n, (const int*)work, perm);
     return hipGetLastError();
 }
 

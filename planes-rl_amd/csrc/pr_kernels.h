@@ -135,8 +135,10 @@ hipError_t launch_fly_split(const FlyAllArgs& a, int waves, int grid, hipStream_
 constexpr int PR_SPLIT_NT = 2048;
 hipError_t fly_all_occupancy(int* blocks_per_cu);
 hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s);
-// perm[0..n) = genome indices stably sorted by panel-slot key (triangle count / 8)
-hipError_t launch_sort_genomes(const int* nt, int n, int* perm, hipStream_t s);
+// perm[0..n) = genome indices stably sorted by panel-slot key (triangle count / 8); work:
+// sort_work_ints(n) ints of device memory (per-tile key counts)
+hipError_t launch_sort_genomes(const int* nt, int n, int* perm, int* work, hipStream_t s);
+size_t sort_work_ints(int n);
 
 hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);
