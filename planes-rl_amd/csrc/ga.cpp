@@ -131,15 +131,109 @@ void pr_ga_sanitize(float* genomes, int n) {  // sim.cu:323-325
         for (int j = 0; j < NG; j++) gene_set(genomes + (size_t)NG * i, j, genomes[(size_t)NG * i + j]);
 }
 
-// ranking (sim.cu:440-441): std::sort of the indices by acc[a] > acc[b].  Sorted here as
-// (score, index) pairs under the same comparison of scores: std::sort's moves depend only on
-// the comparator's answers, which are the same element for element, so the permutation is
-// the same (ties included) without an indirect load per comparison.
+}  // extern "C"
+
+// ------------------------------------------------------------------ ranking
+// The reference ranks with std::sort of the indices by acc[a] > acc[b] (sim.cu:440-441);
+// with equal scores the order is whatever libstdc++'s introsort makes of them, and breeding
+// reads it (elite rows, tournament entrants by rank).  The ranking here runs the same
+// introsort steps -- median of (second, middle, last) moved to the front, an unguarded
+// partition around it, the right part first, ranges of at most 16 left to one final
+// insertion sort, heapsort (std::partial_sort over the whole range) where the depth limit
+// 2 floor(log2 n) runs out -- so it makes the same moves and returns the same permutation,
+// but the two sides of a partition are disjoint, so large right parts go to other threads.
+// The tests compare it with std::sort of the indices on tied and untied data.
+namespace {
+using RankItem = std::pair<float, int>;
+inline bool rank_before(const RankItem& a, const RankItem& b) { return a.first > b.first; }
+
+void rank_median_to_first(RankItem* r, RankItem* a, RankItem* b, RankItem* c) {
+    if (rank_before(*a, *b)) {
+        if (rank_before(*b, *c)) std::iter_swap(r, b);
+        else if (rank_before(*a, *c)) std::iter_swap(r, c);
+        else std::iter_swap(r, a);
+    } else if (rank_before(*a, *c)) {
+        std::iter_swap(r, a);
+    } else if (rank_before(*b, *c)) {
+        std::iter_swap(r, c);
+    } else {
+        std::iter_swap(r, b);
+    }
+}
+
+RankItem* rank_partition(RankItem* first, RankItem* last) {
+    rank_median_to_first(first, first + 1, first + (last - first) / 2, last - 1);
+    RankItem* lo = first + 1;
+    RankItem* hi = last;
+    const RankItem& pivot = *first;
+    for (;;) {
+        while (rank_before(*lo, pivot)) ++lo;
+        --hi;
+        while (rank_before(pivot, *hi)) --hi;
+        if (!(lo < hi)) return lo;
+        std::iter_swap(lo, hi);
+        ++lo;
+    }
+}
+
+// Sorts [first, last).  threads: how many more threads this call may start for right parts
+// (shared with the calls it starts, which receive part of it).  Every range is left sorted
+// by the call that owns it: the partitions order the ranges among themselves, so the final
+// insertion sort moves every element only within its leaf range (at most 16 elements, or a
+// heapsorted range, where it moves nothing) and runs here leaf by leaf.
+void rank_introsort(RankItem* first, RankItem* last, int depth, int threads) {
+    std::vector<std::thread> kids;
+    while (last - first > 16) {
+        if (depth == 0) {
+            std::partial_sort(first, last, last, rank_before);
+            break;
+        }
+        --depth;
+        RankItem* cut = rank_partition(first, last);
+        bool spawned = false;
+        if (threads > 0 && last - cut > 8192) {
+            // the threads (this one included) split between the two sides by their sizes
+            const long long all = threads + 1;
+            const int right = (int)std::min(all - 1, std::max(1LL, (all * (last - cut) + (last - first) / 2) / (last - first)));
+            try {
+                kids.emplace_back(rank_introsort, cut, last, depth, right - 1);
+                threads -= right;
+                spawned = true;
+            } catch (const std::system_error&) {
+                threads = 0;
+            }
+        }
+        if (!spawned) rank_introsort(cut, last, depth, 0);
+        last = cut;
+    }
+    for (RankItem* i = first + 1; i < last; i++) {  // the leaf's insertion sort
+        const RankItem v = *i;
+        RankItem* j = i;
+        while (j > first && rank_before(v, *(j - 1))) {
+            *j = *(j - 1);
+            j--;
+        }
+        *j = v;
+    }
+    for (auto& k : kids) k.join();
+}
+
+void rank_sort(RankItem* a, int n) {
+    if (n < 2) return;
+    int lg = 0;
+    while ((2LL << lg) <= n) lg++;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int threads = n > 32768 ? (int)std::min(15u, hw > 1 ? hw - 1 : 0u) : 0;
+    rank_introsort(a, a + n, 2 * lg, threads);
+}
+}  // namespace
+
+extern "C" {
+
 void pr_ga_rank(const float* acc, int pop, int* idx) {
-    std::vector<std::pair<float, int>> v((size_t)pop);
+    std::vector<RankItem> v((size_t)std::max(pop, 0));
     for (int i = 0; i < pop; i++) v[i] = {acc[i], i};
-    std::sort(v.begin(), v.end(),
-              [](const std::pair<float, int>& a, const std::pair<float, int>& b) { return a.first > b.first; });
+    rank_sort(v.data(), pop);
     for (int i = 0; i < pop; i++) idx[i] = v[i].second;
 }
 

@@ -61,6 +61,27 @@ def test_rank_is_std_sort_order_with_ties():
         assert np.all(np.diff(acc[idx]) <= 0)
 
 
+@pytest.mark.parametrize("n", [33, 8193, 40_000, 250_000])
+def test_rank_matches_std_sort_on_ordered_and_tied_patterns(n):
+    """The ranking's introsort (parallel above 32 768 genomes) against std::sort of the
+    indices: ascending, descending, all equal, organ pipe, two values, signed zeros, a third
+    at -100 (unprintable), uniform."""
+    rng = np.random.default_rng(n)
+    i = np.arange(n)
+    pats = {
+        "ascending": i.astype(np.float32),
+        "descending": (n - i).astype(np.float32),
+        "equal": np.ones(n, np.float32),
+        "organ": np.minimum(i, n - i).astype(np.float32),
+        "two": rng.integers(0, 2, n).astype(np.float32),
+        "zeros": np.where(i % 2 == 1, np.float32(0.0), np.float32(-0.0)).astype(np.float32),
+        "unprintable": np.where(i % 3 == 0, -100.0, rng.uniform(0, 15, n)).astype(np.float32),
+        "uniform": rng.uniform(-5, 15, n).astype(np.float32),
+    }
+    for name, acc in pats.items():
+        assert np.array_equal(PA.ga_rank(acc), O.ga_rank(acc)), name
+
+
 def test_random_genomes_and_breeding_match_oracle_sequence():
     pop, gens = 256, 6
     ga = PA.GA(42)
