@@ -183,6 +183,7 @@ RankItem* rank_partition(RankItem* first, RankItem* last) {
 // heapsorted range, where it moves nothing) and runs here leaf by leaf.
 void rank_introsort(RankItem* first, RankItem* last, int depth, int threads) {
     std::vector<std::thread> kids;
+    kids.reserve(threads > 0 ? threads : 0);  // each start uses at least one of them
     while (last - first > 16) {
         if (depth == 0) {
             std::partial_sort(first, last, last, rank_before);
