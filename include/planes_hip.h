@@ -70,7 +70,9 @@ const char* pr_last_error(void);
 /* Replaces gpu_prepare (sim.h:11, sim.cu:214-218; k_prepare sim.cu:25-49): meshes n
  * genomes (d_genomes [n*35] floats, the Genome byte layout), computes mass properties,
  * the print-volume check and packed panel records.  d_res: optional [n*2] (ch, sp)
- * per genome (null = reference resolution).  Asynchronous on the context stream. */
+ * per genome (null = reference resolution); a resolution outside 1..max_ch x 1..max_sp of
+ * pr_create is not meshed (no triangles: the genome flies as unprintable, -100).
+ * Asynchronous on the context stream. */
 int pr_prepare(pr_ctx* ctx, const float* d_genomes, const int* d_res, int n);
 
 /* Arithmetic of pr_prepare: 32 (default) reproduces the reference's float meshing and

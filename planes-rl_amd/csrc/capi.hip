@@ -319,7 +319,7 @@ int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
     ARG(c && d_genomes, "pr_prepare: null argument");
     ARG(n > 0 && n <= c->cap_genomes, "pr_prepare: n outside 1..max_genomes");
     if (c->prep0) HCK(hipEventRecord(c->prep0, c->stream));  // untimed in the allocation-free path
-    HCK(launch_layout(d_genomes, d_res, 5, 8, n, c->d_ntraw, c->stream));
+    HCK(launch_layout(d_genomes, d_res, 5, 8, c->max_ch, c->max_sp, n, c->d_ntraw, c->stream));
     HCK(launch_zero(c->d_ntraw + n, 1, c->stream));
     size_t tb = c->scan_tmp_bytes;
     HCK(hipcub::DeviceScan::ExclusiveSum(c->d_scan_tmp, tb, c->d_ntraw, c->d_off, n + 1, c->stream));
@@ -333,6 +333,7 @@ int pr_prepare(pr_ctx* c, const float* d_genomes, const int* d_res, int n) {
     }
     PrepareArgs a;
     a.genomes = d_genomes; a.res = d_res; a.def_ch = 5; a.def_sp = 8; a.nv_max = c->nv_max;
+    a.max_ch = c->max_ch; a.max_sp = c->max_sp;
     a.off = c->d_off; a.mass = c->d_mass; a.nt = c->d_nt; a.nv = c->d_nv; a.ext = c->d_ext;
     a.pcx = c->d_panel[0]; a.pcy = c->d_panel[1]; a.pcz = c->d_panel[2]; a.pA = c->d_panel[3];
     a.pnx = c->d_panel[4]; a.pny = c->d_panel[5]; a.pnz = c->d_panel[6];

@@ -61,14 +61,20 @@ __device__ __forceinline__ int wave_slots(int np) {
 }
 
 // ------------------------------------------------------------------ k_layout
+// A mesh resolution outside 1..(max_ch, max_sp) -- the bounds the context's buffers and LDS
+// are sized for -- is not meshed: the genome gets no triangles and flies as unprintable.
+__device__ __forceinline__ bool res_ok(int ch, int sp, int max_ch, int max_sp) {
+    return ch >= 1 && sp >= 1 && ch <= max_ch && sp <= max_sp;
+}
+
 __global__ void k_layout(const float* __restrict__ genomes, const int* __restrict__ res, int def_ch, int def_sp,
-                         int n, int* __restrict__ nt_raw) {
+                         int max_ch, int max_sp, int n, int* __restrict__ nt_raw) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float g[NG];
     for (int k = 0; k < NG; k++) g[k] = genomes[(size_t)i * NG + k];
     int ch = res ? res[2 * i] : def_ch, sp = res ? res[2 * i + 1] : def_sp;
-    nt_raw[i] = layout_of(g, ch, sp).nt;
+    nt_raw[i] = res_ok(ch, sp, max_ch, max_sp) ? layout_of(g, ch, sp).nt : 0;
 }
 
 // ------------------------------------------------------------------ k_prepare
@@ -91,8 +97,6 @@ __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
     extern __shared__ double lds_raw[];
     __shared__ T g[NG];
     const int gi = blockIdx.x, lane = threadIdx.x;
-    if (lane < NG) g[lane] = T(a.genomes[(size_t)gi * NG + lane]);
-    __syncthreads();
     float gf[2];
     gf[0] = a.genomes[(size_t)gi * NG + G_FIN_LAYOUT];
     gf[1] = a.genomes[(size_t)gi * NG + G_DECK_GAP];
@@ -101,7 +105,22 @@ __global__ __launch_bounds__(64) void k_prepare(PrepareArgs a) {
     gl[G_FIN_LAYOUT] = gf[0];
     gl[G_DECK_GAP] = gf[1];
     const int ch = a.res ? a.res[2 * gi] : a.def_ch, sp = a.res ? a.res[2 * gi + 1] : a.def_sp;
+    if (!res_ok(ch, sp, a.max_ch, a.max_sp)) {  // not meshed (the first launch records it)
+        if (a.nv_lo == 0) {
+            float* mo = reinterpret_cast<float*>(a.mass + gi);
+            if (lane < 20) mo[lane] = 0.0f;
+            if (lane < 3) a.ext[3 * gi + lane] = 0.0f;
+            if (lane == 0) {
+                a.nt[gi] = 0;
+                a.nv[gi] = 0;
+            }
+        }
+        return;
+    }
     const MeshLayout L = layout_of(gl, ch, sp);
+    if (L.nv < a.nv_lo || L.nv > a.nv_hi) return;  // the other launch's genome (the whole block)
+    if (lane < NG) g[lane] = T(a.genomes[(size_t)gi * NG + lane]);
+    __syncthreads();
     V3T<T>* Vx = reinterpret_cast<V3T<T>*>(lds_raw);
     T* term = reinterpret_cast<T*>(lds_raw) + 3 * (size_t)a.nv_max;  // [12][64]
 
@@ -1205,21 +1224,43 @@ __global__ __launch_bounds__(64) void k_scores_wave(const float* __restrict__ sa
 }
 
 // ------------------------------------------------------------------ launchers
-hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s) {
+hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int max_ch, int max_sp, int n,
+                         int* nt_raw, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_layout, dim3((n + 255) / 256), dim3(256), 0, s, genomes, res, ch, sp, n, nt_raw);
+    hipLaunchKernelGGL(k_layout, dim3((n + 255) / 256), dim3(256), 0, s, genomes, res, ch, sp, max_ch, max_sp, n,
+                       nt_raw);
     return hipGetLastError();
+}
+
+// One block per genome, LDS for its vertices and the 12 x 64 triangle terms.  The LDS a
+// block needs sets how many blocks share a CU, and the context's vertex bound (from its
+// largest chordwise and spanwise resolutions taken separately) is often twice what any of
+// its genomes has: genomes with at most nv_small vertices go to a launch whose blocks fit
+// 16 to a CU (10 KB each), the rest to one sized for the bound; each block of either launch
+// returns at once when its genome belongs to the other.
+template <class T>
+static void launch_prepare_t(PrepareArgs a, int n, hipStream_t s) {
+    const int nv_small = (10240 / (int)sizeof(T) - 12 * 64) / 3;
+    const int nv_max = a.nv_max;
+    auto go = [&](int lo, int hi) {
+        a.nv_lo = lo;
+        a.nv_hi = hi;
+        a.nv_max = hi;
+        const size_t lds = sizeof(T) * (3 * (size_t)hi + 12 * 64);
+        hipLaunchKernelGGL(k_prepare<T>, dim3(n), dim3(64), lds, s, a);
+    };
+    if (nv_max <= nv_small) {
+        go(0, nv_max);
+    } else {
+        go(0, nv_small);
+        go(nv_small + 1, nv_max);
+    }
 }
 
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (fp64) {
-        size_t lds = sizeof(double) * (3 * (size_t)a.nv_max + 12 * 64);
-        hipLaunchKernelGGL(k_prepare<double>, dim3(n), dim3(64), lds, s, a);
-    } else {
-        size_t lds = sizeof(float) * (3 * (size_t)a.nv_max + 12 * 64);
-        hipLaunchKernelGGL(k_prepare<float>, dim3(n), dim3(64), lds, s, a);
-    }
+    if (fp64) launch_prepare_t<double>(a, n, s);
+    else launch_prepare_t<float>(a, n, s);
     return hipGetLastError();
 }
 

@@ -10,7 +10,9 @@ struct PrepareArgs {
     const float* genomes;  // [n*35]
     const int* res;        // [n*2] (ch, sp) per genome, or null -> def_ch/def_sp
     int def_ch, def_sp;
-    int nv_max;            // LDS vertex capacity
+    int nv_max;            // LDS vertex capacity of this launch
+    int nv_lo, nv_hi;      // this launch prepares the genomes with nv_lo <= vertices <= nv_hi
+    int max_ch, max_sp;    // the context's resolution bounds (others: not meshed, unprintable)
     const int* off;        // [n] exclusive prefix of the closed-form triangle counts
     Mass* mass;            // [n]
     int* nt;               // [n] triangles flown (0 if outside the print volume)
@@ -140,7 +142,8 @@ hipError_t launch_compact(const CompactArgs& a, long long n_total, hipStream_t s
 hipError_t launch_sort_genomes(const int* nt, int n, int* perm, int* work, hipStream_t s);
 size_t sort_work_ints(int n);
 
-hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int n, int* nt_raw, hipStream_t s);
+hipError_t launch_layout(const float* genomes, const int* res, int ch, int sp, int max_ch, int max_sp, int n,
+                         int* nt_raw, hipStream_t s);
 hipError_t launch_prepare(const PrepareArgs& a, int n, int fp64, hipStream_t s);
 hipError_t launch_scenarios(const int* seeds, int n, const ScenarioParams& sp, float* out, hipStream_t s);
 hipError_t launch_queue(const QueueArgs& a, hipStream_t s);

@@ -360,6 +360,28 @@ def test_pose_recording_frames(PA):
         assert abs(fit[i] - f_ref) <= 0.01 * max(abs(f_ref), 1.0)
 
 
+def test_resolution_beyond_the_context_bounds_is_not_meshed(PA):
+    """A (ch, sp) beyond the context's bounds, or below 1, is not meshed: no triangles, its
+    flights end at once with -100, and the other genomes fly as they do without it."""
+    gs = O.random_genomes(6, 5)
+    res = np.tile(np.array([[6, 10]], np.int32), (6, 1))
+    params = PA.SimParams.reference(max_steps=1200)
+    seeds = np.arange(1, 4, dtype=np.int32)
+    pl = PA.Planes(6, 3, 18, max_ch=6, max_sp=10)
+    f0, _ = pl.evaluate(gs, seeds, params, res=res)
+    bad = res.copy()
+    bad[1] = (7, 10)
+    bad[4] = (6, 0)
+    f1, s1 = pl.evaluate(gs, seeds, params, res=bad)
+    _, nt, nv, _ = pl.read_prepared()
+    pl.close()
+    assert nt[1] == 0 and nv[1] == 0 and nt[4] == 0 and nv[4] == 0
+    assert np.all(f1[[1, 4]] == -100.0) and np.all(s1[[1, 4]] == 0)
+    keep = [0, 2, 3, 5]
+    rel = np.abs(f1[keep] - f0[keep]) / np.maximum(np.abs(f0[keep]), 1e-3)
+    assert np.all((f1[keep] == -100.0) == (f0[keep] == -100.0)) and rel.max() < 1e-3
+
+
 def test_unprintable_genome_scores_minus_100(PA):
     gs = O.random_genomes(64, 21)
     bad = [g for g in gs if not O.mass(g)[1]]
