@@ -26,7 +26,7 @@ PR_CLI = os.path.join(ROOT, "planes-rl_amd", "planes_amd")
 LIB = os.path.join(ROOT, "planes-rl_amd", "libplanes_hip.so")
 HEADER = os.path.join(ROOT, "include", "planes_sim.h")
 
-needs_checkout = pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent (GPU box)")
+needs_checkout = pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="reference checkout absent on this machine")
 
 
 def presets():
