@@ -207,7 +207,7 @@ ARMS = {
 def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, gens=4, sur_gens=None, pretrain=2_000_000,
                  explore=0.5, buffer=300_000, mlp_batch=4096, train_epochs=None, lr=None, boot_epochs=16,
                  arms=("plain", "surrogate", "strict", "uniform", "parent"), delta_mode=True, pretrained_state=None,
-                 pretrain_source="c2", target_norm=True):
+                 pretrain_source="c2", target_norm=True, key_weights=None):
     """GA generations/hour with surrogate pre-screening against the plain batched sim, and
     against the same fly budget chosen without the network.  One generation: a pool of n_pool
     Gaussian mutants (sigma 5 % of each gene's range) of an n_elite elite; every arm flies its
@@ -253,6 +253,8 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
                 S = PA.Surrogate(max_batch=mlp_batch, seed=3)
                 ga.set_explore(explore if arm == "surrogate" else 0.0)
                 ga.set_parent_delta(delta_mode if arm == "surrogate" else False)
+                if arm == "surrogate" and key_weights is not None:
+                    ga.set_key_weights(*key_weights)
                 ga.set_target_norm(target_norm)
                 if pretrained:
                     S.set_params(p0)
@@ -335,6 +337,7 @@ def surrogate_ab(PA, n_pool=1_000_000, n_fly=100_000, n_elite=10_000, reps=3, ge
     return {"config": {"pool": n_pool, "fly": n_fly, "elite": n_elite, "environments": 12, "ga_seeds": reps,
                        "plain_gens": gens, "other_gens": max(gens, sur_gens), "explore": explore,
                        "relative_targets": bool(target_norm),
+                       "key_weights": list(key_weights) if key_weights is not None else [1.0, 1.0, -1.0],
                        "fine_tuning": {"lr": lr, "epochs_per_gen": tepochs, "buffer": buffer},
                        "scenarios": "config preset, gusts <= 4 m/s RMS, warp +-1 deg, 12 s cap"},
             "pretraining": pre, "arms": summary, "network_gain": gain, "_pretrained_state": (pre, p0)}
@@ -347,7 +350,8 @@ def pool_ab(PA, args):
                       sur_gens=args.sur_gens, pretrain=args.pretrain, pretrain_source=args.pretrain_source,
                       explore=args.explore, buffer=args.buffer, target_norm=args.target_norm,
                       mlp_batch=args.mlp_batch, train_epochs=args.train_epochs, lr=args.lr,
-                      boot_epochs=args.boot_epochs)
+                      boot_epochs=args.boot_epochs, key_weights=args.key_weights,
+                      **({"arms": tuple(args.arms.split(","))} if args.arms else {}))
     ab.pop("_pretrained_state")
     sur = ab["arms"]["surrogate"]
     line = {"metric": METRIC, "value": sur["gens_per_hour"], "unit": "GA generations/hour",
@@ -712,6 +716,11 @@ def main():
     ap.add_argument("--lr", type=float, default=None,
                     help="c4: surrogate learning rate (Adam; default 1e-4 fine-tuning after pretraining, else 2e-3)")
     ap.add_argument("--reps", type=int, default=5, help="c4: GA seeds per arm")
+    ap.add_argument("--arms", default=None,
+                    help="c4: comma-separated arms to run (plain,surrogate,strict,uniform,parent; default all)")
+    ap.add_argument("--key-weights", type=lambda t: tuple(float(x) for x in t.split(",")), default=None,
+                    help="c4: surrogate arm's key weights a,b,c: a * parent score + b * prediction + "
+                         "c * parent's prediction (default 1,1,-1)")
     ap.add_argument("--mlp-batch", type=int, default=4096, help="c4: surrogate training batch")
     ap.add_argument("--pretrain-source", choices=["c2", "family"], default="c2",
                     help="surrogate pretraining pairs: config 2 GA runs (the configuration's source) or the family "

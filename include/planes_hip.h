@@ -291,6 +291,11 @@ int pr_pool_set_explore(pr_pool* p, float frac);
  * on those three ((1, 1, -1) until a fit exists); 0, and before the first generation: the
  * predicted score itself.  No reference counterpart (the reference has no surrogate). */
 int pr_pool_set_parent_delta(pr_pool* p, int on);
+/* Weights (a, b, c) of mode 1's key a * parent's simulated score + b * mutant's predicted
+ * score + c * parent's predicted score; (1, 1, -1) by default.  b = -c = lambda < 1 lets the
+ * parent's simulated score choose the parents and the network mostly order each parent's
+ * mutants.  No reference counterpart (the reference has no surrogate). */
+int pr_pool_set_key_weights(pr_pool* p, float w_parent, float w_pred, float w_parent_pred);
 /* Generations run without a model (model = null): 0 (default) fly the whole pool; 1 fly the
  * elite + a uniform random sample of the mutants; 2 fly the elite + the mutants of the
  * best-scoring elite parents (the parent's simulated score as the key; a uniform sample

@@ -205,6 +205,7 @@ struct pr_pool {
     int parent_delta = 1;       // mutants' keys from the parent's simulated score (when known); 2: calibrated
     float cw[3] = {1.0f, 1.0f, -1.0f};  // key weights of (parent score, prediction, parent's prediction)
     int cw_fitted = 0;          // cw comes from a fit (mode 2) rather than the default
+    float kw[3] = {kDeltaWeights[0], kDeltaWeights[1], kDeltaWeights[2]};  // mode 1's key weights
     std::vector<float> escore_h;  // host copy of the elite's scores the last selection used
     int *d_vals = nullptr, *d_vals2 = nullptr, *d_idx = nullptr;
     void* d_sort_tmp = nullptr;
@@ -384,7 +385,7 @@ int pr_pool_generation(pr_pool* p, int gen, const pr_sim_params* prm, pr_mlp* mo
         const float p_explore =
             p->n_pool > p->n_elite ? p->explore * (float)(p->n_fly - p->n_elite) / (float)(p->n_pool - p->n_elite) : 0.f;
         const float* escore = p->has_escore && p->parent_delta ? p->d_escore : nullptr;
-        const float* w = p->parent_delta == 2 && p->cw_fitted ? p->cw : kDeltaWeights;
+        const float* w = p->parent_delta == 2 && p->cw_fitted ? p->cw : p->kw;
         hipLaunchKernelGGL(k_sort_keys, dim3((p->n_pool + 255) / 256), dim3(256), 0, 0, p->d_pred, 5, 4, p->n_pool,
                            p->n_elite, p_explore, p->seed, (uint32_t)gen, escore, w[0], w[1], w[2], p->d_keys,
                            p->d_vals);
@@ -593,6 +594,15 @@ int pr_pool_set_explore(pr_pool* p, float frac) {
 int pr_pool_set_parent_delta(pr_pool* p, int on) {
     if (!p || on < 0 || on > 2) { g_pool_err = "pr_pool_set_parent_delta: null pool or mode outside 0..2"; return -1; }
     p->parent_delta = on;
+    return 0;
+}
+
+int pr_pool_set_key_weights(pr_pool* p, float w_parent, float w_pred, float w_parent_pred) {
+    if (!p || !std::isfinite(w_parent) || !std::isfinite(w_pred) || !std::isfinite(w_parent_pred)) {
+        g_pool_err = "pr_pool_set_key_weights: null pool or non-finite weight";
+        return -1;
+    }
+    p->kw[0] = w_parent; p->kw[1] = w_pred; p->kw[2] = w_parent_pred;
     return 0;
 }
 

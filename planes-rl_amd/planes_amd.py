@@ -136,6 +136,7 @@ def _load():
     lib.pr_pool_set_collect.argtypes = [VP, C.c_int]
     lib.pr_pool_set_explore.argtypes = [VP, C.c_float]
     lib.pr_pool_set_parent_delta.argtypes = [VP, C.c_int]
+    lib.pr_pool_set_key_weights.argtypes = [VP, C.c_float, C.c_float, C.c_float]
     lib.pr_pool_set_selection.argtypes = [VP, C.c_int]
     lib.pr_pool_set_target_norm.argtypes = [VP, C.c_int]
     lib.pr_pool_add_training.argtypes = [VP, VP, F32P, C.c_int]
@@ -880,6 +881,12 @@ class PoolGA:
         predicted score itself (False / 0)."""
         m = 2 if mode == "fitted" else int(bool(mode))
         self._ck(LIB.pr_pool_set_parent_delta(self.h, m), "pr_pool_set_parent_delta")
+
+    def set_key_weights(self, w_parent=1.0, w_pred=1.0, w_parent_pred=-1.0):
+        """Weights of the default key (set_parent_delta(True)): w_parent * parent's simulated
+        score + w_pred * mutant's predicted score + w_parent_pred * parent's predicted score."""
+        self._ck(LIB.pr_pool_set_key_weights(self.h, float(w_parent), float(w_pred), float(w_parent_pred)),
+                 "pr_pool_set_key_weights")
 
     def set_selection(self, mode):
         """Fly-set choice of generations without a surrogate: "all" (the whole pool),

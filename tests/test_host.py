@@ -41,6 +41,10 @@ def test_argument_errors_fail_loudly():
     assert rc != 0 and b"capacities" in PA.LIB.pr_last_error()
     with pytest.raises(PA.PlanesError):
         PA._ck(-1, "probe")
+    # pool entry points check their handle before touching the GPU
+    assert PA.LIB.pr_pool_set_key_weights(None, 1.0, 1.0, -1.0) == -1
+    assert b"pr_pool_set_key_weights" in PA.LIB.pr_pool_last_error()
+    assert PA.LIB.pr_pool_set_parent_delta(None, 1) == -1
 
 
 def test_gene_bounds_match_reference_table():
