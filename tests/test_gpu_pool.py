@@ -237,16 +237,20 @@ def test_surrogate_parent_delta_keys(PA):
     simulated score plus the network's predicted change from parent to mutant: the second
     generation's fly set is the stable-sort order of exactly those keys (parents from the
     mutation hash, the elite's scores from the first generation's flights), and with the
-    option off it is the order of the predicted scores themselves."""
+    option off it is the order of the predicted scores themselves.  With key weights
+    (1, 0.25, -0.25) (pr_pool_set_key_weights; powers of two, so every product in the key is
+    exact) the order is that of parent score + 0.25 x the predicted change."""
     n_elite, n_pool, n_fly, k, seed, explore = 32, 8192, 1024, 4, 9, 0.25
     params = PA.SimParams.config(**PARAMS)
     g0, _, _ = PA.family_population(n_elite, seed=4)
     S = PA.Surrogate(max_batch=1024, seed=5)
-    for delta in (True, False):
+    for delta in (True, False, "weighted"):
         ga = PA.PoolGA(n_elite, n_pool, n_fly, k, buf_cap=4 * n_fly, sigma_frac=0.05, seed=seed)
         ga.set_elite(g0)
         ga.set_explore(explore)
-        ga.set_parent_delta(delta)
+        ga.set_parent_delta(bool(delta))
+        if delta == "weighted":
+            ga.set_key_weights(1.0, 0.25, -0.25)
         gen = 3
         ga.generation(gen, params, S, train_epochs=0)
         last = ga.last()
@@ -263,7 +267,10 @@ def test_surrogate_parent_delta_keys(PA):
         par = (hash3(seed, gen + 1, i) % n_elite).astype(np.int64)
         p_explore = np.float32(explore) * np.float32(n_fly - n_elite) / np.float32(n_pool - n_elite)
         is_exp = (i >= n_elite) & (unit24(hash3(seed ^ 0x2545F491, gen + 1, i)) < p_explore)
-        s = escore[par] + (pred - pred[par]) if delta else pred
+        if delta == "weighted":
+            s = escore[par] + np.float32(0.25) * (pred - pred[par])
+        else:
+            s = escore[par] + (pred - pred[par]) if delta else pred
         keys = np.where(i < n_elite, np.float32(-3.0e38), np.where(is_exp, np.float32(-2.0e38), -s)).astype(np.float32)
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(fly, pool[order[:n_fly]]), delta
